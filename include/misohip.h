@@ -1,0 +1,183 @@
+/*
+ * misohip.h -- C ABI of the MI355X (gfx950) backend for MISO's MHD time step.
+ *
+ * Every entry point is plain C: POD descriptors, raw device pointers, sizes
+ * and scalars; no C++ or torch types cross this boundary.  Each call is
+ * stream-ordered on the context's HIP stream and returns 0 on success or a
+ * negative MISOHIP_E* code (see misohip_strerror).
+ *
+ * Device layout: a 3D field of the grid (i_total, j_total, k_total) is stored
+ * x-fastest, element (i, j, k) at ((k * j_total + j) * i_total + i).  MISO's
+ * host arrays are z-fastest (miso/include/miso/array3d.hpp:54-60);
+ * misohip_copy_to_device / misohip_copy_to_host convert between the two.
+ * Field order inside misohip_fields follows miso/include/miso/mhd_fields.hpp:
+ * ro, vx, vy, vz, bx, by, bz, ei, ph.
+ *
+ * The entry points replace these reference interfaces:
+ *   misohip_create / destroy      ExecContext<Real, backend::CUDA>
+ *                                 (miso/include/miso/mhd_core.hpp:15-33)
+ *   misohip_copy_to_device/host   Fields::copy_from H<->D
+ *                                 (miso/include/miso/mhd_fields.hpp:83-107,148-172)
+ *   misohip_ghost_fill            Integrator::apply_boundary_condition =
+ *                                 bc.apply + HaloExchanger::apply
+ *                                 (miso/include/miso/mhd_integrator_host.hpp:250-255,
+ *                                  miso/include/miso/mhd_halo_exchange.hpp:51-266,
+ *                                  miso/include/miso/boundary_condition.hpp:108-182)
+ *   misohip_rk_stage              Integrator::update_sc4
+ *                                 (miso/include/miso/mhd_integrator_host.hpp:57-247,
+ *                                  miso/include/miso/mhd_integrator_cuda.cuh:63-289,340-392)
+ *   misohip_char_vel              ArtificialViscosity::characteristic_velocity_eval
+ *                                 (miso/include/miso/mhd_artificial_viscosity_host.hpp:50-68)
+ *   misohip_av_sweep              ArtificialViscosity::update(direction)
+ *                                 (miso/include/miso/mhd_artificial_viscosity_host.hpp:70-348)
+ *   misohip_cfl                   Integrator::cfl
+ *                                 (miso/include/miso/mhd_integrator_host.hpp:303-334,
+ *                                  miso/include/miso/mhd_integrator_cuda.cuh:449-478)
+ *   misohip_update                Integrator::update
+ *                                 (miso/include/miso/mhd_integrator_host.hpp:337-350)
+ *   misohip_step                  ModelBase::update = cfl + update
+ *                                 (miso/include/miso/mhd_model_base.hpp:57-62)
+ */
+#ifndef MISOHIP_H
+#define MISOHIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define MISOHIP_OK 0
+#define MISOHIP_E_HIP (-1)      /* a HIP runtime call failed            */
+#define MISOHIP_E_ARG (-2)      /* invalid argument / shape             */
+#define MISOHIP_E_DTYPE (-3)    /* unknown dtype                        */
+#define MISOHIP_E_NOMEM (-4)    /* device allocation failed             */
+#define MISOHIP_E_RANGE (-5)    /* size beyond a documented limit       */
+
+typedef enum misohip_dtype { MISOHIP_F64 = 0, MISOHIP_F32 = 1 } misohip_dtype;
+
+typedef struct misohip_ctx misohip_ctx;
+
+/* Shape and metric of one rank's grid (miso/include/miso/grid.hpp:103-141).
+ * is/js/ks are 1 on an axis with more than one cell and 0 otherwise; the
+ * metric arrays are device arrays of the field dtype, ghosts included. */
+typedef struct misohip_grid {
+  int i_size, j_size, k_size;
+  int i_total, j_total, k_total;
+  int i_margin, j_margin, k_margin;
+  int is, js, ks;
+  double min_dxyz; /* already float-narrowed, grid.hpp:241 */
+  const void *dx, *dy, *dz;
+  const void *dxi, *dyi, *dzi;
+} misohip_grid;
+
+/* The nine primitive fields, device pointers, x-fastest. */
+typedef struct misohip_fields {
+  void *q[9];
+} misohip_fields;
+
+/* Boundary description.  A periodic axis is filled by the one-rank halo
+ * self-exchange; a non-periodic axis gets the mirror boundary of
+ * boundary_condition.hpp:108-136 on both faces with sign sym_sign[axis][f]. */
+typedef struct misohip_bc {
+  int periodic[3];
+  int sym_sign[3][9];
+} misohip_bc;
+
+/* Config values the step consumes (config_defaults.hpp, eos.hpp) and the
+ * source term: src_kind 0 = none (EmptySourceTerm), 1 = constant gravity
+ * -grav*ro along grav_axis (0/1/2). */
+typedef struct misohip_params {
+  double gm;
+  double cfl_number;
+  double ep, fh, cs_fac, ca_fac, vv_fac;
+  int src_kind;
+  int grav_axis;
+  double grav;
+} misohip_params;
+
+/* Workspace owned by the caller: the Integrator's qq_argm / qq_rslt and the
+ * frozen AV speed cc (mhd_integrator_host.hpp:13-52). */
+typedef struct misohip_work {
+  misohip_fields argm;
+  misohip_fields rslt;
+  void *cc;
+} misohip_work;
+
+/* Context: device ordinal and the HIP stream every call is ordered on
+ * (NULL = the null stream).  Creates no stream of its own. */
+int misohip_create(int device, void *stream, misohip_ctx **out);
+int misohip_destroy(misohip_ctx *ctx);
+int misohip_set_stream(misohip_ctx *ctx, void *stream);
+int misohip_synchronize(misohip_ctx *ctx);
+const char *misohip_strerror(int code);
+
+/* MISO-order (z-fastest) host array <-> x-fastest device field.  `host`
+ * holds i_total*j_total*k_total elements; synchronous. */
+int misohip_copy_to_device(misohip_ctx *ctx, const misohip_grid *g,
+                           misohip_dtype dt, const void *host, void *dev);
+int misohip_copy_to_host(misohip_ctx *ctx, const misohip_grid *g,
+                         misohip_dtype dt, const void *dev, void *host);
+
+/* Ghost fill of all nine fields. */
+int misohip_ghost_fill(misohip_ctx *ctx, const misohip_grid *g,
+                       misohip_dtype dt, const misohip_fields *qq,
+                       const misohip_bc *bc);
+
+/* One RK stage: rslt (interior) from orgn (pointwise) and argm (radius-2
+ * stencils).  dt_s is the stage step (dt/4, dt/3, dt/2, dt), ch2 = ch_divb^2
+ * and damp = exp(-dt_s / tau_divb), both computed by the host exactly as
+ * mhd_integrator_host.hpp:272,337-341 do. */
+int misohip_rk_stage(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                     const misohip_params *p, const misohip_fields *orgn,
+                     const misohip_fields *argm, const misohip_fields *rslt,
+                     double dt_s, double ch2, double damp);
+
+/* Characteristic AV speed cc over every cell (ghosts included). */
+int misohip_char_vel(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                     const misohip_params *p, const misohip_fields *qq,
+                     void *cc);
+
+/* One directional AV sweep (dir 0/1/2): rslt (interior) from qq and the
+ * frozen cc. */
+int misohip_av_sweep(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                     const misohip_params *p, int dir, double dt_step,
+                     const misohip_fields *qq, const misohip_fields *rslt,
+                     const void *cc);
+
+/* CFL time step over the interior; synchronises and writes *dt_out. */
+int misohip_cfl(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                const misohip_params *p, const misohip_fields *qq,
+                double *dt_out);
+
+/* Integrator::update(dt): four RK stages and three AV sweeps, each followed
+ * by the ghost fill; qq is advanced in place. */
+int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                   const misohip_params *p, const misohip_bc *bc,
+                   const misohip_fields *qq, const misohip_work *w,
+                   double dt_step);
+
+/* ModelBase::update: dt = cfl(qq), then update(dt).  *dt_out receives dt. */
+int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                 const misohip_params *p, const misohip_bc *bc,
+                 const misohip_fields *qq, const misohip_work *w,
+                 double *dt_out);
+
+/* Kernel timing with HIP events recorded on the context's stream around every
+ * launch the library makes (no counterpart in the reference, which has no
+ * timers).  Kinds: see MISOHIP_KIND_*.  misohip_profile_read synchronises,
+ * writes the summed milliseconds and launch counts of the first `nkinds`
+ * kinds and clears the record. */
+#define MISOHIP_KIND_RK_STAGE 0
+#define MISOHIP_KIND_AV_X 1
+#define MISOHIP_KIND_AV_Y 2
+#define MISOHIP_KIND_AV_Z 3
+#define MISOHIP_KIND_GHOST 4
+#define MISOHIP_KIND_CFL 5
+#define MISOHIP_KIND_COUNT 6
+int misohip_profile_enable(misohip_ctx *ctx, int on);
+int misohip_profile_read(misohip_ctx *ctx, double *ms, int *launches, int nkinds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MISOHIP_H */

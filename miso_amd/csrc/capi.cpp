@@ -1,0 +1,456 @@
+// C ABI of the MI355X MHD backend (include/misohip.h).
+//
+// Host-side orchestration of MISO's Integrator (mhd_integrator_host.hpp:
+// 258-350): the scalar set-up (divb_parameters_set, the stage steps dt/4,
+// dt/3, dt/2, dt and exp(-dt_s/tau)) is done here in the Real type of the
+// run with the same expressions as the reference, so the device receives the
+// very same scalars the reference's kernels would.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/misohip.h"
+#include "launch.hpp"
+
+
+using namespace misohip;
+
+struct misohip_ctx {
+  int device;
+  hipStream_t stream;
+  unsigned *d_slot;  // CFL reduction slot (device)
+  unsigned *h_slot;  // pinned host copy
+  // timing: pairs of events recorded around launches, tagged by kind
+  bool profiling = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> rec_kind;
+  std::vector<hipEvent_t> rec_start, rec_stop;
+};
+
+namespace {
+// RAII scope: records a start event on construction and a stop event on
+// destruction when profiling is on.
+struct Timed {
+  misohip_ctx *c;
+  int kind;
+  hipEvent_t a = nullptr, b = nullptr;
+  static hipEvent_t get(misohip_ctx *c) {
+    if (!c->pool.empty()) {
+      hipEvent_t e = c->pool.back();
+      c->pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  Timed(misohip_ctx *c_, int k) : c(c_), kind(k) {
+    if (!c->profiling) return;
+    a = get(c);
+    b = get(c);
+    if (a) (void)hipEventRecord(a, c->stream);
+  }
+  ~Timed() {
+    if (!a || !b) return;
+    (void)hipEventRecord(b, c->stream);
+    c->rec_kind.push_back(kind);
+    c->rec_start.push_back(a);
+    c->rec_stop.push_back(b);
+  }
+};
+}  // namespace
+
+namespace {
+
+thread_local char g_last_hip_error[256];
+
+int hip_fail(hipError_t e) {
+  std::snprintf(g_last_hip_error, sizeof(g_last_hip_error), "%s", hipGetErrorString(e));
+  return MISOHIP_E_HIP;
+}
+
+#define MISOHIP_TRY(expr)                   \
+  do {                                      \
+    hipError_t e_ = (expr);                 \
+    if (e_ != hipSuccess) return hip_fail(e_); \
+  } while (0)
+
+int check_grid(const misohip_grid *g) {
+  if (!g) return MISOHIP_E_ARG;
+  const int sz[3] = {g->i_size, g->j_size, g->k_size};
+  const int tot[3] = {g->i_total, g->j_total, g->k_total};
+  const int mar[3] = {g->i_margin, g->j_margin, g->k_margin};
+  const int st[3] = {g->is, g->js, g->ks};
+  for (int a = 0; a < 3; ++a) {
+    if (sz[a] <= 0 || mar[a] < 0) return MISOHIP_E_ARG;
+    if (st[a] != (sz[a] > 1 ? 1 : 0)) return MISOHIP_E_ARG;
+    if (sz[a] == 1 && mar[a] != 0) return MISOHIP_E_ARG;
+    if (sz[a] > 1 && mar[a] < 2) return MISOHIP_E_ARG;  // radius-2 stencils
+    if (tot[a] != sz[a] + 2 * mar[a]) return MISOHIP_E_ARG;
+  }
+  const double n = (double)g->i_total * g->j_total * g->k_total;
+  if (n >= 2147483647.0) return MISOHIP_E_RANGE;  // int indexing inside kernels
+  if (!g->dx || !g->dy || !g->dz || !g->dxi || !g->dyi || !g->dzi) return MISOHIP_E_ARG;
+  return MISOHIP_OK;
+}
+
+int check_fields(const misohip_fields *f) {
+  if (!f) return MISOHIP_E_ARG;
+  for (int m = 0; m < 9; ++m)
+    if (!f->q[m]) return MISOHIP_E_ARG;
+  return MISOHIP_OK;
+}
+
+Shape make_shape(const misohip_grid *g) {
+  Shape s;
+  s.It = g->i_total; s.Jt = g->j_total; s.Kt = g->k_total;
+  s.mi = g->i_margin; s.mj = g->j_margin; s.mk = g->k_margin;
+  s.is = g->is; s.js = g->js; s.ks = g->ks;
+  return s;
+}
+
+template <typename R> Fld<R> fld(const misohip_fields *f) {
+  Fld<R> r;
+  for (int m = 0; m < 9; ++m) r.q[m] = static_cast<R *>(f->q[m]);
+  return r;
+}
+template <typename R> Fld<const R> cfld(const misohip_fields *f) {
+  Fld<const R> r;
+  for (int m = 0; m < 9; ++m) r.q[m] = static_cast<const R *>(f->q[m]);
+  return r;
+}
+
+BcDev make_bc(const misohip_bc *bc) {
+  BcDev b;
+  for (int a = 0; a < 3; ++a) {
+    b.periodic[a] = bc->periodic[a] ? 1 : 0;
+    for (int f = 0; f < 9; ++f) b.sign[a][f] = bc->sym_sign[a][f] >= 0 ? 1.0f : -1.0f;
+  }
+  return b;
+}
+
+template <typename R>
+int rk_stage_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
+               const misohip_fields *o, const misohip_fields *a, const misohip_fields *r,
+               R dt_s, R ch2, R damp) {
+  Timed t_(ctx, MISOHIP_KIND_RK_STAGE);
+  RkScalars sc;
+  sc.dt = dt_s; sc.ch2 = ch2; sc.damp = damp; sc.gm = (R)p->gm;
+  sc.grav = (R)p->grav; sc.src_kind = p->src_kind; sc.grav_axis = p->grav_axis;
+  MISOHIP_TRY(launch_rk_stage<R>(ctx->stream, cfld<R>(o), cfld<R>(a), fld<R>(r),
+                                 static_cast<const R *>(g->dxi), static_cast<const R *>(g->dyi),
+                                 static_cast<const R *>(g->dzi), make_shape(g), sc));
+  return MISOHIP_OK;
+}
+
+template <typename R> AvScalars av_scalars(const misohip_params *p, R dt) {
+  AvScalars s;
+  s.dt = dt; s.ep = (R)p->ep; s.fh = (R)p->fh; s.gm = (R)p->gm;
+  s.cs_fac = (R)p->cs_fac; s.ca_fac = (R)p->ca_fac; s.vv_fac = (R)p->vv_fac;
+  return s;
+}
+
+template <typename R>
+int av_sweep_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p, int dir, R dt,
+               const misohip_fields *q, const misohip_fields *r, void *cc) {
+  const R *dd = static_cast<const R *>(dir == 0 ? g->dxi : dir == 1 ? g->dyi : g->dzi);
+  Timed t_(ctx, MISOHIP_KIND_AV_X + dir);
+  MISOHIP_TRY(launch_av_sweep<R>(ctx->stream, dir, cfld<R>(q), fld<R>(r), static_cast<R *>(cc),
+                                 static_cast<const R *>(cc), dd, make_shape(g),
+                                 av_scalars<R>(p, dt)));
+  return MISOHIP_OK;
+}
+
+template <typename R>
+int cfl_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
+          const misohip_fields *q, double *dt_out) {
+  {
+    Timed t_(ctx, MISOHIP_KIND_CFL);
+    MISOHIP_TRY(launch_cfl<R>(ctx->stream, cfld<R>(q), static_cast<const R *>(g->dx),
+                            static_cast<const R *>(g->dy), static_cast<const R *>(g->dz),
+                            make_shape(g), (double)(R)p->gm, (double)(R)p->cfl_number,
+                            ctx->d_slot));
+  }
+  MISOHIP_TRY(hipMemcpyAsync(ctx->h_slot, ctx->d_slot, sizeof(unsigned), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+  float f;
+  std::memcpy(&f, ctx->h_slot, sizeof(float));
+  *dt_out = (double)(R)f;
+  return MISOHIP_OK;
+}
+
+template <typename R>
+int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const BcDev &b) {
+  Timed t_(ctx, MISOHIP_KIND_GHOST);
+  MISOHIP_TRY(launch_ghost_fill<R>(ctx->stream, fld<R>(q), sh, b));
+  return MISOHIP_OK;
+}
+
+// Integrator::update (mhd_integrator_host.hpp:258-300,337-350) with the
+// workspace ping-pong: every stage writes rslt, which becomes the next argm.
+template <typename R>
+int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
+             const misohip_bc *bc, const misohip_fields *qq, const misohip_work *w, R dt) {
+  const R cfl_number = (R)p->cfl_number;
+  const R min_dxyz = (R)g->min_dxyz;
+  // divb_parameters_set (337-341)
+  const R ch_divb = 0.8 * cfl_number * min_dxyz / dt;
+  const R ch2 = ch_divb * ch_divb;
+  const R tau = 2.0 * dt;
+  const R dts[4] = {(R)(dt / 4.0), (R)(dt / 3.0), (R)(dt / 2.0), dt};
+  const BcDev b = make_bc(bc);
+  const Shape sh = make_shape(g);
+  // stage 1 reads qq as both orgn and argm; later stages ping-pong between
+  // the two workspace field sets.
+  const misohip_fields *argm = qq;
+  const misohip_fields *bufs[2] = {&w->rslt, &w->argm};
+  for (int s = 0; s < 4; ++s) {
+    const misohip_fields *out = s == 3 ? &w->rslt : bufs[s & 1];
+    if (s == 3 && argm == &w->rslt) out = &w->argm;
+    const R damp = (R)std::exp(-dts[s] / tau);
+    int rc = rk_stage_t<R>(ctx, g, p, qq, argm, out, dts[s], ch2, damp);
+    if (rc) return rc;
+    rc = ghost_t<R>(ctx, out, sh, b);
+    if (rc) return rc;
+    argm = out;
+  }
+  // AV: x sweep reads the post-RK state and stores cc; y and z read cc.
+  const misohip_fields *cur = argm;
+  const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
+  for (int dir = 0; dir < 3; ++dir) {
+    const misohip_fields *dst = dir == 2 ? qq : other;
+    int rc = av_sweep_t<R>(ctx, g, p, dir, dt, cur, dst, w->cc);
+    if (rc) return rc;
+    rc = ghost_t<R>(ctx, dst, sh, b);
+    if (rc) return rc;
+    other = cur;
+    cur = dst;
+  }
+  return MISOHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *misohip_strerror(int code) {
+  switch (code) {
+    case MISOHIP_OK: return "success";
+    case MISOHIP_E_HIP: return g_last_hip_error[0] ? g_last_hip_error : "HIP runtime error";
+    case MISOHIP_E_ARG: return "invalid argument or grid shape";
+    case MISOHIP_E_DTYPE: return "unknown dtype";
+    case MISOHIP_E_NOMEM: return "device allocation failed";
+    case MISOHIP_E_RANGE: return "size beyond the supported maximum";
+    default: return "unknown error";
+  }
+}
+
+int misohip_create(int device, void *stream, misohip_ctx **out) {
+  if (!out) return MISOHIP_E_ARG;
+  *out = nullptr;
+  MISOHIP_TRY(hipSetDevice(device));
+  misohip_ctx *c = new (std::nothrow) misohip_ctx;
+  if (!c) return MISOHIP_E_NOMEM;
+  c->device = device;
+  c->stream = static_cast<hipStream_t>(stream);
+  c->d_slot = nullptr;
+  c->h_slot = nullptr;
+  hipError_t e = hipMalloc(&c->d_slot, 256);
+  if (e == hipSuccess) e = hipHostMalloc(&c->h_slot, 256, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    if (c->d_slot) (void)hipFree(c->d_slot);
+    delete c;
+    return hip_fail(e);
+  }
+  *out = c;
+  return MISOHIP_OK;
+}
+
+int misohip_profile_enable(misohip_ctx *ctx, int on) {
+  if (!ctx) return MISOHIP_E_ARG;
+  ctx->profiling = on != 0;
+  return MISOHIP_OK;
+}
+
+int misohip_profile_read(misohip_ctx *ctx, double *ms, int *launches, int nkinds) {
+  if (!ctx || nkinds < 0 || (nkinds > 0 && (!ms || !launches))) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < nkinds; ++k) {
+    ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (size_t r = 0; r < ctx->rec_kind.size(); ++r) {
+    float t = 0.f;
+    MISOHIP_TRY(hipEventElapsedTime(&t, ctx->rec_start[r], ctx->rec_stop[r]));
+    const int k = ctx->rec_kind[r];
+    if (k < nkinds) {
+      ms[k] += t;
+      launches[k] += 1;
+    }
+    ctx->pool.push_back(ctx->rec_start[r]);
+    ctx->pool.push_back(ctx->rec_stop[r]);
+  }
+  ctx->rec_kind.clear();
+  ctx->rec_start.clear();
+  ctx->rec_stop.clear();
+  return MISOHIP_OK;
+}
+
+int misohip_destroy(misohip_ctx *ctx) {
+  if (!ctx) return MISOHIP_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->rec_start) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->rec_stop) (void)hipEventDestroy(e);
+  if (ctx->d_slot) (void)hipFree(ctx->d_slot);
+  if (ctx->h_slot) (void)hipHostFree(ctx->h_slot);
+  delete ctx;
+  return MISOHIP_OK;
+}
+
+int misohip_set_stream(misohip_ctx *ctx, void *stream) {
+  if (!ctx) return MISOHIP_E_ARG;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return MISOHIP_OK;
+}
+
+int misohip_synchronize(misohip_ctx *ctx) {
+  if (!ctx) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MISOHIP_OK;
+}
+
+int misohip_copy_to_device(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                           const void *host, void *dev) {
+  if (!ctx || !host || !dev) return MISOHIP_E_ARG;
+  if (int rc = check_grid(g)) return rc;
+  if (dt != MISOHIP_F64 && dt != MISOHIP_F32) return MISOHIP_E_DTYPE;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  const size_t es = dt == MISOHIP_F64 ? 8 : 4;
+  const size_t n = (size_t)g->i_total * g->j_total * g->k_total;
+  void *tmp = nullptr;
+  MISOHIP_TRY(hipMalloc(&tmp, n * es));
+  hipError_t e = hipMemcpyAsync(tmp, host, n * es, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess)
+    e = dt == MISOHIP_F64
+            ? launch_transpose<double>(ctx->stream, (const double *)tmp, (double *)dev,
+                                       g->i_total, g->j_total, g->k_total, true)
+            : launch_transpose<float>(ctx->stream, (const float *)tmp, (float *)dev, g->i_total,
+                                      g->j_total, g->k_total, true);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e);
+  return MISOHIP_OK;
+}
+
+int misohip_copy_to_host(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                         const void *dev, void *host) {
+  if (!ctx || !host || !dev) return MISOHIP_E_ARG;
+  if (int rc = check_grid(g)) return rc;
+  if (dt != MISOHIP_F64 && dt != MISOHIP_F32) return MISOHIP_E_DTYPE;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  const size_t es = dt == MISOHIP_F64 ? 8 : 4;
+  const size_t n = (size_t)g->i_total * g->j_total * g->k_total;
+  void *tmp = nullptr;
+  MISOHIP_TRY(hipMalloc(&tmp, n * es));
+  hipError_t e = dt == MISOHIP_F64
+                     ? launch_transpose<double>(ctx->stream, (const double *)dev, (double *)tmp,
+                                                g->i_total, g->j_total, g->k_total, false)
+                     : launch_transpose<float>(ctx->stream, (const float *)dev, (float *)tmp,
+                                               g->i_total, g->j_total, g->k_total, false);
+  if (e == hipSuccess) e = hipMemcpyAsync(host, tmp, n * es, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return hip_fail(e);
+  return MISOHIP_OK;
+}
+
+#define MISOHIP_COMMON_CHECKS(ctx, g, dt)                            \
+  if (!ctx) return MISOHIP_E_ARG;                                    \
+  if (int rc_ = check_grid(g)) return rc_;                           \
+  if (dt != MISOHIP_F64 && dt != MISOHIP_F32) return MISOHIP_E_DTYPE; \
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+
+int misohip_ghost_fill(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                       const misohip_fields *qq, const misohip_bc *bc) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (check_fields(qq) || !bc) return MISOHIP_E_ARG;
+  const BcDev b = make_bc(bc);
+  if (dt == MISOHIP_F64) return ghost_t<double>(ctx, qq, make_shape(g), b);
+  return ghost_t<float>(ctx, qq, make_shape(g), b);
+}
+
+int misohip_rk_stage(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                     const misohip_params *p, const misohip_fields *orgn,
+                     const misohip_fields *argm, const misohip_fields *rslt, double dt_s,
+                     double ch2, double damp) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || check_fields(orgn) || check_fields(argm) || check_fields(rslt)) return MISOHIP_E_ARG;
+  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (dt == MISOHIP_F64) return rk_stage_t<double>(ctx, g, p, orgn, argm, rslt, dt_s, ch2, damp);
+  return rk_stage_t<float>(ctx, g, p, orgn, argm, rslt, (float)dt_s, (float)ch2, (float)damp);
+}
+
+int misohip_char_vel(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                     const misohip_params *p, const misohip_fields *qq, void *cc) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || check_fields(qq) || !cc) return MISOHIP_E_ARG;
+  const int n = g->i_total * g->j_total * g->k_total;
+  if (dt == MISOHIP_F64)
+    MISOHIP_TRY(launch_char_vel<double>(ctx->stream, cfld<double>(qq), (double *)cc, n,
+                                        av_scalars<double>(p, 0.0)));
+  else
+    MISOHIP_TRY(launch_char_vel<float>(ctx->stream, cfld<float>(qq), (float *)cc, n,
+                                       av_scalars<float>(p, 0.0f)));
+  return MISOHIP_OK;
+}
+
+int misohip_av_sweep(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                     const misohip_params *p, int dir, double dt_step, const misohip_fields *qq,
+                     const misohip_fields *rslt, const void *cc) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || check_fields(qq) || check_fields(rslt) || !cc || dir < 0 || dir > 2)
+    return MISOHIP_E_ARG;
+  if (dt == MISOHIP_F64) return av_sweep_t<double>(ctx, g, p, dir, dt_step, qq, rslt, (void *)cc);
+  return av_sweep_t<float>(ctx, g, p, dir, (float)dt_step, qq, rslt, (void *)cc);
+}
+
+int misohip_cfl(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                const misohip_params *p, const misohip_fields *qq, double *dt_out) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || check_fields(qq) || !dt_out) return MISOHIP_E_ARG;
+  if (dt == MISOHIP_F64) return cfl_t<double>(ctx, g, p, qq, dt_out);
+  return cfl_t<float>(ctx, g, p, qq, dt_out);
+}
+
+int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                   const misohip_params *p, const misohip_bc *bc, const misohip_fields *qq,
+                   const misohip_work *w, double dt_step) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || !bc || !w || check_fields(qq) || check_fields(&w->argm) || check_fields(&w->rslt) ||
+      !w->cc)
+    return MISOHIP_E_ARG;
+  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
+  if (dt == MISOHIP_F64) return update_t<double>(ctx, g, p, bc, qq, w, dt_step);
+  return update_t<float>(ctx, g, p, bc, qq, w, (float)dt_step);
+}
+
+int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                 const misohip_params *p, const misohip_bc *bc, const misohip_fields *qq,
+                 const misohip_work *w, double *dt_out) {
+  double d = 0.0;
+  int rc = misohip_cfl(ctx, g, dt, p, qq, &d);
+  if (rc) return rc;
+  if (dt_out) *dt_out = d;
+  return misohip_update(ctx, g, dt, p, bc, qq, w, d);
+}
+
+}  // extern "C"

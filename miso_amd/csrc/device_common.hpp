@@ -1,0 +1,83 @@
+// Device-side building blocks shared by the MHD kernels (gfx950).
+//
+// Arithmetic helpers reproduce MISO's host semantics bit for bit in FP64:
+// min/max narrow their operands to float (miso/include/miso/utility.hpp:91-124),
+// constants are rounded to Real once (miso/include/miso/constants.hpp:7-16).
+// The library is compiled with -ffp-contract=off so no multiply-add is fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace misohip {
+
+template <typename R> struct Const {
+  static constexpr R inv12 = static_cast<R>(1.0 / 12.0);
+  static constexpr R pii4 = static_cast<R>(1.0 / (4.0 * 3.14159265358979323846));
+  static constexpr R pii8 = static_cast<R>(1.0 / (8.0 * 3.14159265358979323846));
+};
+
+// float-narrowing min/max, as the reference's util::min2/max2/min3/max3
+template <typename R> __device__ __forceinline__ R nmin2(R a, R b) {
+  return static_cast<R>(fminf(static_cast<float>(a), static_cast<float>(b)));
+}
+template <typename R> __device__ __forceinline__ R nmax2(R a, R b) {
+  return static_cast<R>(fmaxf(static_cast<float>(a), static_cast<float>(b)));
+}
+template <typename R> __device__ __forceinline__ R nmin3(R a, R b, R c) {
+  return static_cast<R>(
+      fminf(fminf(static_cast<float>(a), static_cast<float>(b)), static_cast<float>(c)));
+}
+template <typename R> __device__ __forceinline__ R nmax3(R a, R b, R c) {
+  return static_cast<R>(
+      fmaxf(fmaxf(static_cast<float>(a), static_cast<float>(b)), static_cast<float>(c)));
+}
+
+__device__ __forceinline__ double dsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float dsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double dfabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float dfabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double dcopysign(double x, double y) { return __builtin_copysign(x, y); }
+__device__ __forceinline__ float dcopysign(float x, float y) { return __builtin_copysignf(x, y); }
+
+// Field set: nine device arrays in MISO's order ro vx vy vz bx by bz ei ph.
+template <typename T> struct Fld {
+  T *q[9];
+};
+
+// Shape of one rank's grid; device arrays are x-fastest:
+// (i, j, k) -> (k * Jt + j) * It + i.
+struct Shape {
+  int It, Jt, Kt;
+  int mi, mj, mk;
+  int is, js, ks;
+  __host__ __device__ int idx(int i, int j, int k) const { return (k * Jt + j) * It + i; }
+  __host__ __device__ int stride(int d) const {
+    return d == 0 ? is : d == 1 ? js * It : ks * It * Jt;
+  }
+};
+
+// Generalised minmod slope, limiter.hpp:10-22.
+template <typename R>
+__device__ __forceinline__ R dqq_eval(R qq_dw, R qq_cn, R qq_up, R ep) {
+  const R dqq_dw = qq_cn - qq_dw;
+  const R dqq_up = qq_up - qq_cn;
+  const R dqq_cn = (qq_up - qq_dw) * R(0.5);
+  const R dqq_max = nmax3(ep * dqq_dw, ep * dqq_up, dqq_cn);
+  const R dqq_min = nmin3(ep * dqq_dw, ep * dqq_up, dqq_cn);
+  return nmin2(R(0.0), dqq_max) + nmax2(R(0.0), dqq_min);
+}
+
+// Limited interface flux, limiter.hpp:24-37.
+template <typename R>
+__device__ __forceinline__ R flux_core(R qq_dw, R qq_up, R dqq_dw, R dqq_up, R cc, R fh) {
+  const R qql = qq_dw + R(0.5) * dqq_dw;
+  const R qqr = qq_up - R(0.5) * dqq_up;
+  R dqq = qq_up - qq_dw;
+  dqq = dcopysign(nmax2(dfabs(dqq), R(1e-20)), dqq);
+  const R ra = nmin2(R(1.0), (qqr - qql) / dqq);
+  const R pp = (R(0.5) + dcopysign(R(0.5), ra)) * nmax2(R(0.0), R(1.0) + fh * (ra - R(1.0)));
+  return R(-0.5) * cc * pp * (qqr - qql);
+}
+
+}  // namespace misohip

@@ -1,0 +1,225 @@
+// In-device ghost fill, CFL reduction and host<->device layout transposes.
+//
+// Ghost fill replaces Integrator::apply_boundary_condition for one rank on
+// one GPU (miso/include/miso/mhd_integrator_host.hpp:250-255): the user
+// mirror boundary on physical faces (boundary_condition.hpp:77-136) and the
+// MPI halo self-exchange on periodic axes (mhd_halo_exchange.hpp:51-266)
+// become one pass in which every ghost cell reads its source cell directly.
+// Each coordinate is mapped on its own (periodic wrap or mirror), so faces
+// carry exactly the values the reference's exchange produces and edges and
+// corners are well defined (the reference leaves them stale; no interior
+// stencil reads them).
+//
+// CFL: Integrator::cfl (mhd_integrator_host.hpp:303-334) narrows every
+// candidate to float through util::min2, so dt = (float)min(candidates).
+// Candidates are positive, so the IEEE bit pattern of the float orders like
+// an unsigned integer: a wavefront shuffle-min on the bits, an LDS block min,
+// then one unsigned atomicMin per block.
+#include "launch.hpp"
+
+namespace misohip {
+
+__device__ __forceinline__ int map_coord(int x, int total, int m, int periodic, bool &mirrored) {
+  mirrored = false;
+  if (m == 0) return x;
+  const int n = total - 2 * m;
+  if (x < m) {
+    if (periodic) return x + n;
+    mirrored = true;
+    return 2 * m - 1 - x;
+  }
+  if (x >= total - m) {
+    if (periodic) return x - n;
+    mirrored = true;
+    return 2 * (total - m) - 1 - x;
+  }
+  return x;
+}
+
+// Flattened over the shell: x slabs (all j, k), then y slabs (interior i,
+// all k), then z slabs (interior i, j).  The three sets partition the ghosts.
+template <typename R>
+__global__ __launch_bounds__(256) void ghost_fill_kernel(Fld<R> qq, Shape sh, BcDev bc, long n_x,
+                                                         long n_y, long n_tot) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tot) return;
+  int i, j, k;
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
+  if (t < n_x) {
+    // x slabs: 2*mi columns x Jt x Kt, i fastest
+    const int w = 2 * sh.mi;
+    const int a = (int)(t % w);
+    const long r = t / w;
+    j = (int)(r % sh.Jt);
+    k = (int)(r / sh.Jt);
+    i = a < sh.mi ? a : sh.It - 2 * sh.mi + a;
+  } else if (t < n_x + n_y) {
+    const long u = t - n_x;
+    i = sh.mi + (int)(u % ni);
+    const long r = u / ni;
+    const int a = (int)(r % (2 * sh.mj));
+    k = (int)(r / (2 * sh.mj));
+    j = a < sh.mj ? a : sh.Jt - 2 * sh.mj + a;
+  } else {
+    const long u = t - n_x - n_y;
+    i = sh.mi + (int)(u % ni);
+    const long r = u / ni;
+    j = sh.mj + (int)(r % nj);
+    const int a = (int)(r / nj);
+    k = a < sh.mk ? a : sh.Kt - 2 * sh.mk + a;
+  }
+  bool mx, my, mz;
+  const int si = map_coord(i, sh.It, sh.mi, bc.periodic[0], mx);
+  const int sj = map_coord(j, sh.Jt, sh.mj, bc.periodic[1], my);
+  const int sk = map_coord(k, sh.Kt, sh.mk, bc.periodic[2], mz);
+  const int dst = sh.idx(i, j, k), src = sh.idx(si, sj, sk);
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    R v = qq.q[f][src];
+    if (mx) v = R(bc.sign[0][f]) * v;
+    if (my) v = R(bc.sign[1][f]) * v;
+    if (mz) v = R(bc.sign[2][f]) * v;
+    qq.q[f][dst] = v;
+  }
+}
+
+template <typename R>
+hipError_t launch_ghost_fill(hipStream_t st, const Fld<R> &q, const Shape &sh, const BcDev &bc) {
+  const long ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
+  const long n_x = 2L * sh.mi * sh.Jt * sh.Kt;
+  const long n_y = 2L * sh.mj * ni * sh.Kt;
+  const long n_z = 2L * sh.mk * ni * nj;
+  const long n_tot = n_x + n_y + n_z;
+  if (n_tot == 0) return hipSuccess;
+  const long blocks = (n_tot + 255) / 256;
+  hipLaunchKernelGGL((ghost_fill_kernel<R>), dim3((unsigned)blocks), dim3(256), 0, st, q, sh, bc,
+                     n_x, n_y, n_tot);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- CFL
+__global__ void cfl_init_kernel(unsigned *slot) {
+  if (threadIdx.x == 0) slot[0] = __float_as_uint(1.e10f);
+}
+
+template <typename R>
+__device__ __forceinline__ float cfl_candidate(const Fld<const R> &q, int c, R gm, R cfl, R hmin) {
+  const R ro = q.q[0][c], vx = q.q[1][c], vy = q.q[2][c], vz = q.q[3][c];
+  const R bx = q.q[4][c], by = q.q[5][c], bz = q.q[6][c], ei = q.q[7][c];
+  const R cs = dsqrt(gm * (gm - R(1)) * ei);
+  const R vv = dsqrt(+vx * vx + vy * vy + vz * vz);
+  const R ca = dsqrt((+bx * bx + by * by + bz * bz) / ro * Const<R>::pii4);
+  const R total_vel = cs + vv + ca;
+  return static_cast<float>(cfl * hmin / total_vel);
+}
+
+__device__ __forceinline__ unsigned wave_umin(unsigned v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// Grid-stride over the interior; one atomic per block.
+template <typename R>
+__global__ __launch_bounds__(256) void cfl_kernel(Fld<const R> q, const R *__restrict__ dx,
+                                                  const R *__restrict__ dy,
+                                                  const R *__restrict__ dz, Shape sh, double gm,
+                                                  double cfl, unsigned *slot) {
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  const long n = (long)ni * nj * nk;
+  unsigned best = __float_as_uint(1.e10f);
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x) {
+    const int i = sh.mi + (int)(t % ni);
+    const long r = t / ni;
+    const int j = sh.mj + (int)(r % nj);
+    const int k = sh.mk + (int)(r / nj);
+    const R hmin = nmin3(dx[i], dy[j], dz[k]);
+    const float cand = cfl_candidate(q, sh.idx(i, j, k), R(gm), R(cfl), hmin);
+    best = min(best, __float_as_uint(cand));
+  }
+  best = wave_umin(best);
+  __shared__ unsigned red[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = min(b, red[w]);
+    atomicMin(slot, b);
+  }
+}
+
+template <typename R>
+hipError_t launch_cfl(hipStream_t st, const Fld<const R> &q, const R *dx, const R *dy,
+                      const R *dz, const Shape &sh, double gm, double cfl, unsigned *slot) {
+  hipLaunchKernelGGL(cfl_init_kernel, dim3(1), dim3(64), 0, st, slot);
+  const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj) * (sh.Kt - 2 * sh.mk);
+  long blocks = (n + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((cfl_kernel<R>), dim3((unsigned)blocks), dim3(256), 0, st, q, dx, dy, dz, sh,
+                     gm, cfl, slot);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------- transposes
+// MISO order (i, j, k) k-fastest  <->  device order x-fastest.  For each j
+// this is a 2D transpose of an It x Kt matrix, tiled through LDS.
+template <typename R, bool TO_DEVICE>
+__global__ __launch_bounds__(256) void transpose_kernel(const R *__restrict__ src,
+                                                        R *__restrict__ dst, int It, int Jt,
+                                                        int Kt) {
+  __shared__ R tile[32][33];
+  const int j = blockIdx.z;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  if (TO_DEVICE) {
+    // read along k (contiguous in MISO order), write along i
+    const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    for (int r = ty; r < 32; r += 8) {
+      const int i = i0 + r, k = k0 + tx;
+      if (i < It && k < Kt) tile[r][tx] = src[((long)i * Jt + j) * Kt + k];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      const int k = k0 + r, i = i0 + tx;
+      if (i < It && k < Kt) dst[((long)k * Jt + j) * It + i] = tile[tx][r];
+    }
+  } else {
+    const int i0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    for (int r = ty; r < 32; r += 8) {
+      const int k = k0 + r, i = i0 + tx;
+      if (i < It && k < Kt) tile[r][tx] = src[((long)k * Jt + j) * It + i];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+      const int i = i0 + r, k = k0 + tx;
+      if (i < It && k < Kt) dst[((long)i * Jt + j) * Kt + k] = tile[tx][r];
+    }
+  }
+}
+
+template <typename R>
+hipError_t launch_transpose(hipStream_t st, const R *src, R *dst, int It, int Jt, int Kt,
+                            bool to_device) {
+  if (to_device) {
+    dim3 grid((Kt + 31) / 32, (It + 31) / 32, Jt);
+    hipLaunchKernelGGL((transpose_kernel<R, true>), grid, dim3(256), 0, st, src, dst, It, Jt, Kt);
+  } else {
+    dim3 grid((It + 31) / 32, (Kt + 31) / 32, Jt);
+    hipLaunchKernelGGL((transpose_kernel<R, false>), grid, dim3(256), 0, st, src, dst, It, Jt, Kt);
+  }
+  return hipGetLastError();
+}
+
+#define MISOHIP_INST(R)                                                                           \
+  template hipError_t launch_ghost_fill<R>(hipStream_t, const Fld<R> &, const Shape &,           \
+                                           const BcDev &);                                        \
+  template hipError_t launch_cfl<R>(hipStream_t, const Fld<const R> &, const R *, const R *,     \
+                                    const R *, const Shape &, double, double, unsigned *);        \
+  template hipError_t launch_transpose<R>(hipStream_t, const R *, R *, int, int, int, bool);
+MISOHIP_INST(double)
+MISOHIP_INST(float)
+#undef MISOHIP_INST
+
+}  // namespace misohip

@@ -1,0 +1,208 @@
+// One Runge-Kutta stage of MISO's 4th-order central-difference MHD scheme
+// (Integrator::update_sc4, miso/include/miso/mhd_integrator_host.hpp:57-247;
+// the CUDA backend splits it over ten kernels, mhd_integrator_cuda.cuh:16-289).
+//
+// Here one fused kernel computes all nine fields of a cell.  Lanes run along
+// x, the contiguous device axis, so every load of a wave is one coalesced
+// 512-byte (FP64) segment; the y/z neighbours of a row are re-read from L1/L2.
+//
+// Bit parity with the reference CPU path: each stencil evaluates
+//   ((((-q[+2]) + 8 q[+1]) - 8 q[-1]) + q[-2]) * inv12 * dxi
+// on the per-point products (a*b)*c exactly as space_centered_4th does
+// (mhd_integrator.hpp:14-57): since -a and 8a are exact, (-a)*b*c and
+// (8a)*b*c equal -(a*b*c) and 8(a*b*c) bit for bit, so forming the product
+// once per point keeps every rounding of the reference.  The sums of each
+// equation keep the reference's term order.
+#include "launch.hpp"
+
+namespace misohip {
+
+// Quantities differentiated along direction D at one point (17 of them):
+//  0 ro*v_D   1 (ro*vx)*v_D  2 (ro*vy)*v_D  3 (ro*vz)*v_D   4 pr   5 bb
+//  6 bx*b_D   7 by*b_D       8 bz*b_D       9..12 induction products
+// 13 ph      14 b_D         15 ht*v_D      16 vb*b_D
+// Induction products (the a*b operands of the induction terms along D):
+//  D=x: vx*by vy*bx vx*bz vz*bx;  D=y: vy*bx vx*by vy*bz vz*by;
+//  D=z: vz*bx vx*bz vz*by vy*bz.
+template <typename R, int D>
+__device__ __forceinline__ void point_terms(const R f[9], R gm, R o[17]) {
+  const R ro = f[0], vx = f[1], vy = f[2], vz = f[3];
+  const R bx = f[4], by = f[5], bz = f[6], ei = f[7], ph = f[8];
+  const R vd = D == 0 ? vx : (D == 1 ? vy : vz);
+  const R bd = D == 0 ? bx : (D == 1 ? by : bz);
+  const R rvx = ro * vx, rvy = ro * vy, rvz = ro * vz;
+  o[0] = D == 0 ? rvx : (D == 1 ? rvy : rvz);
+  o[1] = rvx * vd;
+  o[2] = rvy * vd;
+  o[3] = rvz * vd;
+  // eos.hpp:30: pr = (gm - 1) * ro * ei
+  const R pr = (gm - R(1)) * ro * ei;
+  o[4] = pr;
+  // mhd_integrator_host.hpp:69-89
+  const R bb = bx * bx + by * by + bz * bz;
+  o[5] = bb;
+  o[6] = bx * bd;
+  o[7] = by * bd;
+  o[8] = bz * bd;
+  if (D == 0) {
+    o[9] = vx * by; o[10] = vy * bx; o[11] = vx * bz; o[12] = vz * bx;
+  } else if (D == 1) {
+    o[9] = vy * bx; o[10] = vx * by; o[11] = vy * bz; o[12] = vz * by;
+  } else {
+    o[9] = vz * bx; o[10] = vx * bz; o[11] = vz * by; o[12] = vy * bz;
+  }
+  o[13] = ph;
+  o[14] = bd;
+  const R ht = ro * ei + pr + bb * Const<R>::pii4 + R(0.5) * ro * (vx * vx + vy * vy + vz * vz);
+  o[15] = ht * vd;
+  const R vb = vx * bx + vy * by + vz * bz;
+  o[16] = vb * bd;
+}
+
+// All 17 derivatives along D at linear index c (stride s, inverse spacing dd).
+template <typename R, int D>
+__device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, R dd, R gm,
+                                           R S[17]) {
+  R f[9], t[17];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + 2 * s];
+  point_terms<R, D>(f, gm, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = -t[n];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + s];
+  point_terms<R, D>(f, gm, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = S[n] + R(8) * t[n];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - s];
+  point_terms<R, D>(f, gm, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = S[n] - R(8) * t[n];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - 2 * s];
+  point_terms<R, D>(f, gm, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
+}
+
+// Energy recovery is carried in double for both storage types ("pressure
+// recovery from total energy" is the cancellation-prone step).
+template <typename R, bool HX, bool HY, bool HZ>
+__global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<const R> argm,
+                                                       Fld<R> rslt, const R *__restrict__ dxi,
+                                                       const R *__restrict__ dyi,
+                                                       const R *__restrict__ dzi, Shape sh,
+                                                       RkScalars sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + sh.mi;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y + sh.mj;
+  const int k = blockIdx.z + sh.mk;
+  if (i >= sh.It - sh.mi || j >= sh.Jt - sh.mj) return;
+  const int c = sh.idx(i, j, k);
+  const R gm = R(sc.gm), dt = R(sc.dt);
+
+  R Sx[17], Sy[17], Sz[17];
+  if (HX) {
+    dir_derivs<R, 0>(argm, c, 1, dxi[i], gm, Sx);
+  } else {
+#pragma unroll
+    for (int n = 0; n < 17; ++n) Sx[n] = R(0);
+  }
+  if (HY) {
+    dir_derivs<R, 1>(argm, c, sh.It, dyi[j], gm, Sy);
+  } else {
+#pragma unroll
+    for (int n = 0; n < 17; ++n) Sy[n] = R(0);
+  }
+  if (HZ) {
+    dir_derivs<R, 2>(argm, c, sh.It * sh.Jt, dzi[k], gm, Sz);
+  } else {
+#pragma unroll
+    for (int n = 0; n < 17; ++n) Sz[n] = R(0);
+  }
+
+  const R o_ro = orgn.q[0][c], o_vx = orgn.q[1][c], o_vy = orgn.q[2][c], o_vz = orgn.q[3][c];
+  const R o_bx = orgn.q[4][c], o_by = orgn.q[5][c], o_bz = orgn.q[6][c];
+  const R o_ei = orgn.q[7][c], o_ph = orgn.q[8][c];
+  const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
+
+  // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134)
+  R src[3] = {R(0), R(0), R(0)};
+  if (sc.src_kind == 1) src[sc.grav_axis] = -R(sc.grav) * argm.q[0][c];
+
+  // continuity
+  const R ro = o_ro + dt * (-Sx[0] - Sy[0] - Sz[0]);
+  // equations of motion
+  const R vx = (o_ro * o_vx + dt * (-Sx[1] - Sy[1] - Sz[1] - Sx[4] - Sx[5] * pii8 +
+                                    pii4 * (+Sx[6] + Sy[6] + Sz[6]) + src[0])) / ro;
+  const R vy = (o_ro * o_vy + dt * (-Sx[2] - Sy[2] - Sz[2] - Sy[4] - Sy[5] * pii8 +
+                                    pii4 * (+Sx[7] + Sy[7] + Sz[7]) + src[1])) / ro;
+  const R vz = (o_ro * o_vz + dt * (-Sx[3] - Sy[3] - Sz[3] - Sz[4] - Sz[5] * pii8 +
+                                    pii4 * (+Sx[8] + Sy[8] + Sz[8]) + src[2])) / ro;
+  // induction
+  const R bx = o_bx + dt * (-Sy[9] - Sz[9] + Sy[10] + Sz[10] - Sx[13]);
+  const R by = o_by + dt * (-Sx[9] - Sz[11] + Sx[10] + Sz[12] - Sy[13]);
+  const R bz = o_bz + dt * (-Sx[11] - Sy[11] + Sx[12] + Sy[12] - Sz[13]);
+  // GLM divergence cleaning; damp = exp(-dt/tau) from the host
+  const R ph = (o_ph + dt * R(sc.ch2) * (-Sx[14] - Sy[14] - Sz[14])) * R(sc.damp);
+  // energy: total energy of orgn, flux divergence, recovery of ei
+  const double eo_ro = o_ro;
+  const double Et = +eo_ro * double(o_ei) +
+                    0.5 * eo_ro * (+double(o_vx) * o_vx + double(o_vy) * o_vy + double(o_vz) * o_vz) +
+                    double(pii8) * (+double(o_bx) * o_bx + double(o_by) * o_by + double(o_bz) * o_bz);
+  const R eflux = dt * (-Sx[15] - Sy[15] - Sz[15] + pii4 * (+Sx[16] + Sy[16] + Sz[16]));
+  const double dro = ro, dvx = vx, dvy = vy, dvz = vz, dbx = bx, dby = by, dbz = bz;
+  const double ei = (Et + double(eflux) - 0.5 * dro * (+dvx * dvx + dvy * dvy + dvz * dvz) -
+                     double(pii8) * (+dbx * dbx + dby * dby + dbz * dbz) + 0.0) /
+                    dro;
+
+  rslt.q[0][c] = ro;
+  rslt.q[1][c] = vx;
+  rslt.q[2][c] = vy;
+  rslt.q[3][c] = vz;
+  rslt.q[4][c] = bx;
+  rslt.q[5][c] = by;
+  rslt.q[6][c] = bz;
+  rslt.q[7][c] = R(ei);
+  rslt.q[8][c] = ph;
+}
+
+template <typename R, bool HX, bool HY, bool HZ>
+static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
+                              const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
+                              const Shape &sh, const RkScalars &sc) {
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  dim3 block(64, 4, 1);
+  dim3 grid((ni + 63) / 64, (nj + 3) / 4, nk);
+  hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ>), grid, block, 0, st, o, a, r, dxi, dyi, dzi,
+                     sh, sc);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_rk_stage(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
+                           const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
+                           const Shape &sh, const RkScalars &sc) {
+  const int key = (sh.is ? 4 : 0) | (sh.js ? 2 : 0) | (sh.ks ? 1 : 0);
+  switch (key) {
+    case 7: return launch_rk_t<R, true, true, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 6: return launch_rk_t<R, true, true, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 5: return launch_rk_t<R, true, false, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 3: return launch_rk_t<R, false, true, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 4: return launch_rk_t<R, true, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 2: return launch_rk_t<R, false, true, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 1: return launch_rk_t<R, false, false, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    default: return launch_rk_t<R, false, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+  }
+}
+
+template hipError_t launch_rk_stage<double>(hipStream_t, const Fld<const double> &,
+                                            const Fld<const double> &, const Fld<double> &,
+                                            const double *, const double *, const double *,
+                                            const Shape &, const RkScalars &);
+template hipError_t launch_rk_stage<float>(hipStream_t, const Fld<const float> &,
+                                           const Fld<const float> &, const Fld<float> &,
+                                           const float *, const float *, const float *,
+                                           const Shape &, const RkScalars &);
+
+}  // namespace misohip

@@ -1,0 +1,112 @@
+"""ctypes binding of the C ABI in include/misohip.h (libmisohip.so).
+
+This is the Python side of the drop-in boundary: plain pointers, sizes and
+POD descriptors go across, exactly what a C/C++ caller would pass.  Device
+memory is plumbing (torch tensors); the product is the HIP library.  A missing
+library is an error -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmisohip.so")
+
+F64, F32 = 0, 1
+
+FIELD_NAMES = ("ro", "vx", "vy", "vz", "bx", "by", "bz", "ei", "ph")
+
+
+class MisohipGrid(C.Structure):
+    _fields_ = [(n, C.c_int) for n in (
+        "i_size", "j_size", "k_size", "i_total", "j_total", "k_total",
+        "i_margin", "j_margin", "k_margin", "is_", "js", "ks")] + [
+        ("min_dxyz", C.c_double),
+        ("dx", C.c_void_p), ("dy", C.c_void_p), ("dz", C.c_void_p),
+        ("dxi", C.c_void_p), ("dyi", C.c_void_p), ("dzi", C.c_void_p)]
+
+
+class MisohipFields(C.Structure):
+    _fields_ = [("q", C.c_void_p * 9)]
+
+
+class MisohipBc(C.Structure):
+    _fields_ = [("periodic", C.c_int * 3), ("sym_sign", (C.c_int * 9) * 3)]
+
+
+class MisohipParams(C.Structure):
+    _fields_ = [("gm", C.c_double), ("cfl_number", C.c_double), ("ep", C.c_double),
+                ("fh", C.c_double), ("cs_fac", C.c_double), ("ca_fac", C.c_double),
+                ("vv_fac", C.c_double), ("src_kind", C.c_int), ("grav_axis", C.c_int),
+                ("grav", C.c_double)]
+
+
+class MisohipWork(C.Structure):
+    _fields_ = [("argm", MisohipFields), ("rslt", MisohipFields), ("cc", C.c_void_p)]
+
+
+# (name, argtypes) of every exported entry point of include/misohip.h
+_P = C.POINTER
+SIGNATURES = {
+    "misohip_create": [C.c_int, C.c_void_p, _P(C.c_void_p)],
+    "misohip_destroy": [C.c_void_p],
+    "misohip_set_stream": [C.c_void_p, C.c_void_p],
+    "misohip_synchronize": [C.c_void_p],
+    "misohip_strerror": [C.c_int],
+    "misohip_copy_to_device": [C.c_void_p, _P(MisohipGrid), C.c_int, C.c_void_p, C.c_void_p],
+    "misohip_copy_to_host": [C.c_void_p, _P(MisohipGrid), C.c_int, C.c_void_p, C.c_void_p],
+    "misohip_ghost_fill": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipFields),
+                           _P(MisohipBc)],
+    "misohip_rk_stage": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams),
+                         _P(MisohipFields), _P(MisohipFields), _P(MisohipFields), C.c_double,
+                         C.c_double, C.c_double],
+    "misohip_char_vel": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams),
+                         _P(MisohipFields), C.c_void_p],
+    "misohip_av_sweep": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), C.c_int,
+                         C.c_double, _P(MisohipFields), _P(MisohipFields), C.c_void_p],
+    "misohip_cfl": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipFields),
+                    _P(C.c_double)],
+    "misohip_update": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
+                       _P(MisohipFields), _P(MisohipWork), C.c_double],
+    "misohip_step": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
+                     _P(MisohipFields), _P(MisohipWork), _P(C.c_double)],
+    "misohip_profile_enable": [C.c_void_p, C.c_int],
+    "misohip_profile_read": [C.c_void_p, _P(C.c_double), _P(C.c_int), C.c_int],
+}
+
+KIND_NAMES = ("rk_stage", "av_x", "av_y", "av_z", "ghost_fill", "cfl")
+
+_lib = None
+
+
+class MisohipError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmisohip.so; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MisohipError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                "g.build()'` (make -C miso_amd/csrc); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, argt in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = C.c_int
+        L.misohip_strerror.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().misohip_strerror(rc).decode()
+        raise MisohipError(f"{what}: {msg} (code {rc})")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)
