@@ -39,7 +39,29 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="time a plain device copy of known size (HBM reference for the "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(nbytes=2 << 30, reps=10):
+    """A plain copy kernel of known size: reads and writes `nbytes` per launch."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+                      "avg_ms": ms, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def cpu_baseline(target_s: float):
@@ -82,6 +104,9 @@ def main():
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import numpy as np
     import torch
 

@@ -43,10 +43,12 @@ __global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> r
                                                        const R *__restrict__ cc_in,
                                                        const R *__restrict__ dd_arr, Shape sh,
                                                        int j_lo, int k_lo, int j_n, int k_n,
-                                                       AvScalars sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + sh.mi;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + j_lo;
-  const int k = blockIdx.z + k_lo;
+                                                       AvScalars sc, int nbx, int nby, int nb) {
+  const int V = xcd_virtual_block(blockIdx.x, nb);
+  const int blk_x = V % nbx, rest = V / nbx;
+  const int i = blk_x * 64 + (threadIdx.x & 63) + sh.mi;
+  const int j = (rest % nby) * 4 + (threadIdx.x >> 6) + j_lo;
+  const int k = rest / nby + k_lo;
   if (i >= sh.It - sh.mi || j >= j_lo + j_n || k >= k_lo + k_n) return;
   const int c = sh.idx(i, j, k);
   const R gm = R(sc.gm), cs_fac = R(sc.cs_fac), ca_fac = R(sc.ca_fac), vv_fac = R(sc.vv_fac);
@@ -141,10 +143,12 @@ static hipError_t launch_av_t(hipStream_t st, const Fld<const R> &q, const Fld<R
     k_lo -= sh.ks; k_n += 2 * sh.ks;
   }
   const int ni = sh.It - 2 * sh.mi;
-  dim3 block(64, 4, 1);
-  dim3 grid((ni + 63) / 64, (j_n + 3) / 4, k_n);
-  hipLaunchKernelGGL((av_sweep_kernel<R, DIR>), grid, block, 0, st, q, r, cc_store, cc_in, dd, sh,
-                     j_lo, k_lo, j_n, k_n, sc);
+  const int nbx = (ni + 63) / 64, nby = (j_n + 3) / 4;
+  const long nbl = (long)nbx * nby * k_n;
+  if (nbl > 2147483647L) return hipErrorInvalidValue;
+  const int nb = (int)nbl;
+  hipLaunchKernelGGL((av_sweep_kernel<R, DIR>), dim3(nb), dim3(256), 0, st, q, r, cc_store, cc_in,
+                     dd, sh, j_lo, k_lo, j_n, k_n, sc, nbx, nby, nb);
   return hipGetLastError();
 }
 

@@ -40,6 +40,15 @@ __device__ __forceinline__ float dfabs(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ double dcopysign(double x, double y) { return __builtin_copysign(x, y); }
 __device__ __forceinline__ float dcopysign(float x, float y) { return __builtin_copysignf(x, y); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (linear block L runs
+// on XCD L % 8, and each XCD takes its blocks in increasing L).  Mapping L to
+// the virtual block V below gives every XCD one contiguous range of V, i.e. a
+// contiguous slab of the grid, so stencil halos are re-read from its own L2.
+__device__ __forceinline__ int xcd_virtual_block(int L, int nb) {
+  const int x = L & 7, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + (L >> 3);
+}
+
 // Field set: nine device arrays in MISO's order ro vx vy vz bx by bz ei ph.
 template <typename T> struct Fld {
   T *q[9];
@@ -71,8 +80,9 @@ __device__ __forceinline__ R dqq_eval(R qq_dw, R qq_cn, R qq_up, R ep) {
 // Limited interface flux, limiter.hpp:24-37.
 template <typename R>
 __device__ __forceinline__ R flux_core(R qq_dw, R qq_up, R dqq_dw, R dqq_up, R cc, R fh) {
-  const R qql = qq_dw + R(0.5) * dqq_dw;
-  const R qqr = qq_up - R(0.5) * dqq_up;
+  // 0.5*d is exact, so q + 0.5*d is one correctly rounded fma
+  const R qql = __builtin_fma(R(0.5), dqq_dw, qq_dw);
+  const R qqr = __builtin_fma(R(-0.5), dqq_up, qq_up);
   R dqq = qq_up - qq_dw;
   dqq = dcopysign(nmax2(dfabs(dqq), R(1e-20)), dqq);
   const R ra = nmin2(R(1.0), (qqr - qql) / dqq);

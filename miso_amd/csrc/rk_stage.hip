@@ -2,17 +2,20 @@
 // (Integrator::update_sc4, miso/include/miso/mhd_integrator_host.hpp:57-247;
 // the CUDA backend splits it over ten kernels, mhd_integrator_cuda.cuh:16-289).
 //
-// Here one fused kernel computes all nine fields of a cell.  Lanes run along
-// x, the contiguous device axis, so every load of a wave is one coalesced
-// 512-byte (FP64) segment; the y/z neighbours of a row are re-read from L1/L2.
+// One fused kernel computes all nine fields of a cell.  Lanes run along x,
+// the contiguous device axis, so every load of a wave is one coalesced
+// segment; the y/z neighbours of a row are re-read from L1/L2, and the
+// XCD-aware block order keeps each XCD on one contiguous z-slab.
 //
 // Bit parity with the reference CPU path: each stencil evaluates
 //   ((((-q[+2]) + 8 q[+1]) - 8 q[-1]) + q[-2]) * inv12 * dxi
 // on the per-point products (a*b)*c exactly as space_centered_4th does
 // (mhd_integrator.hpp:14-57): since -a and 8a are exact, (-a)*b*c and
 // (8a)*b*c equal -(a*b*c) and 8(a*b*c) bit for bit, so forming the product
-// once per point keeps every rounding of the reference.  The sums of each
-// equation keep the reference's term order.
+// once per point keeps every rounding of the reference; for the same reason
+// acc + 8 t is one fma(8, t, acc).  The equations' sums keep the reference's
+// term order; they are accumulated direction by direction (x, then y, then
+// z) as running partial sums, which is the same order of operations.
 #include "launch.hpp"
 
 namespace misohip {
@@ -25,7 +28,7 @@ namespace misohip {
 //  D=x: vx*by vy*bx vx*bz vz*bx;  D=y: vy*bx vx*by vy*bz vz*by;
 //  D=z: vz*bx vx*bz vz*by vy*bz.
 template <typename R, int D>
-__device__ __forceinline__ void point_terms(const R f[9], R gm, R o[17]) {
+__device__ __forceinline__ void point_terms(const R f[9], R gm1, R o[17]) {
   const R ro = f[0], vx = f[1], vy = f[2], vz = f[3];
   const R bx = f[4], by = f[5], bz = f[6], ei = f[7], ph = f[8];
   const R vd = D == 0 ? vx : (D == 1 ? vy : vz);
@@ -36,7 +39,7 @@ __device__ __forceinline__ void point_terms(const R f[9], R gm, R o[17]) {
   o[2] = rvy * vd;
   o[3] = rvz * vd;
   // eos.hpp:30: pr = (gm - 1) * ro * ei
-  const R pr = (gm - R(1)) * ro * ei;
+  const R pr = gm1 * ro * ei;
   o[4] = pr;
   // mhd_integrator_host.hpp:69-89
   const R bb = bx * bx + by * by + bz * bz;
@@ -61,29 +64,40 @@ __device__ __forceinline__ void point_terms(const R f[9], R gm, R o[17]) {
 
 // All 17 derivatives along D at linear index c (stride s, inverse spacing dd).
 template <typename R, int D>
-__device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, R dd, R gm,
+__device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, R dd, R gm1,
                                            R S[17]) {
   R f[9], t[17];
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + 2 * s];
-  point_terms<R, D>(f, gm, t);
+  point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = -t[n];
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + s];
-  point_terms<R, D>(f, gm, t);
+  point_terms<R, D>(f, gm1, t);
 #pragma unroll
-  for (int n = 0; n < 17; ++n) S[n] = S[n] + R(8) * t[n];
+  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(8), t[n], S[n]);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - s];
-  point_terms<R, D>(f, gm, t);
+  point_terms<R, D>(f, gm1, t);
 #pragma unroll
-  for (int n = 0; n < 17; ++n) S[n] = S[n] - R(8) * t[n];
+  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(-8), t[n], S[n]);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - 2 * s];
-  point_terms<R, D>(f, gm, t);
+  point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
+}
+
+template <typename R, int D, bool ACTIVE>
+__device__ __forceinline__ void derivs_or_zero(const Fld<const R> &a, int c, int s, R dd, R gm1,
+                                               R S[17]) {
+  if (ACTIVE) {
+    dir_derivs<R, D>(a, c, s, dd, gm1, S);
+  } else {
+#pragma unroll
+    for (int n = 0; n < 17; ++n) S[n] = R(0);
+  }
 }
 
 // Energy recovery is carried in double for both storage types ("pressure
@@ -93,64 +107,105 @@ __global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<co
                                                        Fld<R> rslt, const R *__restrict__ dxi,
                                                        const R *__restrict__ dyi,
                                                        const R *__restrict__ dzi, Shape sh,
-                                                       RkScalars sc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + sh.mi;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y + sh.mj;
-  const int k = blockIdx.z + sh.mk;
+                                                       RkScalars sc, int nbx, int nby, int nb) {
+  const int V = xcd_virtual_block(blockIdx.x, nb);
+  const int blk_x = V % nbx, rest = V / nbx;
+  const int blk_y = rest % nby, blk_z = rest / nby;
+  const int i = blk_x * 64 + (threadIdx.x & 63) + sh.mi;
+  const int j = blk_y * 4 + (threadIdx.x >> 6) + sh.mj;
+  const int k = blk_z + sh.mk;
   if (i >= sh.It - sh.mi || j >= sh.Jt - sh.mj) return;
   const int c = sh.idx(i, j, k);
-  const R gm = R(sc.gm), dt = R(sc.dt);
+  const R gm1 = R(sc.gm) - R(1), dt = R(sc.dt);
+  const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
+  R S[17];
 
-  R Sx[17], Sy[17], Sz[17];
-  if (HX) {
-    dir_derivs<R, 0>(argm, c, 1, dxi[i], gm, Sx);
-  } else {
-#pragma unroll
-    for (int n = 0; n < 17; ++n) Sx[n] = R(0);
-  }
-  if (HY) {
-    dir_derivs<R, 1>(argm, c, sh.It, dyi[j], gm, Sy);
-  } else {
-#pragma unroll
-    for (int n = 0; n < 17; ++n) Sy[n] = R(0);
-  }
-  if (HZ) {
-    dir_derivs<R, 2>(argm, c, sh.It * sh.Jt, dzi[k], gm, Sz);
-  } else {
-#pragma unroll
-    for (int n = 0; n < 17; ++n) Sz[n] = R(0);
-  }
+  // ---- x: start every running sum
+  derivs_or_zero<R, 0, HX>(argm, c, 1, HX ? dxi[i] : R(0), gm1, S);
+  R a_ro = -S[0];
+  R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
+  R a_vy = -S[2], t_vy = S[7];
+  R a_vz = -S[3], t_vz = S[8];
+  R a_by = -S[9], k_by = S[10];
+  R a_bz = -S[11], k_bz = S[12];
+  R ph_bx = S[13];
+  R a_ph = -S[14];
+  R a_ei = -S[15], t_ei = S[16];
+  // Phase fence: the y loads' index depends (opaquely) on an x result, so
+  // the compiler cannot hoist them above the x phase; each phase keeps 36
+  // loads in flight instead of 108, which keeps the wave count per SIMD up.
+  int cy = c;
+  asm volatile("" : "+v"(cy), "+v"(a_ro), "+v"(a_vx), "+v"(d4x), "+v"(d5x), "+v"(t_vx),
+               "+v"(a_vy), "+v"(t_vy), "+v"(a_vz), "+v"(t_vz), "+v"(a_by), "+v"(k_by),
+               "+v"(a_bz), "+v"(k_bz), "+v"(ph_bx), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
+
+  // ---- y
+  derivs_or_zero<R, 1, HY>(argm, cy, sh.It, HY ? dyi[j] : R(0), gm1, S);
+  a_ro = a_ro - S[0];
+  a_vx = a_vx - S[1];
+  t_vx = t_vx + S[6];
+  a_vy = a_vy - S[2];
+  const R d4y = S[4], d5y = S[5] * pii8;
+  t_vy = t_vy + S[7];
+  a_vz = a_vz - S[3];
+  t_vz = t_vz + S[8];
+  R a_bx = -S[9];
+  const R k_bx = S[10];
+  const R ph_by = S[13];
+  a_bz = a_bz - S[11] + k_bz + S[12];
+  a_ph = a_ph - S[14];
+  a_ei = a_ei - S[15];
+  t_ei = t_ei + S[16];
+  int cz = c;
+  R d4y_ = d4y, d5y_ = d5y, k_bx_ = k_bx, ph_by_ = ph_by;
+  asm volatile("" : "+v"(cz), "+v"(a_ro), "+v"(a_vx), "+v"(d4x), "+v"(d5x), "+v"(t_vx),
+               "+v"(a_vy), "+v"(t_vy), "+v"(d4y_), "+v"(d5y_), "+v"(a_vz), "+v"(t_vz),
+               "+v"(a_bx), "+v"(k_bx_), "+v"(a_by), "+v"(k_by), "+v"(ph_by_), "+v"(ph_bx),
+               "+v"(a_bz), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
+
+  // ---- z
+  derivs_or_zero<R, 2, HZ>(argm, cz, sh.It * sh.Jt, HZ ? dzi[k] : R(0), gm1, S);
+  a_ro = a_ro - S[0];
+  a_vx = a_vx - S[1];
+  t_vx = t_vx + S[6];
+  a_vy = a_vy - S[2];
+  t_vy = t_vy + S[7];
+  a_vz = a_vz - S[3];
+  const R d4z = S[4], d5z = S[5] * pii8;
+  t_vz = t_vz + S[8];
+  a_bx = a_bx - S[9] + k_bx_ + S[10] - ph_bx;
+  a_by = a_by - S[11] + k_by + S[12] - ph_by_;
+  a_bz = a_bz - S[13];
+  a_ph = a_ph - S[14];
+  a_ei = a_ei - S[15];
+  t_ei = t_ei + S[16];
 
   const R o_ro = orgn.q[0][c], o_vx = orgn.q[1][c], o_vy = orgn.q[2][c], o_vz = orgn.q[3][c];
   const R o_bx = orgn.q[4][c], o_by = orgn.q[5][c], o_bz = orgn.q[6][c];
   const R o_ei = orgn.q[7][c], o_ph = orgn.q[8][c];
-  const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
 
   // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134)
   R src[3] = {R(0), R(0), R(0)};
   if (sc.src_kind == 1) src[sc.grav_axis] = -R(sc.grav) * argm.q[0][c];
 
   // continuity
-  const R ro = o_ro + dt * (-Sx[0] - Sy[0] - Sz[0]);
-  // equations of motion
-  const R vx = (o_ro * o_vx + dt * (-Sx[1] - Sy[1] - Sz[1] - Sx[4] - Sx[5] * pii8 +
-                                    pii4 * (+Sx[6] + Sy[6] + Sz[6]) + src[0])) / ro;
-  const R vy = (o_ro * o_vy + dt * (-Sx[2] - Sy[2] - Sz[2] - Sy[4] - Sy[5] * pii8 +
-                                    pii4 * (+Sx[7] + Sy[7] + Sz[7]) + src[1])) / ro;
-  const R vz = (o_ro * o_vz + dt * (-Sx[3] - Sy[3] - Sz[3] - Sz[4] - Sz[5] * pii8 +
-                                    pii4 * (+Sx[8] + Sy[8] + Sz[8]) + src[2])) / ro;
+  const R ro = o_ro + dt * a_ro;
+  // equations of motion: (ro v)_0 + dt*(flux - grad p - grad bb/8pi + tension/4pi + src)
+  const R vx = (o_ro * o_vx + dt * (a_vx - d4x - d5x + pii4 * t_vx + src[0])) / ro;
+  const R vy = (o_ro * o_vy + dt * (a_vy - d4y_ - d5y_ + pii4 * t_vy + src[1])) / ro;
+  const R vz = (o_ro * o_vz + dt * (a_vz - d4z - d5z + pii4 * t_vz + src[2])) / ro;
   // induction
-  const R bx = o_bx + dt * (-Sy[9] - Sz[9] + Sy[10] + Sz[10] - Sx[13]);
-  const R by = o_by + dt * (-Sx[9] - Sz[11] + Sx[10] + Sz[12] - Sy[13]);
-  const R bz = o_bz + dt * (-Sx[11] - Sy[11] + Sx[12] + Sy[12] - Sz[13]);
+  const R bx = o_bx + dt * a_bx;
+  const R by = o_by + dt * a_by;
+  const R bz = o_bz + dt * a_bz;
   // GLM divergence cleaning; damp = exp(-dt/tau) from the host
-  const R ph = (o_ph + dt * R(sc.ch2) * (-Sx[14] - Sy[14] - Sz[14])) * R(sc.damp);
+  const R ph = (o_ph + dt * R(sc.ch2) * a_ph) * R(sc.damp);
   // energy: total energy of orgn, flux divergence, recovery of ei
   const double eo_ro = o_ro;
   const double Et = +eo_ro * double(o_ei) +
                     0.5 * eo_ro * (+double(o_vx) * o_vx + double(o_vy) * o_vy + double(o_vz) * o_vz) +
                     double(pii8) * (+double(o_bx) * o_bx + double(o_by) * o_by + double(o_bz) * o_bz);
-  const R eflux = dt * (-Sx[15] - Sy[15] - Sz[15] + pii4 * (+Sx[16] + Sy[16] + Sz[16]));
+  const R eflux = dt * (a_ei + pii4 * t_ei);
   const double dro = ro, dvx = vx, dvy = vy, dvz = vz, dbx = bx, dby = by, dbz = bz;
   const double ei = (Et + double(eflux) - 0.5 * dro * (+dvx * dvx + dvy * dvy + dvz * dvz) -
                      double(pii8) * (+dbx * dbx + dby * dby + dbz * dbz) + 0.0) /
@@ -172,10 +227,12 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
                               const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
                               const Shape &sh, const RkScalars &sc) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
-  dim3 block(64, 4, 1);
-  dim3 grid((ni + 63) / 64, (nj + 3) / 4, nk);
-  hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ>), grid, block, 0, st, o, a, r, dxi, dyi, dzi,
-                     sh, sc);
+  const int nbx = (ni + 63) / 64, nby = (nj + 3) / 4;
+  const long nbl = (long)nbx * nby * nk;
+  if (nbl > 2147483647L) return hipErrorInvalidValue;
+  const int nb = (int)nbl;
+  hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ>), dim3(nb), dim3(256), 0, st, o, a, r, dxi,
+                     dyi, dzi, sh, sc, nbx, nby, nb);
   return hipGetLastError();
 }
 
