@@ -162,6 +162,18 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                  const misohip_fields *qq, const misohip_work *w,
                  double *dt_out);
 
+/* Launch shapes of the stencil kernels (no counterpart in the reference,
+ * whose CUDA kernels use fixed 8x8x8 blocks, cuda_util.cuh:35-60).
+ * rk_pencil_bx/by: pencil of the RK-stage block order, in blocks of 64x4
+ * cells; sweep_chunk: cells each thread of the y/z AV sweeps walks.
+ * Defaults {2, 8, 64}; the autotuner sets them per grid. */
+typedef struct misohip_tiles {
+  int rk_pencil_bx, rk_pencil_by;
+  int sweep_chunk;
+} misohip_tiles;
+int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t);
+int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t);
+
 /* Kernel timing with HIP events recorded on the context's stream around every
  * launch the library makes (no counterpart in the reference, which has no
  * timers).  Kinds: see MISOHIP_KIND_*.  misohip_profile_read synchronises,

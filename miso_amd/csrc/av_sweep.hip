@@ -123,6 +123,200 @@ __global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> r
   rslt.q[8][c] = bn[3];
 }
 
+// ---------------------------------------------------------------------------
+// y and z sweeps that walk along the sweep axis.  Lanes run along x
+// (coalesced); each thread walks `chunk` cells along DIR keeping a window of
+// the conserved values, so the slope at every point and the flux at every
+// interface are computed once and reused by both cells that share them (the
+// reference recomputes them three and two times with identical operands, so
+// the reuse is exact).  Consecutive blocks are consecutive chunks of the
+// same column, so chunk halos are L2 hits.
+template <typename R> struct AvCons {
+  R v[9];  // ro, ro*vx, ro*vy, ro*vz, bx, by, bz, ph, ro*ei
+  R vx, vy, vz, ei;
+};
+
+template <typename R>
+__device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R> &a) {
+  const R ro = q.q[0][c];
+  a.vx = q.q[1][c];
+  a.vy = q.q[2][c];
+  a.vz = q.q[3][c];
+  a.ei = q.q[7][c];
+  a.v[0] = ro;
+  a.v[1] = ro * a.vx;
+  a.v[2] = ro * a.vy;
+  a.v[3] = ro * a.vz;
+  a.v[4] = q.q[4][c];
+  a.v[5] = q.q[5][c];
+  a.v[6] = q.q[6][c];
+  a.v[7] = q.q[8][c];
+  a.v[8] = ro * a.ei;
+}
+
+// conservative update of one cell from its two interface fluxes
+// (mhd_artificial_viscosity_host.hpp:131-344)
+template <typename R>
+__device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9], const R fup[9],
+                                              R dd, R dt, Fld<R> &out, int c) {
+  const R ro_n = a.v[0] - (fup[0] - fdw[0]) * dd * dt;
+  const R vxn = (a.v[1] - (fup[1] - fdw[1]) * dd * dt) / ro_n;
+  const R vyn = (a.v[2] - (fup[2] - fdw[2]) * dd * dt) / ro_n;
+  const R vzn = (a.v[3] - (fup[3] - fdw[3]) * dd * dt) / ro_n;
+  const R bxn = a.v[4] - (fup[4] - fdw[4]) * dd * dt;
+  const R byn = a.v[5] - (fup[5] - fdw[5]) * dd * dt;
+  const R bzn = a.v[6] - (fup[6] - fdw[6]) * dd * dt;
+  const R phn = a.v[7] - (fup[7] - fdw[7]) * dd * dt;
+  const double pii8 = double(Const<R>::pii8);
+  const double ro = a.v[0];
+  const double Et = ro * double(a.ei) +
+                    0.5 * ro * (double(a.vx) * a.vx + double(a.vy) * a.vy + double(a.vz) * a.vz) +
+                    pii8 * (double(a.v[4]) * a.v[4] + double(a.v[5]) * a.v[5] +
+                            double(a.v[6]) * a.v[6]);
+  const double dro = ro_n, dvx = vxn, dvy = vyn, dvz = vzn, dbx = bxn, dby = byn, dbz = bzn;
+  const double ein = (Et - double((fup[8] - fdw[8]) * dd * dt) -
+                      0.5 * dro * (dvx * dvx + dvy * dvy + dvz * dvz) -
+                      pii8 * (dbx * dbx + dby * dby + dbz * dbz)) /
+                     dro;
+  out.q[0][c] = ro_n;
+  out.q[1][c] = vxn;
+  out.q[2][c] = vyn;
+  out.q[3][c] = vzn;
+  out.q[4][c] = bxn;
+  out.q[5][c] = byn;
+  out.q[6][c] = bzn;
+  out.q[7][c] = R(ein);
+  out.q[8][c] = phn;
+}
+
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                     const R *__restrict__ cc,
+                                                     const R *__restrict__ dd_arr, Shape sh,
+                                                     int chunk, int nchunks, int nbx, int nbo,
+                                                     int nb, AvScalars sc) {
+  const int V = xcd_virtual_block(blockIdx.x, nb);
+  const int ch = V % nchunks, r = V / nchunks;
+  const int bx = r % nbx, bo = r / nbx;
+  const int i = bx * 64 + (threadIdx.x & 63) + sh.mi;
+  const int o = bo * 4 + (threadIdx.x >> 6);  // the other transverse axis
+  if (i >= sh.It - sh.mi) return;
+  int j, k, m_dir, n_dir;
+  if (DIR == 1) {
+    k = o + sh.mk;
+    if (k >= sh.Kt - sh.mk) return;
+    m_dir = sh.mj; n_dir = sh.Jt - 2 * sh.mj;
+    j = m_dir + ch * chunk;
+  } else {
+    j = o + sh.mj;
+    if (j >= sh.Jt - sh.mj) return;
+    m_dir = sh.mk; n_dir = sh.Kt - 2 * sh.mk;
+    k = m_dir + ch * chunk;
+  }
+  const int p0 = ch * chunk;  // offset along DIR from the first interior cell
+  const int len = min(chunk, n_dir - p0);
+  if (len <= 0) return;
+  const int s = sh.stride(DIR);
+  const R ep = R(sc.ep), fh = R(sc.fh), dt = R(sc.dt);
+  int c = sh.idx(i, j, k);
+
+  // window: a0 = p, a1 = p+1 (and p+2 loaded per step); slopes at p; flux p-1/2
+  AvCons<R> am, a0, a1, a2;
+  load_cons(qq, c - 2 * s, am);
+  load_cons(qq, c - s, a0);
+  load_cons(qq, c, a1);
+  load_cons(qq, c + s, a2);
+  R d0[9], fdw[9];
+  R cc_m = cc[c - s], cc_0 = cc[c], cc_1 = cc[c + s];
+  {
+    const R cdw = R(0.5) * (cc_m + cc_0);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+      const R dm = dqq_eval(am.v[f], a0.v[f], a1.v[f], ep);  // slope at p0-1
+      d0[f] = dqq_eval(a0.v[f], a1.v[f], a2.v[f], ep);       // slope at p0
+      fdw[f] = flux_core(a0.v[f], a1.v[f], dm, d0[f], cdw, fh);
+    }
+  }
+  // shift so that a0 = p0, a1 = p0+1
+  a0 = a1;
+  a1 = a2;
+  for (int t = 0; t < len; ++t) {
+    AvCons<R> an;
+    load_cons(qq, c + 2 * s, an);
+    const R cc_2 = cc[c + 2 * s];
+    const R cup = R(0.5) * (cc_0 + cc_1);
+    R fup[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+      const R d1 = dqq_eval(a0.v[f], a1.v[f], an.v[f], ep);  // slope at p+1
+      fup[f] = flux_core(a0.v[f], a1.v[f], d0[f], d1, cup, fh);
+      d0[f] = d1;
+    }
+    const R dd = dd_arr[m_dir + p0 + t];
+    av_cell_update(a0, fdw, fup, dd, dt, rslt, c);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) fdw[f] = fup[f];
+    a0 = a1;
+    a1 = an;
+    cc_0 = cc_1;
+    cc_1 = cc_2;
+    c += s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep with lane shuffles: a wave covers x cells [i0-2, i0+62) and
+// outputs [i0, i0+60); every lane evaluates cc, the slope and the right
+// interface flux at its own point once and takes its neighbours' from the
+// adjacent lanes.  It also stores the frozen cc for the y and z sweeps over
+// the interior plus a one-cell ring in y and z.
+template <typename R>
+__device__ __forceinline__ R lane_up(R v) {  // value of lane - 1
+  return __shfl_up(v, 1, 64);
+}
+template <typename R>
+__device__ __forceinline__ R lane_dn(R v) {  // value of lane + 1
+  return __shfl_down(v, 1, 64);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void av_xshfl_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                      R *__restrict__ cc_store,
+                                                      const R *__restrict__ dxi, Shape sh,
+                                                      int j_lo, int k_lo, int j_n, int k_n,
+                                                      int nseg, int nb, AvScalars sc) {
+  const int V = xcd_virtual_block(blockIdx.x, nb);
+  const int seg = V % nseg, r = V / nseg;
+  const int lane = threadIdx.x & 63;
+  const int row = r * 4 + (threadIdx.x >> 6);  // (j, k) row, j fastest
+  if (row >= j_n * k_n) return;  // whole wave leaves together
+  const int j = j_lo + row % j_n, k = k_lo + row / j_n;
+  const int i0 = sh.mi + seg * 60;
+  const int i = i0 - 2 + lane;
+  const int ic = min(i, sh.It - 1);
+  const int c = sh.idx(ic, j, k);
+  const R gm = R(sc.gm);
+  const R ccv = char_speed(qq, c, gm, R(sc.cs_fac), R(sc.ca_fac), R(sc.vv_fac));
+  const bool out_lane = lane >= 2 && lane < 62 && i < sh.It - sh.mi;
+  if (out_lane) cc_store[c] = ccv;
+  const bool ring = (j < sh.mj) || (j >= sh.Jt - sh.mj) || (k < sh.mk) || (k >= sh.Kt - sh.mk);
+  if (ring) return;  // uniform per wave
+  AvCons<R> a;
+  load_cons(qq, c, a);
+  const R ep = R(sc.ep), fh = R(sc.fh), dt = R(sc.dt);
+  const R cup = R(0.5) * (ccv + lane_dn(ccv));
+  R fdw[9], fup[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    const R vm = lane_up(a.v[f]), vp = lane_dn(a.v[f]);
+    const R d = dqq_eval(vm, a.v[f], vp, ep);
+    const R dn = lane_dn(d);
+    fup[f] = flux_core(a.v[f], vp, d, dn, cup, fh);
+    fdw[f] = lane_up(fup[f]);
+  }
+  if (out_lane) av_cell_update(a, fdw, fup, dxi[i], dt, rslt, c);
+}
+
 // Characteristic speed over every cell (misohip_char_vel, kept for the
 // per-function parity tests; the step computes it inside the x sweep).
 template <typename R>
@@ -152,12 +346,52 @@ static hipError_t launch_av_t(hipStream_t st, const Fld<const R> &q, const Fld<R
   return hipGetLastError();
 }
 
+template <typename R, int DIR>
+static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
+                                const R *cc_in, const R *dd, const Shape &sh,
+                                const AvScalars &sc) {
+  const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
+  const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
+  const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
+  const int nchunks = (n_dir + chunk - 1) / chunk;
+  const int nbx = (sh.It - 2 * sh.mi + 63) / 64, nbo = (n_oth + 3) / 4;
+  const long nbl = (long)nchunks * nbx * nbo;
+  if (nbl > 2147483647L) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((av_walk_kernel<R, DIR>), dim3((unsigned)nbl), dim3(256), 0, st, q, r, cc_in,
+                     dd, sh, chunk, nchunks, nbx, nbo, (int)nbl, sc);
+  return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_xshfl(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
+                               R *cc_store, const R *dxi, const Shape &sh, const AvScalars &sc) {
+  const int j_lo = sh.mj - sh.js, j_n = sh.Jt - 2 * sh.mj + 2 * sh.js;
+  const int k_lo = sh.mk - sh.ks, k_n = sh.Kt - 2 * sh.mk + 2 * sh.ks;
+  const int nseg = (sh.It - 2 * sh.mi + 59) / 60;
+  const long rows = (long)j_n * k_n;
+  const long nbl = (long)nseg * ((rows + 3) / 4);
+  if (nbl > 2147483647L) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((av_xshfl_kernel<R>), dim3((unsigned)nbl), dim3(256), 0, st, q, r, cc_store,
+                     dxi, sh, j_lo, k_lo, j_n, k_n, nseg, (int)nbl, sc);
+  return hipGetLastError();
+}
+
+// Active directions use the x-shuffle and walking kernels; a direction of
+// extent one (2D, 1D runs) is the pointwise degenerate sweep of the
+// reference, done by the generic kernel with stride 0.
 template <typename R>
 hipError_t launch_av_sweep(hipStream_t st, int dir, const Fld<const R> &q, const Fld<R> &r,
                            R *cc_store, const R *cc_in, const R *dd, const Shape &sh,
                            const AvScalars &sc) {
-  if (dir == 0) return launch_av_t<R, 0>(st, q, r, cc_store, cc_in, dd, sh, sc);
-  if (dir == 1) return launch_av_t<R, 1>(st, q, r, cc_store, cc_in, dd, sh, sc);
+  if (dir == 0) {
+    if (sh.is) return launch_xshfl<R>(st, q, r, cc_store, dd, sh, sc);
+    return launch_av_t<R, 0>(st, q, r, cc_store, cc_in, dd, sh, sc);
+  }
+  if (dir == 1) {
+    if (sh.js) return launch_walk_t<R, 1>(st, q, r, cc_in, dd, sh, sc);
+    return launch_av_t<R, 1>(st, q, r, cc_store, cc_in, dd, sh, sc);
+  }
+  if (sh.ks) return launch_walk_t<R, 2>(st, q, r, cc_in, dd, sh, sc);
   return launch_av_t<R, 2>(st, q, r, cc_store, cc_in, dd, sh, sc);
 }
 

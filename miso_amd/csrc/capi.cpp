@@ -29,6 +29,7 @@ struct misohip_ctx {
   std::vector<hipEvent_t> pool;
   std::vector<int> rec_kind;
   std::vector<hipEvent_t> rec_start, rec_stop;
+  misohip_tiles tile = {2, 8, 64};
 };
 
 namespace {
@@ -141,14 +142,17 @@ int rk_stage_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   RkScalars sc;
   sc.dt = dt_s; sc.ch2 = ch2; sc.damp = damp; sc.gm = (R)p->gm;
   sc.grav = (R)p->grav; sc.src_kind = p->src_kind; sc.grav_axis = p->grav_axis;
+  sc.pencil_bx = ctx->tile.rk_pencil_bx;
+  sc.pencil_by = ctx->tile.rk_pencil_by;
   MISOHIP_TRY(launch_rk_stage<R>(ctx->stream, cfld<R>(o), cfld<R>(a), fld<R>(r),
                                  static_cast<const R *>(g->dxi), static_cast<const R *>(g->dyi),
                                  static_cast<const R *>(g->dzi), make_shape(g), sc));
   return MISOHIP_OK;
 }
 
-template <typename R> AvScalars av_scalars(const misohip_params *p, R dt) {
+template <typename R> AvScalars av_scalars(const misohip_params *p, R dt, int chunk = 64) {
   AvScalars s;
+  s.chunk = chunk;
   s.dt = dt; s.ep = (R)p->ep; s.fh = (R)p->fh; s.gm = (R)p->gm;
   s.cs_fac = (R)p->cs_fac; s.ca_fac = (R)p->ca_fac; s.vv_fac = (R)p->vv_fac;
   return s;
@@ -161,7 +165,7 @@ int av_sweep_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   Timed t_(ctx, MISOHIP_KIND_AV_X + dir);
   MISOHIP_TRY(launch_av_sweep<R>(ctx->stream, dir, cfld<R>(q), fld<R>(r), static_cast<R *>(cc),
                                  static_cast<const R *>(cc), dd, make_shape(g),
-                                 av_scalars<R>(p, dt)));
+                                 av_scalars<R>(p, dt, ctx->tile.sweep_chunk)));
   return MISOHIP_OK;
 }
 
@@ -268,6 +272,19 @@ int misohip_create(int device, void *stream, misohip_ctx **out) {
     return hip_fail(e);
   }
   *out = c;
+  return MISOHIP_OK;
+}
+
+int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t) {
+  if (!ctx || !t) return MISOHIP_E_ARG;
+  if (t->rk_pencil_bx < 1 || t->rk_pencil_by < 1 || t->sweep_chunk < 4) return MISOHIP_E_ARG;
+  ctx->tile = *t;
+  return MISOHIP_OK;
+}
+
+int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t) {
+  if (!ctx || !t) return MISOHIP_E_ARG;
+  *t = ctx->tile;
   return MISOHIP_OK;
 }
 

@@ -49,6 +49,36 @@ __device__ __forceinline__ int xcd_virtual_block(int L, int nb) {
   return x * q + (x < r ? x : r) + (L >> 3);
 }
 
+// Pencil order of a 3D grid of blocks (nbx, nby, nbz): the x-y block plane
+// is cut into pencils of pbx x pby blocks; inside a pencil the block plane is
+// fastest and z next, so the z-neighbour planes of a block were touched a few
+// hundred blocks earlier on the same XCD and are still in its L2.  Combined
+// with xcd_virtual_block every XCD sweeps whole pencils.
+struct Pencils {
+  int nbx, nby, nbz, pbx, pby, npx, nb;
+  __host__ bool init(int nbx_, int nby_, int nbz_, int pbx_, int pby_) {
+    nbx = nbx_; nby = nby_; nbz = nbz_;
+    pbx = pbx_ < 1 ? 1 : (pbx_ > nbx ? nbx : pbx_);
+    pby = pby_ < 1 ? 1 : (pby_ > nby ? nby : pby_);
+    npx = (nbx + pbx - 1) / pbx;
+    const int npy = (nby + pby - 1) / pby;
+    const long n = (long)npx * npy * pbx * pby * nbz;
+    if (n > 2147483647L || n <= 0) return false;
+    nb = (int)n;
+    return true;
+  }
+  // false for the padding blocks of partial edge pencils
+  __device__ bool decode(int V, int &bx, int &by, int &bz) const {
+    const int per = pbx * pby;
+    const int in = V % per, r = V / per;
+    bz = r % nbz;
+    const int pid = r / nbz;
+    bx = (pid % npx) * pbx + in % pbx;
+    by = (pid / npx) * pby + in / pbx;
+    return bx < nbx && by < nby;
+  }
+};
+
 // Field set: nine device arrays in MISO's order ro vx vy vz bx by bz ei ph.
 template <typename T> struct Fld {
   T *q[9];

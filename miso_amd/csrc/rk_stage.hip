@@ -107,10 +107,9 @@ __global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<co
                                                        Fld<R> rslt, const R *__restrict__ dxi,
                                                        const R *__restrict__ dyi,
                                                        const R *__restrict__ dzi, Shape sh,
-                                                       RkScalars sc, int nbx, int nby, int nb) {
-  const int V = xcd_virtual_block(blockIdx.x, nb);
-  const int blk_x = V % nbx, rest = V / nbx;
-  const int blk_y = rest % nby, blk_z = rest / nby;
+                                                       RkScalars sc, Pencils pc) {
+  int blk_x, blk_y, blk_z;
+  if (!pc.decode(xcd_virtual_block(blockIdx.x, pc.nb), blk_x, blk_y, blk_z)) return;
   const int i = blk_x * 64 + (threadIdx.x & 63) + sh.mi;
   const int j = blk_y * 4 + (threadIdx.x >> 6) + sh.mj;
   const int k = blk_z + sh.mk;
@@ -227,12 +226,11 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
                               const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
                               const Shape &sh, const RkScalars &sc) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
-  const int nbx = (ni + 63) / 64, nby = (nj + 3) / 4;
-  const long nbl = (long)nbx * nby * nk;
-  if (nbl > 2147483647L) return hipErrorInvalidValue;
-  const int nb = (int)nbl;
-  hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ>), dim3(nb), dim3(256), 0, st, o, a, r, dxi,
-                     dyi, dzi, sh, sc, nbx, nby, nb);
+  Pencils pc;
+  if (!pc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.pencil_bx, sc.pencil_by))
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ>), dim3(pc.nb), dim3(256), 0, st, o, a, r,
+                     dxi, dyi, dzi, sh, sc, pc);
   return hipGetLastError();
 }
 

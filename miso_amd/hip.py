@@ -42,6 +42,10 @@ class MisohipParams(C.Structure):
                 ("grav", C.c_double)]
 
 
+class MisohipTiles(C.Structure):
+    _fields_ = [("rk_pencil_bx", C.c_int), ("rk_pencil_by", C.c_int), ("sweep_chunk", C.c_int)]
+
+
 class MisohipWork(C.Structure):
     _fields_ = [("argm", MisohipFields), ("rslt", MisohipFields), ("cc", C.c_void_p)]
 
@@ -71,6 +75,8 @@ SIGNATURES = {
                        _P(MisohipFields), _P(MisohipWork), C.c_double],
     "misohip_step": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
                      _P(MisohipFields), _P(MisohipWork), _P(C.c_double)],
+    "misohip_set_tiles": [C.c_void_p, _P(MisohipTiles)],
+    "misohip_get_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_profile_enable": [C.c_void_p, C.c_int],
     "misohip_profile_read": [C.c_void_p, _P(C.c_double), _P(C.c_int), C.c_int],
 }

@@ -79,6 +79,14 @@ OG_DECL(float, f32)
 /* Scratch needed by og_update_* / og_run_*: 9N (argm) + 9N (rslt) + 6N. */
 size_t og_work_elems(const og_grid *g);
 
+/* Radiative transfer (rt_oracle.c), Real = double. */
+int og_rt_quadrature(int num_rays, double *w, double *mx, double *my, double *mz);
+int og_rt_solve(const og_grid *g, const double *dxyz, int num_rays, const double *kap,
+                const double *S, double *I, const int *periodic, const int *face_kind,
+                const double *face_value, double tolerance, int max_iters, double *max_diff);
+void og_rt_qrad(const og_grid *g, int num_rays, const double *kap, const double *S,
+                const double *I, double *qrad);
+
 /* OpenMP threads used by the oracle (0 = leave the runtime default). */
 void og_set_threads(int n);
 int og_get_threads(void);

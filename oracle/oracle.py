@@ -40,7 +40,8 @@ class OgParams(C.Structure):
 def build(force: bool = False) -> str:
     """Compile the oracle with the flags its parity relies on."""
     src = os.path.join(HERE, "mhd_oracle.c")
-    deps = [src, os.path.join(HERE, "mhd_oracle_body.inc"), os.path.join(HERE, "mhd_oracle.h")]
+    deps = [src, os.path.join(HERE, "mhd_oracle_body.inc"), os.path.join(HERE, "mhd_oracle.h"),
+            os.path.join(HERE, "rt_oracle.c")]
     if (not force and os.path.exists(LIB_PATH)
             and all(os.path.getmtime(LIB_PATH) >= os.path.getmtime(d) for d in deps)):
         return LIB_PATH
@@ -89,6 +90,14 @@ def lib():
             getattr(L, f"og_dqq_eval_{S}").restype = R
             getattr(L, f"og_flux_core_{S}").argtypes = [R, R, R, R, R, R]
             getattr(L, f"og_flux_core_{S}").restype = R
+        D = C.POINTER(C.c_double)
+        I = C.POINTER(C.c_int)
+        L.og_rt_quadrature.argtypes = [C.c_int, D, D, D, D]
+        L.og_rt_quadrature.restype = C.c_int
+        L.og_rt_solve.argtypes = [C.POINTER(OgGrid), D, C.c_int, D, D, D, I, I, D, C.c_double,
+                                  C.c_int, D]
+        L.og_rt_solve.restype = C.c_int
+        L.og_rt_qrad.argtypes = [C.POINTER(OgGrid), C.c_int, D, D, D, D]
         L.og_work_elems.argtypes = [C.POINTER(OgGrid)]
         L.og_work_elems.restype = C.c_size_t
         L.og_set_threads.argtypes = [C.c_int]
@@ -255,3 +264,35 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().og_get_threads())
+
+
+# ---------------------------------------------------------------- radiative transfer
+def rt_quadrature(num_rays: int = 24):
+    """(weights, mu_x, mu_y, mu_z) of rt_quadrature.hpp."""
+    arrs = [np.zeros(24) for _ in range(4)]
+    n = lib().og_rt_quadrature(num_rays, *[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
+    return tuple(a[:n].copy() for a in arrs)
+
+
+def rt_solve(grid: Grid, kappa, S, rint, periodic=(1, 1, 0), face_kind=(0, 0, 0, 0, 1, 2),
+             face_value=None, tolerance=1e-8, max_iters=200, num_rays=24):
+    """RT::solve on one rank (FP64); rint (rays, It, Jt, Kt) is updated in
+    place.  face_kind per (axis, side): 0 none, 1 I=S, 2 I=0, 3 face_value.
+    Returns (iterations, last max difference)."""
+    assert grid.dtype == np.float64
+    dxyz = np.array([grid.axis("dx")[0], grid.axis("dy")[0], grid.axis("dz")[0]])
+    per = (C.c_int * 3)(*periodic)
+    fk = (C.c_int * 6)(*face_kind)
+    fv = (C.c_double * 6)(*(face_value or (0.0,) * 6))
+    md = C.c_double()
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    it = lib().og_rt_solve(C.byref(grid.g), P(dxyz), num_rays, P(kappa), P(S), P(rint), per, fk,
+                           fv, tolerance, max_iters, C.byref(md))
+    return it, md.value
+
+
+def rt_qrad(grid: Grid, kappa, S, rint, num_rays=24) -> np.ndarray:
+    q = np.zeros(grid.shape)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    lib().og_rt_qrad(C.byref(grid.g), num_rays, P(kappa), P(S), P(rint), P(q))
+    return q

@@ -67,9 +67,9 @@ class OracleVariant:
 
 def planted(n):
     out = os.path.join(ORACLE_DIR, "_build", f"libmiso_oracle_plant{n}.so")
-    src = os.path.join(ORACLE_DIR, "mhd_oracle.c")
+    srcs = [os.path.join(ORACLE_DIR, f) for f in ("mhd_oracle.c", "rt_oracle.c")]
     subprocess.run(["gcc", "-std=gnu11", "-O2", "-ffp-contract=off", "-fopenmp", "-fPIC",
-                    "-shared", f"-DOG_PLANT={n}", "-o", out, src, "-lm"], check=True)
+                    "-shared", f"-DOG_PLANT={n}", "-o", out] + srcs + ["-lm"], check=True)
     return OracleVariant(out)
 
 
