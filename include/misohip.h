@@ -162,6 +162,48 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                  const misohip_fields *qq, const misohip_work *w,
                  double *dt_out);
 
+/* ---------------------------------------------------------------------
+ * Radiative transfer (FP64).  Replaces rt::RT::solve and its helpers
+ * (miso/include/miso/rt_host.hpp:104-689, CPU-only in the reference) and
+ * the quadrature (miso/include/miso/rt_quadrature.hpp:14-63).  kappa and S
+ * are fields of the grid (x-fastest, ghosts included); rint and rint_old
+ * hold num_rays such fields back to back.  face_kind per (axis, side) of a
+ * non-periodic axis: 0 untouched, 1 incoming I = S (thermal), 2 incoming
+ * I = 0, 3 incoming I = face_value (set_incoming_boundary_on_physical_faces,
+ * rt_host.hpp:50-101).  num_rays 24 = Carlson A4, otherwise one +z ray. */
+typedef struct misohip_rt_config {
+  int num_rays;
+  int periodic[3];
+  int face_kind[6];
+  double face_value[6];
+  double tolerance;
+  int max_iters;
+} misohip_rt_config;
+
+/* RT::solve: iterate halo exchange, boundary, sweep until the float-
+ * narrowed max change is below tolerance (rt_host.hpp:632-651).  Writes the
+ * iteration count and the last max change; synchronises once per
+ * iteration. */
+int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
+                     const void *kappa, const void *S, void *rint, void *rint_old,
+                     int *iters, double *max_diff);
+
+/* Q_rad = 4 pi kappa (J - S), J = sum_r w_r I_r, on the RT lattice (no
+ * counterpart in the reference, which has no MHD coupling). */
+int misohip_rt_qrad(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
+                    const void *kappa, const void *S, const void *rint, void *qrad);
+
+/* Grey opacity and source function of the radiative-MHD workload from the
+ * fields, on every cell: T = (gm - 1) ei, kappa = kappa0 ro T^3, S = T^4. */
+int misohip_rt_source(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
+                      double gm, double kappa0, void *kappa, void *S);
+
+/* Radiative heating as an operator-split update after Integrator::update,
+ * interior cells: ei += dt * qrad / ro.  (MISO's Source::ei is added outside
+ * dt, mhd_integrator_host.hpp:289-308, so a rate cannot go through it.) */
+int misohip_rt_heat(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
+                    const void *qrad, double dt);
+
 /* Launch shapes of the stencil kernels (no counterpart in the reference,
  * whose CUDA kernels use fixed 8x8x8 blocks, cuda_util.cuh:35-60).
  * rk_pencil_bx/by: pencil of the RK-stage block order, in blocks of 64x4
@@ -185,7 +227,9 @@ int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t);
 #define MISOHIP_KIND_AV_Z 3
 #define MISOHIP_KIND_GHOST 4
 #define MISOHIP_KIND_CFL 5
-#define MISOHIP_KIND_COUNT 6
+#define MISOHIP_KIND_RT 6
+#define MISOHIP_KIND_MLP 7
+#define MISOHIP_KIND_COUNT 8
 int misohip_profile_enable(misohip_ctx *ctx, int on);
 int misohip_profile_read(misohip_ctx *ctx, double *ms, int *launches, int nkinds);
 

@@ -11,7 +11,6 @@
 #include <cstdio>
 #include <cstring>
 #include <new>
-#include <vector>
 
 #include "../../include/misohip.h"
 #include "launch.hpp"
@@ -19,66 +18,18 @@
 
 using namespace misohip;
 
-struct misohip_ctx {
-  int device;
-  hipStream_t stream;
-  unsigned *d_slot;  // CFL reduction slot (device)
-  unsigned *h_slot;  // pinned host copy
-  // timing: pairs of events recorded around launches, tagged by kind
-  bool profiling = false;
-  std::vector<hipEvent_t> pool;
-  std::vector<int> rec_kind;
-  std::vector<hipEvent_t> rec_start, rec_stop;
-  misohip_tiles tile = {2, 8, 64};
-};
+#include "ctx_internal.hpp"
 
-namespace {
-// RAII scope: records a start event on construction and a stop event on
-// destruction when profiling is on.
-struct Timed {
-  misohip_ctx *c;
-  int kind;
-  hipEvent_t a = nullptr, b = nullptr;
-  static hipEvent_t get(misohip_ctx *c) {
-    if (!c->pool.empty()) {
-      hipEvent_t e = c->pool.back();
-      c->pool.pop_back();
-      return e;
-    }
-    hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
-    return e;
-  }
-  Timed(misohip_ctx *c_, int k) : c(c_), kind(k) {
-    if (!c->profiling) return;
-    a = get(c);
-    b = get(c);
-    if (a) (void)hipEventRecord(a, c->stream);
-  }
-  ~Timed() {
-    if (!a || !b) return;
-    (void)hipEventRecord(b, c->stream);
-    c->rec_kind.push_back(kind);
-    c->rec_start.push_back(a);
-    c->rec_stop.push_back(b);
-  }
-};
-}  // namespace
-
-namespace {
-
+namespace misohip_detail {
 thread_local char g_last_hip_error[256];
-
 int hip_fail(hipError_t e) {
   std::snprintf(g_last_hip_error, sizeof(g_last_hip_error), "%s", hipGetErrorString(e));
   return MISOHIP_E_HIP;
 }
+}  // namespace misohip_detail
+using misohip_detail::g_last_hip_error;
 
-#define MISOHIP_TRY(expr)                   \
-  do {                                      \
-    hipError_t e_ = (expr);                 \
-    if (e_ != hipSuccess) return hip_fail(e_); \
-  } while (0)
+namespace {
 
 int check_grid(const misohip_grid *g) {
   if (!g) return MISOHIP_E_ARG;

@@ -53,6 +53,11 @@ hipError_t launch_rt_maxdiff(hipStream_t, const double *, const double *, const 
 hipError_t launch_rt_qrad(hipStream_t, const double *, const double *, const double *, double *,
                           const int[3], const int[3], int, int, long, int, const RtRays &);
 
+hipError_t launch_rt_source(hipStream_t, const double *, const double *, double *, double *, long,
+                            double, double);
+hipError_t launch_rt_heat(hipStream_t, const double *, double *, const double *, const Shape &,
+                          double);
+
 template <typename R>
 hipError_t launch_rk_stage(hipStream_t, const Fld<const R> &, const Fld<const R> &,
                            const Fld<R> &, const R *, const R *, const R *, const Shape &,

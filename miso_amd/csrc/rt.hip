@@ -195,3 +195,51 @@ hipError_t launch_rt_qrad(hipStream_t st, const double *I, const double *kap, co
 }
 
 }  // namespace misohip
+
+namespace misohip {
+
+// T = (gm-1) ei, kappa = kappa0 ro T^3, S = T^4 on every cell
+__global__ __launch_bounds__(256) void rt_source_kernel(const double *__restrict__ ro,
+                                                        const double *__restrict__ ei,
+                                                        double *__restrict__ kap,
+                                                        double *__restrict__ S, long n, double gm,
+                                                        double kappa0) {
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const double T = (gm - 1.0) * ei[c];
+  const double T2 = T * T;
+  kap[c] = kappa0 * ro[c] * (T2 * T);
+  S[c] = T2 * T2;
+}
+
+// ei += dt * q / ro on the interior
+__global__ __launch_bounds__(256) void rt_heat_kernel(const double *__restrict__ ro,
+                                                      double *__restrict__ ei,
+                                                      const double *__restrict__ q, Shape sh,
+                                                      double dt) {
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ni * nj * nk) return;
+  const int i = sh.mi + (int)(t % ni);
+  const long r = t / ni;
+  const int j = sh.mj + (int)(r % nj), k = sh.mk + (int)(r / nj);
+  const int c = sh.idx(i, j, k);
+  ei[c] = ei[c] + dt * q[c] / ro[c];
+}
+
+hipError_t launch_rt_source(hipStream_t st, const double *ro, const double *ei, double *kap,
+                            double *S, long n, double gm, double kappa0) {
+  hipLaunchKernelGGL(rt_source_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ro, ei,
+                     kap, S, n, gm, kappa0);
+  return hipGetLastError();
+}
+
+hipError_t launch_rt_heat(hipStream_t st, const double *ro, double *ei, const double *q,
+                          const Shape &sh, double dt) {
+  const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj) * (sh.Kt - 2 * sh.mk);
+  hipLaunchKernelGGL(rt_heat_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ro, ei,
+                     q, sh, dt);
+  return hipGetLastError();
+}
+
+}  // namespace misohip

@@ -1,0 +1,270 @@
+// C ABI of the GPU radiative transfer (include/misohip.h, misohip_rt_*).
+//
+// Host side of RT::solve (miso/include/miso/rt_host.hpp:632-651): per-ray
+// sweep constants are computed here with the reference's expressions
+// (rt_host.hpp:461-508, float-narrowed max/min included), then every
+// iteration is: copy rint -> rint_old, periodic halo (z, y, x), incoming
+// boundary on physical faces, one launch per plane for all rays, and the
+// max-change reduction read back to decide convergence.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "ctx_internal.hpp"
+#include "launch.hpp"
+
+using namespace misohip;
+
+namespace {
+
+struct RtSetup {
+  int nrays;
+  double w[24], mu[3][24];
+  RtRays rays;
+  int max_nodes, max_planes;
+  int b0[3], b1[3], size[3], st[3];
+  long ray_stride;
+  unsigned pos_mask[3];
+};
+
+// rt_quadrature.hpp:39-62
+int quadrature(int num_rays, double *w, double *mx, double *my, double *mz) {
+  if (num_rays == 24) {
+    const double mu0 = 1.0 / 3.0;
+    const double mu1 = std::sqrt(1.0 - 2.0 * mu0 * mu0);
+    const double ww0 = 1.0 / 24.0;
+    for (int r = 0; r < 24; ++r) {
+      // rays 0-7: |mu_x| = mu1, 8-15: |mu_y| = mu1, 16-23: |mu_z| = mu1; signs
+      // cycle x fastest, then y, then z (the tables of rt_quadrature.hpp:46-54)
+      const int g = r / 8, b = r % 8;
+      const double sx = (b & 1) ? 1.0 : -1.0, sy = (b & 2) ? 1.0 : -1.0, sz = (b & 4) ? 1.0 : -1.0;
+      w[r] = ww0;
+      mx[r] = sx * (g == 0 ? mu1 : mu0);
+      my[r] = sy * (g == 1 ? mu1 : mu0);
+      mz[r] = sz * (g == 2 ? mu1 : mu0);
+    }
+    return 24;
+  }
+  w[0] = 1.0;
+  mx[0] = 0.0;
+  my[0] = 0.0;
+  mz[0] = 1.0;
+  return 1;
+}
+
+inline double nmax2(double a, double b) { return (double)std::fmax((float)a, (float)b); }
+inline double nmin3(double a, double b, double c) {
+  return (double)std::fmin(std::fmin((float)a, (float)b), (float)c);
+}
+
+int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, RtSetup &S) {
+  S.nrays = quadrature(rt->num_rays, S.w, S.mu[0], S.mu[1], S.mu[2]);
+  const int It = g->i_total, Jt = g->j_total, Kt = g->k_total;
+  const int sizes[3] = {g->i_size, g->j_size, g->k_size};
+  const int margins[3] = {g->i_margin, g->j_margin, g->k_margin};
+  const int sts[3] = {g->is, g->js, g->ks};
+  const long strides[3] = {1, It, (long)It * Jt};
+  for (int a = 0; a < 3; ++a) {
+    S.size[a] = sizes[a];
+    S.st[a] = sts[a];
+    S.b0[a] = margins[a] - sts[a];
+    // on an axis of one cell the lattice is the single node 0 (the
+    // reference's b0 + size would address index 1 of a length-1 axis)
+    S.b1[a] = S.b0[a] + (sizes[a] > 1 ? sizes[a] : 0);
+  }
+  S.ray_stride = (long)It * Jt * Kt;
+  // uniform spacings (rt_host.hpp:467-469 uses dx[0] etc.)
+  double d[3] = {1.e30, 1.e30, 1.e30};
+  const void *dptr[3] = {g->dx, g->dy, g->dz};
+  for (int a = 0; a < 3; ++a)
+    if (sizes[a] > 1)
+      MISOHIP_TRY(hipMemcpy(&d[a], dptr[a], sizeof(double), hipMemcpyDeviceToHost));
+  S.max_nodes = 0;
+  S.max_planes = 0;
+  for (int a = 0; a < 3; ++a) S.pos_mask[a] = 0u;
+  for (int n = 0; n < S.nrays; ++n) {
+    const double mx = S.mu[0][n], my = S.mu[1][n], mz = S.mu[2][n];
+    const double eps_mu = 1.e-30;
+    const double am[3] = {nmax2(std::fabs(mx), eps_mu), nmax2(std::fabs(my), eps_mu),
+                          nmax2(std::fabs(mz), eps_mu)};
+    const double dsa[3] = {d[0] / am[0], d[1] / am[1], d[2] / am[2]};
+    const double ds = nmin3(dsa[0], dsa[1], dsa[2]);
+    const double cc[3] = {am[0] * ds / d[0], am[1] * ds / d[1], am[2] * ds / d[2]};
+    int dom;
+    if (dsa[0] <= dsa[1] && dsa[0] <= dsa[2]) dom = 0;
+    else if (dsa[1] <= dsa[0] && dsa[1] <= dsa[2]) dom = 1;
+    else dom = 2;
+    const double mus[3] = {mx, my, mz};
+    int beg[3], end_[3], t[3];
+    for (int a = 0; a < 3; ++a) {
+      if (mus[a] >= 0.0) {
+        beg[a] = S.b0[a] + sts[a];
+        end_[a] = S.b1[a];
+        t[a] = 1;
+        S.pos_mask[a] |= 1u << n;
+      } else {
+        beg[a] = S.b1[a] - sts[a];
+        end_[a] = S.b0[a];
+        t[a] = -1;
+      }
+    }
+    const int b = dom == 0 ? 1 : 0, c = dom == 2 ? 1 : 2;
+    RtRay &R = S.rays.r[n];
+    const int lo_b = std::min(beg[b], end_[b]), lo_c = std::min(beg[c], end_[c]);
+    int idx[3];
+    idx[dom] = beg[dom];
+    idx[b] = lo_b;
+    idx[c] = lo_c;
+    R.start = (int)(idx[0] * strides[0] + idx[1] * strides[1] + idx[2] * strides[2]);
+    R.da = (int)(t[dom] * strides[dom]);
+    R.db = (int)strides[b];
+    R.dc = (int)strides[c];
+    R.ua = (int)(t[dom] * sts[dom] * strides[dom]);
+    R.ub = (int)(t[b] * sts[b] * strides[b]);
+    R.uc = (int)(t[c] * sts[c] * strides[c]);
+    R.nb = std::abs(end_[b] - beg[b]) + 1;
+    R.nc = std::abs(end_[c] - beg[c]) + 1;
+    R.nplanes = std::abs(end_[dom] - beg[dom]) + 1;
+    const double cx = cc[b], cy = cc[c];
+    R.w00 = (1.0 - cx) * (1.0 - cy);
+    R.w10 = cx * (1.0 - cy);
+    R.w01 = (1.0 - cx) * cy;
+    R.w11 = cx * cy;
+    R.ds = ds;
+    R.wq = S.w[n];
+    S.max_nodes = std::max(S.max_nodes, R.nb * R.nc);
+    S.max_planes = std::max(S.max_planes, R.nplanes);
+  }
+  return MISOHIP_OK;
+}
+
+int check_rt(const misohip_grid *g, const misohip_rt_config *rt) {
+  if (!g || !rt) return MISOHIP_E_ARG;
+  if (rt->num_rays != 24 && rt->num_rays != 1) return MISOHIP_E_ARG;
+  if (rt->max_iters < 1) return MISOHIP_E_ARG;
+  const double n = (double)g->i_total * g->j_total * g->k_total;
+  if (n >= 2147483647.0) return MISOHIP_E_RANGE;  // int indices within one ray
+  if (!g->dx || !g->dy || !g->dz) return MISOHIP_E_ARG;
+  return MISOHIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
+                     const void *kappa, const void *S, void *rint, void *rint_old, int *iters,
+                     double *max_diff) {
+  if (!ctx || !kappa || !S || !rint || !rint_old) return MISOHIP_E_ARG;
+  if (int rc = check_rt(g, rt)) return rc;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  RtSetup su;
+  if (int rc = setup(ctx, g, rt, su)) return rc;
+  double *I = static_cast<double *>(rint);
+  double *old = static_cast<double *>(rint_old);
+  const double *kap = static_cast<const double *>(kappa);
+  const double *src = static_cast<const double *>(S);
+  const size_t bytes = sizeof(double) * su.nrays * su.ray_stride;
+  const int It = g->i_total, Jt = g->j_total, Kt = g->k_total;
+  int lo[3], cnt[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = su.b0[a];
+    cnt[a] = su.b1[a] - su.b0[a] + 1;
+  }
+  int it = 0;
+  double md = 0.0;
+  for (it = 0; it < rt->max_iters; ++it) {
+    {
+      Timed t_(ctx, MISOHIP_KIND_RT);
+      MISOHIP_TRY(hipMemcpyAsync(old, I, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      for (int a = 2; a >= 0; --a)
+        if (rt->periodic[a] && su.size[a] > 1)
+          MISOHIP_TRY(launch_rt_halo(ctx->stream, I, a, su.b0[a], su.b1[a], It, Jt, Kt,
+                                     su.pos_mask[a], su.ray_stride, su.nrays));
+      for (int a = 0; a < 3; ++a)
+        for (int side = 0; side < 2; ++side) {
+          const int kind = rt->face_kind[2 * a + side];
+          if (rt->periodic[a] || su.size[a] <= 1 || kind == 0) continue;
+          unsigned in_mask = 0u;
+          for (int n = 0; n < su.nrays; ++n) {
+            const double mu = su.mu[a][n];
+            if (side == 0 ? mu > 0.0 : mu < 0.0) in_mask |= 1u << n;
+          }
+          const int t0 = a == 0 ? 1 : 0, t1 = a == 2 ? 1 : 2;
+          MISOHIP_TRY(launch_rt_face(ctx->stream, I, src, a, side == 0 ? su.b0[a] : su.b1[a],
+                                     lo[t0], lo[t1], cnt[t0], cnt[t1], It, Jt, kind,
+                                     rt->face_value[2 * a + side], in_mask, su.ray_stride,
+                                     su.nrays));
+        }
+      for (int p = 0; p < su.max_planes; ++p)
+        MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, su.rays, su.nrays, p,
+                                          su.ray_stride, su.max_nodes));
+      MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo, cnt, It, Jt, su.ray_stride,
+                                    su.nrays, ctx->d_slot + 1));
+    }
+    MISOHIP_TRY(hipMemcpyAsync(ctx->h_slot + 1, ctx->d_slot + 1, sizeof(unsigned),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+    float f;
+    std::memcpy(&f, ctx->h_slot + 1, sizeof(float));
+    md = (double)f;
+    if (md < rt->tolerance) {
+      ++it;
+      break;
+    }
+  }
+  if (iters) *iters = it;
+  if (max_diff) *max_diff = md;
+  return MISOHIP_OK;
+}
+
+int misohip_rt_qrad(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
+                    const void *kappa, const void *S, const void *rint, void *qrad) {
+  if (!ctx || !kappa || !S || !rint || !qrad) return MISOHIP_E_ARG;
+  if (int rc = check_rt(g, rt)) return rc;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  RtSetup su;
+  if (int rc = setup(ctx, g, rt, su)) return rc;
+  int lo[3], cnt[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = su.b0[a];
+    cnt[a] = su.b1[a] - su.b0[a] + 1;
+  }
+  Timed t_(ctx, MISOHIP_KIND_RT);
+  MISOHIP_TRY(hipMemsetAsync(qrad, 0, sizeof(double) * su.ray_stride, ctx->stream));
+  MISOHIP_TRY(launch_rt_qrad(ctx->stream, static_cast<const double *>(rint),
+                             static_cast<const double *>(kappa), static_cast<const double *>(S),
+                             static_cast<double *>(qrad), lo, cnt, g->i_total, g->j_total,
+                             su.ray_stride, su.nrays, su.rays));
+  return MISOHIP_OK;
+}
+
+int misohip_rt_source(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
+                      double gm, double kappa0, void *kappa, void *S) {
+  if (!ctx || !g || !qq || !kappa || !S || !qq->q[0] || !qq->q[7]) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  const long n = (long)g->i_total * g->j_total * g->k_total;
+  Timed t_(ctx, MISOHIP_KIND_RT);
+  MISOHIP_TRY(launch_rt_source(ctx->stream, static_cast<const double *>(qq->q[0]),
+                               static_cast<const double *>(qq->q[7]), static_cast<double *>(kappa),
+                               static_cast<double *>(S), n, gm, kappa0));
+  return MISOHIP_OK;
+}
+
+int misohip_rt_heat(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
+                    const void *qrad, double dt) {
+  if (!ctx || !g || !qq || !qrad || !qq->q[0] || !qq->q[7]) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  Shape sh;
+  sh.It = g->i_total; sh.Jt = g->j_total; sh.Kt = g->k_total;
+  sh.mi = g->i_margin; sh.mj = g->j_margin; sh.mk = g->k_margin;
+  sh.is = g->is; sh.js = g->js; sh.ks = g->ks;
+  Timed t_(ctx, MISOHIP_KIND_RT);
+  MISOHIP_TRY(launch_rt_heat(ctx->stream, static_cast<const double *>(qq->q[0]),
+                             static_cast<double *>(qq->q[7]), static_cast<const double *>(qrad),
+                             sh, dt));
+  return MISOHIP_OK;
+}
+
+}  // extern "C"

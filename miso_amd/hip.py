@@ -46,6 +46,11 @@ class MisohipTiles(C.Structure):
     _fields_ = [("rk_pencil_bx", C.c_int), ("rk_pencil_by", C.c_int), ("sweep_chunk", C.c_int)]
 
 
+class MisohipRtConfig(C.Structure):
+    _fields_ = [("num_rays", C.c_int), ("periodic", C.c_int * 3), ("face_kind", C.c_int * 6),
+                ("face_value", C.c_double * 6), ("tolerance", C.c_double), ("max_iters", C.c_int)]
+
+
 class MisohipWork(C.Structure):
     _fields_ = [("argm", MisohipFields), ("rslt", MisohipFields), ("cc", C.c_void_p)]
 
@@ -75,13 +80,20 @@ SIGNATURES = {
                        _P(MisohipFields), _P(MisohipWork), C.c_double],
     "misohip_step": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
                      _P(MisohipFields), _P(MisohipWork), _P(C.c_double)],
+    "misohip_rt_solve": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,
+                         C.c_void_p, C.c_void_p, _P(C.c_int), _P(C.c_double)],
+    "misohip_rt_qrad": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,
+                        C.c_void_p, C.c_void_p],
+    "misohip_rt_source": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_double, C.c_double,
+                          C.c_void_p, C.c_void_p],
+    "misohip_rt_heat": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_void_p, C.c_double],
     "misohip_set_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_get_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_profile_enable": [C.c_void_p, C.c_int],
     "misohip_profile_read": [C.c_void_p, _P(C.c_double), _P(C.c_int), C.c_int],
 }
 
-KIND_NAMES = ("rk_stage", "av_x", "av_y", "av_z", "ghost_fill", "cfl")
+KIND_NAMES = ("rk_stage", "av_x", "av_y", "av_z", "ghost_fill", "cfl", "rt", "mlp")
 
 _lib = None
 

@@ -1,0 +1,82 @@
+"""Python host mirror of MISO's rt::RT (miso/include/miso/rt_host.hpp:104-689)
+over the GPU solver in libmisohip.so.
+
+RT(grid, num_rays) owns the device arrays of the reference's struct --
+rint and rint_old (num_rays fields), src_func and abs_coeff -- laid out
+x-fastest like the MHD fields.  solve() is RT::solve with the boundary set
+given per face (periodic axes are self-exchanged); qrad() is the radiative
+heating rate Q_rad = 4 pi kappa (J - S) this backend adds.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import hip
+
+
+class RT:
+    def __init__(self, grid, num_rays: int = 24):
+        import torch
+        assert grid.dtype == np.float64, "the RT solver runs in FP64"
+        self.grid = grid
+        self.num_rays = 24 if num_rays == 24 else 1
+        shp = (grid.k_total, grid.j_total, grid.i_total)
+        dev = grid.device
+        self.rint = torch.zeros((self.num_rays,) + shp, dtype=torch.float64, device=dev)
+        self.rint_old = torch.zeros_like(self.rint)
+        self.src_func = torch.zeros(shp, dtype=torch.float64, device=dev)
+        self.abs_coeff = torch.zeros(shp, dtype=torch.float64, device=dev)
+        self.qrad_field = torch.zeros(shp, dtype=torch.float64, device=dev)
+
+    def config(self, periodic=(True, True, False), face_kind=(0, 0, 0, 0, 1, 2),
+               face_value=(0.0,) * 6, tolerance=1e-5, max_iters=100) -> hip.MisohipRtConfig:
+        c = hip.MisohipRtConfig()
+        c.num_rays = self.num_rays
+        for a in range(3):
+            c.periodic[a] = 1 if periodic[a] else 0
+        for f in range(6):
+            c.face_kind[f] = int(face_kind[f])
+            c.face_value[f] = float(face_value[f])
+        c.tolerance = float(tolerance)
+        c.max_iters = int(max_iters)
+        return c
+
+    def solve(self, ctx, **kw):
+        """RT::solve; returns (iterations, last max change)."""
+        cfg = self.config(**kw)
+        it, md = C.c_int(), C.c_double()
+        hip.call("misohip_rt_solve", ctx.handle, C.byref(self.grid.c_struct()), C.byref(cfg),
+                 C.c_void_p(self.abs_coeff.data_ptr()), C.c_void_p(self.src_func.data_ptr()),
+                 C.c_void_p(self.rint.data_ptr()), C.c_void_p(self.rint_old.data_ptr()),
+                 C.byref(it), C.byref(md))
+        return it.value, md.value
+
+    def qrad(self, ctx, **kw):
+        cfg = self.config(**kw)
+        hip.call("misohip_rt_qrad", ctx.handle, C.byref(self.grid.c_struct()), C.byref(cfg),
+                 C.c_void_p(self.abs_coeff.data_ptr()), C.c_void_p(self.src_func.data_ptr()),
+                 C.c_void_p(self.rint.data_ptr()), C.c_void_p(self.qrad_field.data_ptr()))
+        return self.qrad_field
+
+    def source_from_fields(self, ctx, qq, gm: float, kappa0: float) -> None:
+        """kappa = kappa0 ro T^3, S = T^4 with T = (gm-1) ei, every cell."""
+        hip.call("misohip_rt_source", ctx.handle, C.byref(self.grid.c_struct()),
+                 C.byref(qq.c_struct()), float(gm), float(kappa0),
+                 C.c_void_p(self.abs_coeff.data_ptr()), C.c_void_p(self.src_func.data_ptr()))
+
+    def heat(self, ctx, qq, dt: float) -> None:
+        """Operator-split radiative heating ei += dt*Q_rad/ro (interior)."""
+        hip.call("misohip_rt_heat", ctx.handle, C.byref(self.grid.c_struct()),
+                 C.byref(qq.c_struct()), C.c_void_p(self.qrad_field.data_ptr()), float(dt))
+
+    # host <-> device helpers in MISO order
+    @staticmethod
+    def to_dev_field(a: np.ndarray, device):
+        import torch
+        return torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(device)
+
+    @staticmethod
+    def to_host_field(t) -> np.ndarray:
+        return np.ascontiguousarray(t.permute(2, 1, 0).cpu().numpy())

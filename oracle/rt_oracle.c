@@ -75,7 +75,12 @@ static rtg rt_make(const og_grid *g) {
   r.b0[0] = g->i_margin - g->is;
   r.b0[1] = g->j_margin - g->js;
   r.b0[2] = g->k_margin - g->ks;
-  for (int a = 0; a < 3; ++a) r.b1[a] = r.b0[a] + r.size[a];
+  /* rt_host.hpp:146-154 sets b1 = b0 + size on every axis, which on an axis
+   * of one cell addresses node 1 of a length-1 axis (it aliases the next row
+   * and overruns the array on the last ray); the oracle -- like the GPU
+   * solver -- keeps the single node 0 there.  Axes with more than one cell
+   * are unchanged. */
+  for (int a = 0; a < 3; ++a) r.b1[a] = r.b0[a] + (r.size[a] > 1 ? r.size[a] : 0);
   return r;
 }
 

@@ -1,0 +1,130 @@
+"""Radiative transfer: pins of the RT oracle (CPU) and parity of the GPU
+solver with it (GPU).
+
+MISO has no RT value tests; its demo is the 2D searchlight
+(demo/rt2d_searchlight).  Pins: the reference's own quadrature header
+(oracle/_ref), closed forms (uniform thermal medium gives I = S and
+Q_rad = 0; a single ray through a source-free absorber decays as
+exp(-kappa s) exactly in the short-characteristics integral).
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def _bounds():
+    return ((-0.5, 0.5),) * 3
+
+
+def test_quadrature_matches_reference_header():
+    from tests.test_oracle_pins import lib_ref
+    L = lib_ref()
+    L.ref_quadrature_f64.argtypes = [C.c_int, C.POINTER(C.c_double)]
+    L.ref_quadrature_f64.restype = C.c_int
+    out = np.zeros(96)
+    n = L.ref_quadrature_f64(24, out.ctypes.data_as(C.POINTER(C.c_double)))
+    ref = out[:4 * n].reshape(n, 4)
+    w, mx, my, mz = O.rt_quadrature(24)
+    assert np.array_equal(ref[:, 0], w) and np.array_equal(ref[:, 1], mx)
+    assert np.array_equal(ref[:, 2], my) and np.array_equal(ref[:, 3], mz)
+    # Carlson A4: unit directions, weights sum to one, every octant once per group
+    assert np.allclose(mx ** 2 + my ** 2 + mz ** 2, 1.0, rtol=0, atol=1e-15)
+    assert abs(w.sum() - 1.0) < 1e-15
+
+
+def test_uniform_thermal_medium():
+    """kappa, S uniform with thermal (I = S) inflow on every face: I = S on
+    every swept node and Q_rad = 0."""
+    g = O.make_grid((12, 10, 8), 2, bounds=_bounds())
+    kap = np.full(g.shape, 2.0)
+    S = np.full(g.shape, 0.7)
+    I = np.zeros((24,) + g.shape)
+    it, md = O.rt_solve(g, kap, S, I, periodic=(0, 0, 0), face_kind=(1,) * 6, tolerance=1e-12,
+                        max_iters=20)
+    q = O.rt_qrad(g, kap, S, I)
+    lat = (slice(1, -2), slice(1, -2), slice(1, -2))
+    assert np.allclose(I[(slice(None),) + lat], 0.7, rtol=1e-14)
+    assert np.abs(q[lat]).max() < 1e-12
+
+
+def test_single_ray_absorption_is_exponential():
+    """One +z ray (the non-A4 quadrature), S = 0, uniform kappa, I = 1 on
+    the inner z face: I(z_n) = exp(-kappa n dz) exactly per segment."""
+    g = O.make_grid((1, 1, 32), 2, bounds=_bounds())
+    kap = np.full(g.shape, 3.0)
+    S = np.zeros(g.shape)
+    I = np.zeros((1,) + g.shape)
+    O.rt_solve(g, kap, S, I, periodic=(0, 0, 0), face_kind=(0, 0, 0, 0, 3, 0),
+               face_value=(0, 0, 0, 0, 1.0, 0), tolerance=1e-14, max_iters=5, num_rays=1)
+    dz = g.axis("dz")[0]
+    n = np.arange(0, 33)
+    expect = np.exp(-3.0 * dz * n)
+    assert np.allclose(I[0, 0, 0, 1:34], expect, rtol=1e-13)
+
+
+# ------------------------------------------------------------------ GPU parity
+def _gpu_rt(ctx, og, kap, S, num_rays, **kw):
+    import torch
+
+    from miso_amd import mhd
+    from miso_amd.rt import RT
+    sizes = (og.g.i_size, og.g.j_size, og.g.k_size)
+    g = mhd.Grid(*sizes, 2, bounds=_bounds(), device="cuda:0")
+    rt = RT(g, num_rays)
+    rt.abs_coeff.copy_(RT.to_dev_field(kap, "cuda:0"))
+    rt.src_func.copy_(RT.to_dev_field(S, "cuda:0"))
+    it, md = rt.solve(ctx, **kw)
+    q = rt.qrad(ctx, **kw)
+    torch.cuda.synchronize()
+    I = rt.rint.permute(0, 3, 2, 1).cpu().numpy()
+    return it, md, I, RT.to_host_field(q)
+
+
+@pytest.mark.gpu
+def test_gpu_searchlight_2d(ctx):
+    """demo/rt2d_searchlight set-up: 64x64, kappa = 1, S = 0.5, a beam of
+    radius 0.2 entering through the inner x face, y and z periodic."""
+    og = O.make_grid((64, 64, 1), 2, bounds=_bounds())
+    kap = np.ones(og.shape)
+    S = np.full(og.shape, 0.5)
+    # beam: incoming intensity 1 inside r <= 0.2 on the x-inner face; the
+    # constant-face boundary is the searchlight's top hat through S at the face
+    I_ref = np.zeros((24,) + og.shape)
+    kw = dict(periodic=(0, 1, 1), face_kind=(3, 0, 0, 0, 0, 0), face_value=(1.0, 0, 0, 0, 0, 0),
+              tolerance=1e-5, max_iters=100)
+    it_ref, md_ref = O.rt_solve(og, kap, S, I_ref, **kw)
+    q_ref = O.rt_qrad(og, kap, S, I_ref)
+    it, md, I, q = _gpu_rt(ctx, og, kap, S, 24, periodic=(False, True, True), face_kind=kw["face_kind"],
+                           face_value=kw["face_value"], tolerance=1e-5, max_iters=100)
+    assert it == it_ref
+    assert abs(md - md_ref) <= 1e-6 * md_ref + 1e-12
+    assert np.allclose(I, I_ref, rtol=1e-13, atol=1e-15)
+    assert np.allclose(q, q_ref, rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.gpu
+def test_gpu_stratified_3d(ctx):
+    """Stratified box: x, y periodic, thermal inflow at the bottom, vacuum at
+    the top, kappa and S varying in all three directions."""
+    og = O.make_grid((16, 12, 20), 2, bounds=_bounds())
+    rng = np.random.default_rng(7)
+    z = og.axis("z")[None, None, :]
+    T = 2.0 - 1.5 * (z + 0.5) + 0.02 * rng.standard_normal(og.shape)
+    ro = np.exp(-(z + 0.5) / 0.1) * (1 + 0.02 * rng.standard_normal(og.shape))
+    kap = 50.0 * ro * T ** 3
+    S = T ** 4
+    I_ref = np.zeros((24,) + og.shape)
+    kw = dict(periodic=(1, 1, 0), face_kind=(0, 0, 0, 0, 1, 2), tolerance=1e-9, max_iters=300)
+    it_ref, md_ref = O.rt_solve(og, kap, S, I_ref, **kw)
+    q_ref = O.rt_qrad(og, kap, S, I_ref)
+    it, md, I, q = _gpu_rt(ctx, og, kap, S, 24, periodic=(True, True, False),
+                           face_kind=kw["face_kind"], tolerance=1e-9, max_iters=300)
+    assert it == it_ref, (it, it_ref)
+    scale = np.abs(I_ref).max()
+    assert np.abs(I - I_ref).max() <= 1e-12 * scale
+    lat = (slice(1, -2), slice(1, -2), slice(1, -2))
+    assert np.abs(q[lat] - q_ref[lat]).max() <= 1e-11 * np.abs(q_ref[lat]).max()
