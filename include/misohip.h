@@ -204,6 +204,44 @@ int misohip_rt_source(misohip_ctx *ctx, const misohip_grid *g, const misohip_fie
 int misohip_rt_heat(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
                     const void *qrad, double dt);
 
+/* ---------------------------------------------------------------------
+ * Per-column MLP surrogate of the radiative heating (no counterpart in the
+ * reference).  Network n_in -> n_hidden (x n_hidden_layers, SiLU) -> n_out,
+ * weights stored [out][in] as FP32 masters with BF16 copies; GEMMs run on
+ * the bf16 MFMA units with FP32 accumulation.  X is bf16 [batch][n_in],
+ * Y and T fp32 [batch][n_out].  All of n_in, n_hidden, n_out and max_batch
+ * must be multiples of 32.  The handle owns its device memory. */
+typedef struct misohip_mlp misohip_mlp;
+int misohip_mlp_create(misohip_ctx *ctx, int n_in, int n_hidden, int n_hidden_layers, int n_out,
+                       int max_batch, misohip_mlp **out);
+int misohip_mlp_destroy(misohip_mlp *m);
+/* set layer weights (host, [out][in]) and biases; resets the Adam moments */
+int misohip_mlp_set_layer(misohip_ctx *ctx, misohip_mlp *m, int layer, const float *w_host,
+                          const float *b_host);
+/* which: 0 weights, 1 biases, 2 weight gradients, 3 bias gradients */
+int misohip_mlp_get_layer(misohip_ctx *ctx, misohip_mlp *m, int layer, int which, float *host);
+/* feature and target normalisation (host arrays of n_in and n_out) */
+int misohip_mlp_set_norm(misohip_ctx *ctx, misohip_mlp *m, const float *mu_in, const float *sd_in,
+                         const float *mu_out, const float *sd_out);
+int misohip_mlp_forward(misohip_ctx *ctx, misohip_mlp *m, const void *X, int batch, float *Y);
+/* forward + MSE loss + backward into the gradient buffers */
+int misohip_mlp_grad(misohip_ctx *ctx, misohip_mlp *m, const void *X, const float *T, int batch,
+                     double *loss);
+int misohip_mlp_adam(misohip_ctx *ctx, misohip_mlp *m, float lr, float beta1, float beta2,
+                     float eps, float weight_decay);
+/* column features of a radiative box (columns along z, i fastest):
+ * normalised (ln ro, ln T, |B|^2) of the n_in/3 interior cells of each
+ * column; FP64 fields.  Xf (optional) receives the fp32 values. */
+int misohip_mlp_features(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g,
+                         const misohip_fields *qq, double gm, void *X, float *Xf);
+/* Q_rad on every interior cell from the network; flagged columns take q_rt */
+int misohip_mlp_qrad(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, const void *X,
+                     float *Y, const int *flags, const void *q_rt, void *qrad);
+/* residual indicator: flag columns whose network Q_rad differs from q_rt by
+ * more than tol in relative L2 over the column */
+int misohip_mlp_residual(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, const float *Y,
+                         const void *q_rt, int *flags, double tol);
+
 /* Launch shapes of the stencil kernels (no counterpart in the reference,
  * whose CUDA kernels use fixed 8x8x8 blocks, cuda_util.cuh:35-60).
  * rk_pencil_bx/by: pencil of the RK-stage block order, in blocks of 64x4

@@ -87,6 +87,24 @@ SIGNATURES = {
     "misohip_rt_source": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_double, C.c_double,
                           C.c_void_p, C.c_void_p],
     "misohip_rt_heat": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_void_p, C.c_double],
+    "misohip_mlp_create": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                           _P(C.c_void_p)],
+    "misohip_mlp_destroy": [C.c_void_p],
+    "misohip_mlp_set_layer": [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
+    "misohip_mlp_get_layer": [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p],
+    "misohip_mlp_set_norm": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p],
+    "misohip_mlp_forward": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p],
+    "misohip_mlp_grad": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                         _P(C.c_double)],
+    "misohip_mlp_adam": [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float,
+                         C.c_float],
+    "misohip_mlp_features": [C.c_void_p, C.c_void_p, _P(MisohipGrid), _P(MisohipFields),
+                             C.c_double, C.c_void_p, C.c_void_p],
+    "misohip_mlp_qrad": [C.c_void_p, C.c_void_p, _P(MisohipGrid), C.c_void_p, C.c_void_p,
+                         C.c_void_p, C.c_void_p, C.c_void_p],
+    "misohip_mlp_residual": [C.c_void_p, C.c_void_p, _P(MisohipGrid), C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_double],
     "misohip_set_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_get_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_profile_enable": [C.c_void_p, C.c_int],
