@@ -45,6 +45,21 @@ def parse():
     return ap.parse_args()
 
 
+def rk_traffic(n, dtype):
+    """HBM bytes per rk_stage launch (FETCH_SIZE + WRITE_SIZE, scaled by a
+    plain copy's counters) from the committed rocprofv3 summary of this
+    workload, or None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"mhd_step_turbulence_{n}cubed_{dtype}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cols = line.split("|")
+        if line.startswith("| `rk_stage_kernel") and len(cols) >= 10:  # the traffic table
+            return float(cols[7]) * 1e6
+    return None
+
+
 def calibrate_copy(nbytes=2 << 30, reps=10):
     """A plain copy kernel of known size: reads and writes `nbytes` per launch."""
     import torch
@@ -192,7 +207,8 @@ def main():
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": (rk_achieved / PEAK_HBM) if rk else None,
-            "traffic": None,
+            "traffic": rk_traffic(n, args.dtype),
+            "traffic_source": f"profiles/mhd_step_turbulence_{n}cubed_{args.dtype}.md",
         },
         "kernels": kernels,
         "dt_last": dts[-1],

@@ -58,3 +58,21 @@ def loss_and_grads(Ws, bs, X, T, bf16=True):
             dact = d @ r(Ws[l])
             d = r(dact * dsilu(pres[l - 1]))
     return loss, gWs, gbs
+
+
+def adam_step(params, grads, state, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    """One Adam step (bias-corrected, weight decay added to the gradient) on
+    a list of arrays, in place; `state` is a dict holding t, m, v."""
+    if not state:
+        state.update(t=0, m=[np.zeros_like(p) for p in params],
+                     v=[np.zeros_like(p) for p in params])
+    state["t"] += 1
+    c1 = 1.0 - b1 ** state["t"]
+    c2 = 1.0 - b2 ** state["t"]
+    for p, g, m, v in zip(params, grads, state["m"], state["v"]):
+        g = g + wd * p
+        m *= b1
+        m += (1.0 - b1) * g
+        v *= b2
+        v += (1.0 - b2) * g * g
+        p -= lr * (m / c1) / (np.sqrt(v / c2) + eps)

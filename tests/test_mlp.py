@@ -152,33 +152,59 @@ def test_gpu_gemm_odd_shapes(ctx):
     net.close()
 
 
+def _oracle_training(Ws, bs, Xs, Ts, epochs, batch, lr):
+    Ws = [w.astype(np.float32).astype(np.float64) for w in Ws]
+    bs = [b.astype(np.float32).astype(np.float64) for b in bs]
+    st, losses = {}, []
+    for _ in range(epochs):
+        for b0 in range(0, len(Xs), batch):
+            loss, gW, gb = MO.loss_and_grads(Ws, bs, Xs[b0:b0 + batch], Ts[b0:b0 + batch])
+            MO.adam_step(Ws + bs, gW + gb, st, lr)
+            losses.append(loss)
+    return losses
+
+
+def test_oracle_adam_single_step_closed_form():
+    # first Adam step moves every parameter by lr * sign(g) (m/c1 = g,
+    # sqrt(v/c2) = |g|), up to eps
+    p = [np.array([1.0, -2.0, 3.0])]
+    g = [np.array([0.5, -1e-3, 7.0])]
+    MO.adam_step(p, g, {}, 0.01, eps=0.0)
+    assert np.allclose(p[0], [0.99, -1.99, 2.99], rtol=0, atol=1e-15)
+
+
 @pytest.mark.gpu
-def test_gpu_training_reduces_loss(ctx):
-    """Adam on the GPU fits a smooth synthetic map (a stand-in column
-    problem) and lowers the loss by a large factor."""
+def test_gpu_training_tracks_oracle(ctx):
+    """Adam on the GPU (bf16 GEMMs, fp32 master weights) follows the float64
+    oracle's training curve on a smooth synthetic map."""
     import torch
     sizes = [96, 128, 128, 128, 64]
     Ws, bs = _params(sizes, 7)
     for w in Ws:
         w *= 0.5
     rng = np.random.default_rng(8)
-    n = 4096
-    Xs = rng.standard_normal((n, sizes[0]))
+    n, batch, epochs, lr = 4096, 1024, 30, 2e-3
+    Xs = MO.to_bf16(rng.standard_normal((n, sizes[0])))
     A = rng.standard_normal((sizes[0], sizes[-1])) / np.sqrt(sizes[0])
-    Ts = np.tanh(Xs @ A)
-    net = GpuMLP(ctx, sizes, 1024)
+    Ts = np.tanh(Xs @ A).astype(np.float32).astype(np.float64)
+    net = GpuMLP(ctx, sizes, batch)
     net.set(Ws, bs)
     Xd = _bf16_dev(Xs)
     Td = torch.from_numpy(Ts.astype(np.float32)).to("cuda:0")
     losses = []
-    for ep in range(60):
-        for b0 in range(0, n, 1024):
+    for ep in range(epochs):
+        for b0 in range(0, n, batch):
             loss = C.c_double()
             net.hip.call("misohip_mlp_grad", ctx.handle, net.h,
-                         C.c_void_p(Xd[b0:b0 + 1024].data_ptr()),
-                         C.c_void_p(Td[b0:b0 + 1024].data_ptr()), 1024, C.byref(loss))
-            net.hip.call("misohip_mlp_adam", ctx.handle, net.h, C.c_float(2e-3), C.c_float(0.9),
+                         C.c_void_p(Xd[b0:b0 + batch].data_ptr()),
+                         C.c_void_p(Td[b0:b0 + batch].data_ptr()), batch, C.byref(loss))
+            net.hip.call("misohip_mlp_adam", ctx.handle, net.h, C.c_float(lr), C.c_float(0.9),
                          C.c_float(0.999), C.c_float(1e-8), C.c_float(0.0))
             losses.append(loss.value)
-    assert losses[-1] < 0.05 * losses[0], (losses[0], losses[-1])
     net.close()
+    ref = _oracle_training(Ws, bs, Xs, Ts, epochs, batch, lr)
+    assert losses[0] == pytest.approx(ref[0], rel=1e-3)
+    assert losses[-1] < 0.2 * losses[0]
+    # bf16 rounding makes the curves drift apart slowly; they stay close
+    for k in (3, 15, 60, len(ref) - 1):
+        assert abs(losses[k] - ref[k]) <= 0.1 * ref[k] + 1e-4, (k, losses[k], ref[k])
