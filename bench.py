@@ -8,6 +8,12 @@ CFL reduction, then Integrator::update (4 RK stages + 3 AV sweeps, each
 followed by the in-device ghost fill).  Fields are resident in HBM before the
 timed region starts.
 
+Then the radiative-MHD leg (`rmhd`): steps/s of the stratified radiative box
+with a plain GPU RT solve on every step (256^3 and 384^3) and with the
+learned RT (384^3: the per-column network on every step, a plain solve every
+16 steps and on flagged columns), the network trained in the same run on
+columns of the GPU RT solver, and its Q_rad error against converged solves.
+
 Prints ONE JSON line.  `roofline` is for the dominant kernel (the fused RK
 stage), from HIP events recorded on the library's stream around every launch
 of the timed region; `cpu_baseline` times the CPU oracle (a C restatement of
@@ -39,6 +45,17 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rmhd", action="store_true", help="skip the radiative-MHD leg")
+    ap.add_argument("--rmhd-only", action="store_true", help="run only the radiative-MHD leg")
+    ap.add_argument("--rmhd-n", type=int, default=384, help="learned-RT box, cells per side")
+    ap.add_argument("--rmhd-plain-n", type=int, default=256, help="plain-RT box, cells per side")
+    ap.add_argument("--rmhd-plain-steps", type=int, default=8)
+    ap.add_argument("--rmhd-cycles", type=int, default=3,
+                    help="timed learned-RT cycles of rt_every steps")
+    ap.add_argument("--rmhd-rt-every", type=int, default=16)
+    ap.add_argument("--rmhd-flag-tol", type=float, default=0.03)
+    ap.add_argument("--rmhd-train-columns", type=int, default=100000)
+    ap.add_argument("--rmhd-epochs", type=int, default=40)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of known size (HBM reference for the "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -113,6 +130,127 @@ def cpu_baseline(target_s: float):
     }
 
 
+def rmhd_leg(args):
+    """Radiative MHD on the stratified box: plain GPU RT on every step
+    (args.rmhd_plain_n^3 and args.rmhd_n^3) against the learned RT
+    (args.rmhd_n^3: the column network on every step, a plain solve every
+    rt_every steps and on flagged columns).  The network is trained here on
+    columns the GPU RT solver produces for boxes with other seeds; its
+    Q_rad is checked against converged RT solves of the same states."""
+    import numpy as np
+    import torch
+
+    from miso_amd.mhd import ExecContext, Grid
+    from miso_amd.rmhd import RadiativeMHD, rel_l2
+    from miso_amd.surrogate import ColumnSurrogate, collect_columns
+
+    dev = "cuda:0"
+    ctx = ExecContext(0)
+
+    def log(s):
+        print(f"[rmhd] {s}", file=sys.stderr, flush=True)
+
+    def make_box(n, seed):
+        g = Grid(n, n, n, bounds=((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), device=dev)
+        b = RadiativeMHD(g, ctx)
+        b.initialize(seed)
+        return b
+
+    def time_plain(n, steps, seed):
+        box = make_box(n, seed)
+        for _ in range(2):
+            box.step_plain()
+        torch.cuda.synchronize()
+        i0 = len(box.rt_iters)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            box.step_plain()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        r = {"cells": n ** 3, "steps": steps, "ms_per_step": 1e3 * el / steps,
+             "steps_per_s": steps / el, "rt_iterations_mean": float(np.mean(box.rt_iters[i0:]))}
+        log(f"plain RT {n}^3: {r}")
+        del box
+        torch.cuda.empty_cache()
+        return r
+
+    out = {"workload": "stratified radiative MHD box, FP64 fields, T 2 -> 0.5, scale height "
+                       "0.1 box, kappa = kappa0 ro T^3 (tau = 1 at mid-height), 24 rays, "
+                       "1e-2 seeded v and B perturbations, x/y periodic, fixed z ghosts"}
+    out["plain_rt_small"] = time_plain(args.rmhd_plain_n, args.rmhd_plain_steps, seed=7)
+    n = args.rmhd_n
+    out["plain_rt"] = time_plain(n, args.rmhd_plain_steps, seed=7)
+    every = args.rmhd_rt_every
+    nsteps = every * args.rmhd_cycles
+    # training data from other seeds, over the same span of steps
+    t0 = time.perf_counter()
+    data_box = make_box(n, 101)
+    F, T = collect_columns(data_box, seeds=(101, 102), steps=nsteps + every,
+                           sample_every=4, max_columns=args.rmhd_train_columns)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    del data_box
+    torch.cuda.empty_cache()
+    log(f"training data: {F.shape[0]} columns in {gen_s:.1f} s")
+    sur = ColumnSurrogate(ctx, n, hidden=256, hidden_layers=3, max_batch=n * n, seed=0)
+    t0 = time.perf_counter()
+    mse, val = sur.fit(F, T, epochs=args.rmhd_epochs, batch=1024, lr=1e-3)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    ncols = int(F.shape[0])
+    del F, T
+    torch.cuda.empty_cache()
+    log(f"trained: mse {mse:.3e}, validation rel L2 {val:.3e}, {train_s:.1f} s")
+    # timed learned run: one untimed cycle, then args.rmhd_cycles cycles
+    box = make_box(n, 7)
+    box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol)
+    for _ in range(every):
+        box.step_learned()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        box.step_learned()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    learned = {"cells": n ** 3, "steps": nsteps, "rt_every": every,
+               "ms_per_step": 1e3 * el / nsteps, "steps_per_s": nsteps / el,
+               "flagged_column_fraction": float(box.flags.float().mean())}
+    del box
+    torch.cuda.empty_cache()
+    # accuracy: the same run again, Q_rad checked against converged solves
+    box = make_box(n, 7)
+    box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol)
+    errs = {"learned": [], "network_only": [], "stale_rt": []}
+
+    def probe(q, solved):
+        s = box.steps_done
+        if solved or s % 3:
+            return
+        truth = box.reference_qrad()
+        errs["learned"].append(rel_l2(q, truth, box.grid))
+        errs["stale_rt"].append(rel_l2(box.q_rt, truth, box.grid))
+        tmp = torch.empty_like(q)
+        sur.predict(box.grid, box.model.qq, box.strat.gm, tmp)
+        errs["network_only"].append(rel_l2(tmp, truth, box.grid))
+
+    for _ in range(nsteps + every):
+        box.step_learned(probe)
+    for k, v in errs.items():
+        learned[f"qrad_rel_l2_{k}"] = {"mean": float(np.mean(v)), "max": float(np.max(v)),
+                                       "checks": len(v)}
+    learned["training"] = {"columns": ncols, "epochs": args.rmhd_epochs, "train_mse": mse,
+                           "validation_rel_l2": val, "data_seconds": gen_s,
+                           "train_seconds": train_s}
+    log(f"learned RT {n}^3: {learned}")
+    out["learned_rt"] = learned
+    out["speedup_learned_vs_plain"] = learned["steps_per_s"] / out["plain_rt"]["steps_per_s"]
+    sur.close()
+    del box
+    ctx.close()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -121,6 +259,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.rmhd_only:
+        print(json.dumps({"rmhd": rmhd_leg(args)}), flush=True)
         return
     import numpy as np
     import torch
@@ -216,6 +357,8 @@ def main():
     ctx.close()
     del m
     torch.cuda.empty_cache()
+    if not args.no_rmhd:
+        out["rmhd"] = rmhd_leg(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -75,9 +75,12 @@ typedef struct misohip_fields {
   void *q[9];
 } misohip_fields;
 
-/* Boundary description.  A periodic axis is filled by the one-rank halo
- * self-exchange; a non-periodic axis gets the mirror boundary of
- * boundary_condition.hpp:108-136 on both faces with sign sym_sign[axis][f]. */
+/* Boundary description.  periodic[a] = 1: the axis is filled by the
+ * one-rank halo self-exchange; 0: the mirror boundary of
+ * boundary_condition.hpp:108-136 on both faces with sign sym_sign[axis][f];
+ * 2: fixed, the ghost cells of the axis keep their values (a user boundary
+ * functor that does nothing on that axis, as EmptyBoundaryCondition,
+ * mhd_model_base.hpp:125; edges shared with a periodic axis still wrap). */
 typedef struct misohip_bc {
   int periodic[3];
   int sym_sign[3][9];
@@ -194,9 +197,11 @@ int misohip_rt_qrad(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_co
                     const void *kappa, const void *S, const void *rint, void *qrad);
 
 /* Grey opacity and source function of the radiative-MHD workload from the
- * fields, on every cell: T = (gm - 1) ei, kappa = kappa0 ro T^3, S = T^4. */
+ * fields, on every cell: T = (gm - 1) ei, kappa = kappa0 ro T^3,
+ * S = s_rad T^4 (s_rad: the Stefan-Boltzmann constant over pi in code
+ * units, which sets the radiative relaxation time against the gas energy). */
 int misohip_rt_source(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
-                      double gm, double kappa0, void *kappa, void *S);
+                      double gm, double kappa0, double s_rad, void *kappa, void *S);
 
 /* Radiative heating as an operator-split update after Integrator::update,
  * interior cells: ei += dt * qrad / ro.  (MISO's Source::ei is added outside

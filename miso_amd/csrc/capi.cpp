@@ -79,7 +79,7 @@ template <typename R> Fld<const R> cfld(const misohip_fields *f) {
 BcDev make_bc(const misohip_bc *bc) {
   BcDev b;
   for (int a = 0; a < 3; ++a) {
-    b.periodic[a] = bc->periodic[a] ? 1 : 0;
+    b.periodic[a] = bc->periodic[a] == 2 ? 2 : (bc->periodic[a] ? 1 : 0);
     for (int f = 0; f < 9; ++f) b.sign[a][f] = bc->sym_sign[a][f] >= 0 ? 1.0f : -1.0f;
   }
   return b;

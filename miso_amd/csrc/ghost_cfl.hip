@@ -8,7 +8,9 @@
 // Each coordinate is mapped on its own (periodic wrap or mirror), so faces
 // carry exactly the values the reference's exchange produces and edges and
 // corners are well defined (the reference leaves them stale; no interior
-// stencil reads them).
+// stencil reads them).  A fixed axis (a user boundary functor that leaves
+// the faces alone, like EmptyBoundaryCondition, mhd_model_base.hpp:125)
+// keeps its ghost values: the coordinate maps to itself.
 //
 // CFL: Integrator::cfl (mhd_integrator_host.hpp:303-334) narrows every
 // candidate to float through util::min2, so dt = (float)min(candidates).
@@ -19,17 +21,18 @@
 
 namespace misohip {
 
-__device__ __forceinline__ int map_coord(int x, int total, int m, int periodic, bool &mirrored) {
+// kind: 0 mirror, 1 periodic, 2 fixed (the ghost cell is its own source)
+__device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool &mirrored) {
   mirrored = false;
-  if (m == 0) return x;
+  if (m == 0 || kind == 2) return x;
   const int n = total - 2 * m;
   if (x < m) {
-    if (periodic) return x + n;
+    if (kind == 1) return x + n;
     mirrored = true;
     return 2 * m - 1 - x;
   }
   if (x >= total - m) {
-    if (periodic) return x - n;
+    if (kind == 1) return x - n;
     mirrored = true;
     return 2 * (total - m) - 1 - x;
   }

@@ -54,7 +54,7 @@ hipError_t launch_rt_qrad(hipStream_t, const double *, const double *, const dou
                           const int[3], const int[3], int, int, long, int, const RtRays &);
 
 hipError_t launch_rt_source(hipStream_t, const double *, const double *, double *, double *, long,
-                            double, double);
+                            double, double, double);
 hipError_t launch_rt_heat(hipStream_t, const double *, double *, const double *, const Shape &,
                           double);
 

@@ -198,18 +198,18 @@ hipError_t launch_rt_qrad(hipStream_t st, const double *I, const double *kap, co
 
 namespace misohip {
 
-// T = (gm-1) ei, kappa = kappa0 ro T^3, S = T^4 on every cell
+// T = (gm-1) ei, kappa = kappa0 ro T^3, S = s_rad T^4 on every cell
 __global__ __launch_bounds__(256) void rt_source_kernel(const double *__restrict__ ro,
                                                         const double *__restrict__ ei,
                                                         double *__restrict__ kap,
                                                         double *__restrict__ S, long n, double gm,
-                                                        double kappa0) {
+                                                        double kappa0, double s_rad) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const double T = (gm - 1.0) * ei[c];
   const double T2 = T * T;
   kap[c] = kappa0 * ro[c] * (T2 * T);
-  S[c] = T2 * T2;
+  S[c] = s_rad * (T2 * T2);
 }
 
 // ei += dt * q / ro on the interior
@@ -228,9 +228,9 @@ __global__ __launch_bounds__(256) void rt_heat_kernel(const double *__restrict__
 }
 
 hipError_t launch_rt_source(hipStream_t st, const double *ro, const double *ei, double *kap,
-                            double *S, long n, double gm, double kappa0) {
+                            double *S, long n, double gm, double kappa0, double s_rad) {
   hipLaunchKernelGGL(rt_source_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ro, ei,
-                     kap, S, n, gm, kappa0);
+                     kap, S, n, gm, kappa0, s_rad);
   return hipGetLastError();
 }
 

@@ -241,14 +241,14 @@ int misohip_rt_qrad(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_co
 }
 
 int misohip_rt_source(misohip_ctx *ctx, const misohip_grid *g, const misohip_fields *qq,
-                      double gm, double kappa0, void *kappa, void *S) {
+                      double gm, double kappa0, double s_rad, void *kappa, void *S) {
   if (!ctx || !g || !qq || !kappa || !S || !qq->q[0] || !qq->q[7]) return MISOHIP_E_ARG;
   MISOHIP_TRY(hipSetDevice(ctx->device));
   const long n = (long)g->i_total * g->j_total * g->k_total;
   Timed t_(ctx, MISOHIP_KIND_RT);
   MISOHIP_TRY(launch_rt_source(ctx->stream, static_cast<const double *>(qq->q[0]),
                                static_cast<const double *>(qq->q[7]), static_cast<double *>(kappa),
-                               static_cast<double *>(S), n, gm, kappa0));
+                               static_cast<double *>(S), n, gm, kappa0, s_rad));
   return MISOHIP_OK;
 }
 

@@ -85,6 +85,7 @@ SIGNATURES = {
     "misohip_rt_qrad": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,
                         C.c_void_p, C.c_void_p],
     "misohip_rt_source": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_double, C.c_double,
+                          C.c_double,
                           C.c_void_p, C.c_void_p],
     "misohip_rt_heat": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_void_p, C.c_double],
     "misohip_mlp_create": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,

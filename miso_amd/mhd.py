@@ -215,17 +215,22 @@ class ExecContext:
 
 
 class BoundaryCondition:
-    """Per-axis boundary: periodic self-exchange or mirror with per-field signs
-    (boundary_condition.hpp:108-136; signs +1 unless given)."""
+    """Per-axis boundary: periodic self-exchange (True / 1), mirror with
+    per-field signs (False / 0; boundary_condition.hpp:108-136, signs +1
+    unless given) or fixed ghost values ("fixed" / 2; a user functor that
+    leaves the axis alone, mhd_model_base.hpp:125)."""
+
+    FIXED = 2
 
     def __init__(self, periodic=(True, True, True), sym_sign=None):
-        self.periodic = tuple(bool(p) for p in periodic)
+        self.periodic = tuple(2 if (p == "fixed" or (not isinstance(p, bool) and p == 2))
+                              else int(bool(p)) for p in periodic)
         self.sym_sign = sym_sign or ((1,) * 9,) * 3
 
     def c_struct(self) -> hip.MisohipBc:
         b = hip.MisohipBc()
         for a in range(3):
-            b.periodic[a] = 1 if self.periodic[a] else 0
+            b.periodic[a] = self.periodic[a]
             for f in range(9):
                 b.sym_sign[a][f] = int(self.sym_sign[a][f])
         return b

@@ -60,16 +60,19 @@ class RT:
                  C.c_void_p(self.rint.data_ptr()), C.c_void_p(self.qrad_field.data_ptr()))
         return self.qrad_field
 
-    def source_from_fields(self, ctx, qq, gm: float, kappa0: float) -> None:
-        """kappa = kappa0 ro T^3, S = T^4 with T = (gm-1) ei, every cell."""
+    def source_from_fields(self, ctx, qq, gm: float, kappa0: float, s_rad: float = 1.0) -> None:
+        """kappa = kappa0 ro T^3, S = s_rad T^4 with T = (gm-1) ei, every cell."""
         hip.call("misohip_rt_source", ctx.handle, C.byref(self.grid.c_struct()),
-                 C.byref(qq.c_struct()), float(gm), float(kappa0),
+                 C.byref(qq.c_struct()), float(gm), float(kappa0), float(s_rad),
                  C.c_void_p(self.abs_coeff.data_ptr()), C.c_void_p(self.src_func.data_ptr()))
 
-    def heat(self, ctx, qq, dt: float) -> None:
-        """Operator-split radiative heating ei += dt*Q_rad/ro (interior)."""
+    def heat(self, ctx, qq, dt: float, qrad=None) -> None:
+        """Operator-split radiative heating ei += dt*Q_rad/ro (interior), from
+        qrad_field or the given field of the same shape."""
+        q = self.qrad_field if qrad is None else qrad
+        assert q.shape == self.qrad_field.shape and q.dtype == self.qrad_field.dtype
         hip.call("misohip_rt_heat", ctx.handle, C.byref(self.grid.c_struct()),
-                 C.byref(qq.c_struct()), C.c_void_p(self.qrad_field.data_ptr()), float(dt))
+                 C.byref(qq.c_struct()), C.c_void_p(q.data_ptr()), float(dt))
 
     # host <-> device helpers in MISO order
     @staticmethod

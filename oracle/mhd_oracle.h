@@ -34,10 +34,12 @@ typedef struct og_grid {
 
 /* Run parameters: the config keys the step consumes plus the boundary
  * description (miso/include/miso/config_defaults.hpp, eos.hpp,
- * boundary_condition.hpp).  `periodic[a]` selects the halo self-exchange on
- * axis a; a non-periodic axis gets the mirror boundary of
+ * boundary_condition.hpp).  `periodic[a]` = 1 selects the halo
+ * self-exchange on axis a; 0 the mirror boundary of
  * boundary_condition.hpp:108-136 on both sides with per-field sign
- * `sym_sign[a][f]` (+1 / -1).  `src_kind` 1 = constant gravity along
+ * `sym_sign[a][f]` (+1 / -1); 2 leaves the axis' ghost cells alone (a user
+ * boundary functor that does nothing there, like EmptyBoundaryCondition,
+ * mhd_model_base.hpp:125).  `src_kind` 1 = constant gravity along
  * `grav_axis` (the force -g*ro of demo/hd2d_rayleigh_taylor/src/main.cpp:109-112).
  */
 typedef struct og_params {

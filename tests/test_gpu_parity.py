@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from miso_amd import problems
+from miso_amd.rmhd import Stratification, stratified_box
 from tests.parity_util import assert_bitwise, random_state, rel_linf
 
 pytestmark = pytest.mark.gpu
@@ -52,6 +53,8 @@ def _bcs(kind):
     from oracle import oracle as O
     if kind == "periodic":
         return mhd.BoundaryCondition((True, True, True)), dict(periodic=(1, 1, 1))
+    if kind == "fixed_z":
+        return mhd.BoundaryCondition((True, True, "fixed")), dict(periodic=(1, 1, 2))
     # mirror on y and z with antisymmetric normal velocity / field
     signs = [[1] * 9 for _ in range(3)]
     signs[1][2] = signs[1][5] = -1
@@ -75,7 +78,7 @@ def _integrator(ctx, g, **kw):
 
 
 @pytest.mark.parametrize("geom", list(GEOMS))
-@pytest.mark.parametrize("kind", ["periodic", "mirror"])
+@pytest.mark.parametrize("kind", ["periodic", "mirror", "fixed_z"])
 def test_ghost_fill(ctx, geom, kind):
     from oracle import oracle as O
     sizes = GEOMS[geom]
@@ -90,6 +93,11 @@ def test_ghost_fill(ctx, geom, kind):
     out = F.to_host(ctx)
     mask = _face_mask(g)
     assert_bitwise(out[:, mask], ref[:, mask], f"ghost fill {geom}/{kind}")
+    if kind == "fixed_z" and g.k_size > 1:
+        # a fixed axis leaves its ghost planes alone (faces and edges along x/y wrap)
+        ii, jj = slice(g.i_margin, -g.i_margin), slice(g.j_margin, -g.j_margin)
+        for k in list(range(g.k_margin)) + list(range(g.k_total - g.k_margin, g.k_total)):
+            assert np.array_equal(out[:, ii, jj, k], q[:, ii, jj, k])
 
 
 def test_ghost_fill_reference_kat(ctx):
@@ -213,7 +221,7 @@ def _run_both(ctx, problem, sizes, nsteps, dtype=np.float64, periodic=(1, 1, 1),
     ref_dts = O.run(og, op, ref, nsteps)
     cfg = mhd.Config({"mhd": {"cfl_number": cfl, "artificial_viscosity": av or {}}})
     m = mhd.MHD(cfg, g, ctx, gm=gm, **src)
-    bc = mhd.BoundaryCondition(tuple(bool(p) for p in periodic), sym)
+    bc = mhd.BoundaryCondition(tuple(int(p) for p in periodic), sym)
     m.apply_initial_condition(q0.astype(dtype), bc)
     dts = [m.integrator.step(bc, m.qq) for _ in range(nsteps)]
     out = m.qq.to_host(ctx).astype(np.float64)
@@ -228,6 +236,10 @@ STEP_CASES = {
         functools.partial(problems.kelvin_helmholtz, seed=3), (48, 32, 1), 10,
         dict(periodic=(1, 0, 1), gm=1.4, bounds=((-0.5, 0.5), (-0.5, 0.5), (-0.5, 0.5)),
              sym=((1,) * 9, (1, 1, -1, 1, 1, -1, 1, 1, 1), (1,) * 9))),
+    "stratified_box_fixed_z_16x16x32": (
+        lambda g: stratified_box(g, Stratification(), seed=4), (16, 16, 32), 5,
+        dict(periodic=(1, 1, 2), bounds=((0.0, 0.5), (0.0, 0.5), (0.0, 1.0)),
+             src=dict(src_kind=1, grav_axis=2, grav=Stratification().grav))),
     "stratified_gravity_16": (
         functools.partial(problems.turbulence, v_rms=0.05, b_rms=0.05), (16, 16, 16), 5,
         dict(periodic=(1, 1, 0), src=dict(src_kind=1, grav_axis=2, grav=0.5),

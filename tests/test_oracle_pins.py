@@ -191,6 +191,25 @@ def test_ghost_fill_reference_kat():
         assert np.array_equal(q[:, i, j, k], q[:, src[0], src[1], src[2]])
 
 
+def test_ghost_fill_fixed_axis_keeps_its_ghosts():
+    """periodic[a] = 2: the axis' ghost planes keep their values; the x and
+    y halo exchange still wraps the cells they share with those planes."""
+    g = O.make_grid((6, 5, 7), 2)
+    It, Jt, Kt = g.shape
+    q = np.random.default_rng(3).standard_normal((9,) + g.shape)
+    q0 = q.copy()
+    O.ghost_fill(g, O.Params(periodic=(1, 1, 2)), q)
+    ii, jj = slice(2, -2), slice(2, -2)
+    for k in (0, 1, Kt - 2, Kt - 1):
+        assert np.array_equal(q[:, ii, jj, k], q0[:, ii, jj, k])
+        # x-ghost / z-ghost edge: periodic image along x of the kept plane
+        assert np.array_equal(q[:, 0, 3, k], q0[:, It - 4, 3, k])
+        assert np.array_equal(q[:, It - 1, 3, k], q0[:, 3, 3, k])
+    # interior untouched, x faces wrap as with a periodic z
+    assert np.array_equal(q[:, ii, jj, 2:-2], q0[:, ii, jj, 2:-2])
+    assert np.array_equal(q[:, 1, 3, 4], q0[:, It - 3, 3, 4])
+
+
 def sod_exact(x, t, x0, gm, rl, pl, ul, rr, pr, ur):
     """Exact Riemann solution of the Euler equations (Toro, Riemann Solvers
     and Numerical Methods for Fluid Dynamics, ch. 4)."""

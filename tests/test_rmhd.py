@@ -1,0 +1,164 @@
+"""Radiative MHD coupling and the learned RT surrogate: the stratified
+workload's closed forms (CPU), the coupling kernels and column
+features/outputs against numpy (GPU), and the trained surrogate against the
+method without it (GPU)."""
+import math
+
+import numpy as np
+import pytest
+
+from miso_amd.rmhd import Stratification
+
+
+def test_stratification_is_hydrostatic_with_the_stated_scale_height():
+    s = Stratification()
+    z = np.linspace(0.0, 1.0, 20001)
+    p, ro, T = s.pressure(z), s.density(z), s.temperature(z)
+    dpdz = np.gradient(p, z)
+    assert np.max(np.abs(dpdz + s.grav * ro) / (s.grav * ro)) < 1e-3
+    assert T[0] == 2.0 and abs(T[-1] - 0.5) < 1e-15
+    # pressure scale height at the mean temperature is a tenth of the box
+    assert abs(1.25 / s.grav - 0.1) < 1e-15
+    assert abs(p[0] - 2.0) < 1e-15 and abs(ro[0] - 1.0) < 1e-15
+
+
+def test_kappa0_gives_unit_optical_depth_at_mid_height():
+    s = Stratification()
+    nz = 384
+    dz = 1.0 / nz
+    zc = (np.arange(nz) + 0.5) * dz
+    k0 = s.kappa0(zc, dz)
+    up = zc >= 0.5
+    tau = np.sum(k0 * s.density(zc[up]) * s.temperature(zc[up]) ** 3) * dz
+    assert abs(tau - 1.0) < 1e-12
+    # the same integral by quadrature of the closed form agrees to O(dz^2)
+    zf = np.linspace(0.5, 1.0, 200001)
+    f = k0 * s.density(zf) * s.temperature(zf) ** 3
+    tau_exact = float(np.sum(0.5 * (f[1:] + f[:-1]) * np.diff(zf)))
+    assert abs(tau_exact - 1.0) < 1e-2
+    sr = s.s_rad(zc, k0)
+    T, ro = s.temperature(zc), s.density(zc)
+    tc = (T / (s.gm - 1)) * ro / (4 * math.pi * k0 * ro * T ** 3 * sr * T ** 4)
+    assert abs(tc.min() - s.t_relax) < 1e-12 * s.t_relax
+
+
+# ---------------------------------------------------------------------- GPU
+def _box(ctx, n_xy, nz, seed=3, **kw):
+    from miso_amd.mhd import Grid
+    from miso_amd.rmhd import RadiativeMHD
+    lxy = n_xy / nz
+    g = Grid(n_xy, n_xy, nz, bounds=((0.0, lxy), (0.0, lxy), (0.0, 1.0)), device="cuda:0")
+    box = RadiativeMHD(g, ctx, **kw)
+    box.initialize(seed)
+    return box
+
+
+@pytest.mark.gpu
+def test_rt_source_and_heat_match_numpy(ctx):
+    import torch
+    box = _box(ctx, 8, 64)
+    g = box.grid
+    q = box.model.qq.data.cpu().numpy()
+    box.rt.source_from_fields(ctx, box.model.qq, box.strat.gm, box.kappa0, box.s_rad)
+    torch.cuda.synchronize()
+    T = (box.strat.gm - 1.0) * q[7]
+    kap = box.kappa0 * q[0] * T ** 3
+    S = box.s_rad * T ** 4
+    assert np.allclose(box.rt.abs_coeff.cpu().numpy(), kap, rtol=1e-14, atol=0)
+    assert np.allclose(box.rt.src_func.cpu().numpy(), S, rtol=1e-14, atol=0)
+    qr = torch.randn_like(box.rt.qrad_field)
+    before = box.model.qq.data[7].clone()
+    box.rt.heat(ctx, box.model.qq, 1e-3, qr)
+    torch.cuda.synchronize()
+    k0, j0, i0 = g.k_margin, g.j_margin, g.i_margin
+    sl = (slice(k0, k0 + g.k_size), slice(j0, j0 + g.j_size), slice(i0, i0 + g.i_size))
+    exp = before.clone()
+    exp[sl] = before[sl] + 1e-3 * qr[sl] / box.model.qq.data[0][sl]
+    assert torch.equal(box.model.qq.data[7], exp)
+
+
+@pytest.mark.gpu
+def test_column_features_outputs_and_flags(ctx):
+    import torch
+    from miso_amd.surrogate import ColumnSurrogate, interior_columns
+    box = _box(ctx, 8, 64)
+    g, nz = box.grid, 64
+    sur = ColumnSurrogate(ctx, nz, hidden=64, hidden_layers=2, max_batch=64)
+    rng = np.random.default_rng(0)
+    mu_in, sd_in = rng.standard_normal(3 * nz), 0.5 + rng.random(3 * nz)
+    mu_out, sd_out = rng.standard_normal(nz), 0.5 + rng.random(nz)
+    sur.set_normalization(mu_in, sd_in, mu_out, sd_out)
+    X, Xf = sur.features(g, box.model.qq, box.strat.gm, keep_float=True)
+    torch.cuda.synchronize()
+    q = box.model.qq.data.double()
+    ro = interior_columns(q[0], g)
+    T = interior_columns((box.strat.gm - 1.0) * q[7], g)
+    b2 = interior_columns(q[4] ** 2 + q[5] ** 2 + q[6] ** 2, g)
+    raw = torch.cat([ro.log(), T.log(), b2], dim=1)
+    ref = (raw.float() - torch.as_tensor(mu_in, dtype=torch.float32, device="cuda:0")) / \
+        torch.as_tensor(sd_in, dtype=torch.float32, device="cuda:0")
+    assert torch.allclose(Xf, ref, rtol=1e-5, atol=1e-5)
+    assert torch.equal(X, Xf.to(torch.bfloat16))
+    # output: denormalised network output, flagged columns from q_rt
+    out = torch.zeros_like(box.rt.qrad_field)
+    q_rt = torch.randn_like(out)
+    flags = torch.zeros(g.i_size * g.j_size, dtype=torch.int32, device="cuda:0")
+    flags[5] = 1
+    flags[17] = 1
+    sur.predict(g, box.model.qq, box.strat.gm, out, flags=flags, q_rt=q_rt)
+    Y = sur.forward(X).double()
+    exp = Y * torch.as_tensor(sd_out.astype(np.float32), device="cuda:0").double() + \
+        torch.as_tensor(mu_out.astype(np.float32), device="cuda:0").double()
+    got = interior_columns(out, g)
+    rt_cols = interior_columns(q_rt, g)
+    for c in range(g.i_size * g.j_size):
+        if c in (5, 17):
+            assert torch.equal(got[c], rt_cols[c])
+        else:
+            assert torch.allclose(got[c], exp[c], rtol=1e-6, atol=1e-6)
+    # residual flags: relative L2 per column against a reference field made
+    # of the unflagged prediction with column 3 off by 10 %, column 9 by 0.1 %
+    sur.predict(g, box.model.qq, box.strat.gm, out)
+    ref_q = out.clone()
+    k0, j0, i0 = g.k_margin, g.j_margin, g.i_margin
+    ref_q[k0:k0 + nz, j0 + 0, i0 + 3] *= 1.1
+    ref_q[k0:k0 + nz, j0 + 1, i0 + 1] *= 1.001
+    sur.residual(g, ref_q, flags, 1e-2)
+    torch.cuda.synchronize()
+    f = flags.cpu().numpy()
+    assert f[3] == 1 and f[9] == 0 and f.sum() == 1
+    sur.close()
+
+
+@pytest.mark.gpu
+def test_learned_rt_beats_stale_rt_between_solves(ctx):
+    """Train the column network on GPU-RT columns of boxes with other seeds,
+    then run the learned stepping (plain RT every 8 steps) and compare, at
+    every step between solves, the network's Q_rad and the last solved
+    Q_rad (the method without the network) against a converged RT solve of
+    the current state."""
+    import torch
+    from miso_amd.surrogate import ColumnSurrogate, collect_columns
+    from miso_amd.rmhd import rel_l2
+    nxy, nz = 32, 64
+    data_box = _box(ctx, nxy, nz, seed=1)
+    F, Tq = collect_columns(data_box, seeds=(1, 2, 3), steps=24, sample_every=3)
+    assert F.shape == (3 * 8 * nxy * nxy, 3 * nz)
+    sur = ColumnSurrogate(ctx, nz, hidden=256, hidden_layers=3, max_batch=nxy * nxy, seed=0)
+    train_mse, val = sur.fit(F, Tq, epochs=30, batch=256, lr=2e-3)
+    print(f"train mse {train_mse:.3e}  validation rel L2 {val:.3e}", flush=True)
+    assert val < 5e-2
+    box = _box(ctx, nxy, nz, seed=11)
+    box.attach_surrogate(sur, rt_every=8, flag_tol=0.05)
+    err_mlp, err_stale = [], []
+    for s in range(24):
+        if s % 8 != 0:
+            truth = box.reference_qrad().clone()
+            box.surrogate.predict(box.grid, box.model.qq, box.strat.gm, box.q_mlp)
+            err_mlp.append(rel_l2(box.q_mlp, truth, box.grid))
+            err_stale.append(rel_l2(box.q_rt, truth, box.grid))
+        box.step_learned()
+    em, es = float(np.mean(err_mlp)), float(np.mean(err_stale))
+    print(f"mean rel L2 vs converged RT: network {em:.3e}, stale RT {es:.3e}", flush=True)
+    assert em < es
+    sur.close()
