@@ -140,6 +140,30 @@ def test_float_narrowing_and_constants_match_reference():
         assert L.ref_const_f32(w) == float(np.float32(v))
 
 
+def test_oracle_matches_committed_reference_vectors():
+    """The limiter, constants and quadrature of the oracle against vectors
+    the reference's own headers produced (tests/golden/make_ref_vectors.py;
+    this check needs neither the reference nor oracle/_ref)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "ref_limiter_quadrature.npz")
+    d = np.load(path)
+    for S, dt in (("f64", np.float64), ("f32", np.float32)):
+        for row, r in zip(d[f"dqq_in_{S}"], d[f"dqq_out_{S}"]):
+            o = O.dqq_eval(*row, dtype=dt)
+            assert dt(o) == r or (np.isnan(o) and np.isnan(r)), (S, row, o, r)
+        for row, r in zip(d[f"flux_in_{S}"], d[f"flux_out_{S}"]):
+            assert dt(O.flux_core(*row, dtype=dt)) == r, (S, row)
+    # the narrowing min/max the kernels reproduce: (double)fminf((float)a, (float)b)
+    f = lambda *xs: [float(np.float32(x)) for x in xs]
+    for (a, b, c), m2, m3, x3 in zip(d["minmax_in"], d["min2_out"], d["min3_out"], d["max3_out"]):
+        assert min(f(a, b)) == m2 and min(f(a, b, c)) == m3 and max(f(a, b, c)) == x3
+    w, mx, my, mz = O.rt_quadrature(24)
+    q = d["quadrature_24"]
+    assert np.array_equal(q, np.stack([w, mx, my, mz], 1))
+    inv12, pii4, pii8 = d["const_f64"][:3]
+    assert (inv12, pii4, pii8) == (1.0 / 12.0, 1.0 / (4 * math.pi), 1.0 / (8 * math.pi))
+
+
 def minmod_slope(a, b, c, ep):
     """minmod(ep*(b-a), (c-a)/2, ep*(c-b)) on float-rounded arguments."""
     args = [np.float32(ep * (b - a)), np.float32(ep * (c - b)), np.float32((c - a) * 0.5)]
