@@ -1,0 +1,396 @@
+// miso_hip.hpp -- C++17 host layer of the MI355X backend, over misohip.h.
+//
+// The classes below carry MISO's names, constructor arguments and method
+// signatures for the MHD time step so that a MISO model can run on the HIP
+// backend by changing its Backend type (see INTEGRATION.md for the
+// specialisations MISO itself would add):
+//
+//   miso::hip::ExecContext<Real>     ExecContext<Real, backend::CUDA>
+//                                    (miso/include/miso/mhd_core.hpp:15-33)
+//   miso::hip::HostGrid<Real>        Grid<Real, backend::Host> for one rank
+//                                    (miso/include/miso/grid.hpp:22-99,147-242)
+//   miso::hip::Grid<Real>            Grid<Real, backend::CUDA> (device metric)
+//   miso::hip::HostFields<Real>      Fields<Real, backend::Host> (z-fastest)
+//                                    (miso/include/miso/mhd_fields.hpp:13-54)
+//   miso::hip::Fields<Real>          Fields<Real, backend::CUDA> (x-fastest)
+//   miso::hip::Integrator<Real>      Integrator<Real, Backend>::update / cfl /
+//                                    apply_boundary_condition
+//                                    (miso/include/miso/mhd_integrator_host.hpp:13-357)
+//   miso::hip::MHD<Real>             mhd::MHD<Real, Backend> (miso/include/miso/mhd.hpp:15-50)
+//
+// User boundary and source functors cannot cross a C ABI; they are given as
+// descriptors: Boundary (per axis periodic / mirror with per-field signs /
+// fixed ghosts) and Gravity / NoSource.  Errors follow the reference's
+// convention (miso/include/miso/cuda_util.cuh:17-33): a failed backend call
+// prints file, line and message and aborts; invalid configuration throws
+// std::runtime_error (miso/include/miso/grid.hpp:244-276).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "misohip.h"
+
+namespace miso {
+namespace backend {
+/// Tag of the MI355X backend (beside MISO's Host and CUDA, backend.hpp:8,12).
+struct HIP {};
+}  // namespace backend
+
+namespace hip {
+
+inline void check(int rc, const char *file, int line) {
+  if (rc != MISOHIP_OK) {
+    std::fprintf(stderr, "MISO HIP Error: %s %s %d\n", misohip_strerror(rc), file, line);
+    std::fflush(stderr);
+    std::abort();
+  }
+}
+#define MISO_HIP_CHECK(call) ::miso::hip::check((call), __FILE__, __LINE__)
+
+inline void check_hip(hipError_t e, const char *file, int line) {
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "HIP Error: %s %s %d\n", hipGetErrorString(e), file, line);
+    std::fflush(stderr);
+    std::abort();
+  }
+}
+#define MISO_HIP_RT_CHECK(call) ::miso::hip::check_hip((call), __FILE__, __LINE__)
+
+template <typename Real> constexpr misohip_dtype dtype_of() {
+  static_assert(std::is_same<Real, double>::value || std::is_same<Real, float>::value,
+                "Real must be double or float");
+  return std::is_same<Real, double>::value ? MISOHIP_F64 : MISOHIP_F32;
+}
+
+/// Device buffer, move-only (as Array3D<T, CUDA>, array3d.hpp:284-303).
+template <typename T> class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(std::size_t n) : n_(n) {
+    MISO_HIP_RT_CHECK(hipMalloc(reinterpret_cast<void **>(&p_), sizeof(T) * (n ? n : 1)));
+    MISO_HIP_RT_CHECK(hipMemset(p_, 0, sizeof(T) * (n ? n : 1)));
+  }
+  DeviceArray(const DeviceArray &) = delete;
+  DeviceArray &operator=(const DeviceArray &) = delete;
+  DeviceArray(DeviceArray &&o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DeviceArray &operator=(DeviceArray &&o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = o.p_; n_ = o.n_;
+      o.p_ = nullptr; o.n_ = 0;
+    }
+    return *this;
+  }
+  ~DeviceArray() { release(); }
+  T *data() const { return p_; }
+  std::size_t size() const { return n_; }
+  void upload(const T *h, std::size_t n) {
+    MISO_HIP_RT_CHECK(hipMemcpy(p_, h, sizeof(T) * n, hipMemcpyHostToDevice));
+  }
+
+ private:
+  void release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+  }
+  T *p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+/// Execution context: device ordinal and the HIP stream of every call.
+template <typename Real> class ExecContext {
+ public:
+  explicit ExecContext(int device = 0, hipStream_t stream = nullptr) {
+    MISO_HIP_RT_CHECK(hipSetDevice(device));
+    MISO_HIP_CHECK(misohip_create(device, stream, &ctx_));
+  }
+  ExecContext(const ExecContext &) = delete;
+  ExecContext &operator=(const ExecContext &) = delete;
+  ~ExecContext() {
+    if (ctx_) misohip_destroy(ctx_);
+  }
+  misohip_ctx *handle() const { return ctx_; }
+  void synchronize() const { MISO_HIP_CHECK(misohip_synchronize(ctx_)); }
+
+ private:
+  misohip_ctx *ctx_ = nullptr;
+};
+
+/// One axis of a uniform grid (AxisGrid + UniformAxisProvider, grid.hpp:20-99).
+template <typename Real> struct AxisGrid {
+  int size = 0, total = 0, margin = 0, stride = 0;
+  std::vector<Real> s, ds, dsi;
+  AxisGrid() = default;
+  AxisGrid(int size_, int margin_, Real min_value, Real max_value) {
+    if (size_ <= 0 || margin_ < 1) throw std::runtime_error("Invalid grid size or margin");
+    if (!(min_value < max_value)) throw std::runtime_error("Invalid grid range");
+    size = size_;
+    stride = size > 1 ? 1 : 0;
+    margin = margin_ * stride;
+    total = size + 2 * margin;
+    s.resize(total);
+    ds.resize(total);
+    dsi.resize(total);
+    const Real space = (max_value - min_value) / Real(size);
+    if (size == 1) {
+      s[0] = min_value + (Real(0) + Real(0.5)) * space;
+      ds[0] = Real(1.e30);
+      dsi[0] = Real(0.0);
+      return;
+    }
+    for (int i = 0; i < total; ++i) {
+      const int ig = i - margin;
+      s[i] = min_value + (Real(ig) + Real(0.5)) * space;
+      ds[i] = space;
+      dsi[i] = Real(1.0 / ds[i]);
+    }
+  }
+};
+
+/// Host grid of one rank (the single-rank case of grid.hpp:147-242).
+template <typename Real> struct HostGrid {
+  AxisGrid<Real> xg, yg, zg;
+  Real x_min, x_max, y_min, y_max, z_min, z_max;
+  Real min_dxyz;
+  HostGrid(int i_size, int j_size, int k_size, int margin, Real x_min_, Real x_max_,
+           Real y_min_, Real y_max_, Real z_min_, Real z_max_)
+      : xg(i_size, margin, x_min_, x_max_), yg(j_size, margin, y_min_, y_max_),
+        zg(k_size, margin, z_min_, z_max_), x_min(x_min_), x_max(x_max_), y_min(y_min_),
+        y_max(y_max_), z_min(z_min_), z_max(z_max_) {
+    // util::min3 narrows through float (utility.hpp:91-124)
+    const float m = std::fmin(std::fmin(float(xg.ds[0]), float(yg.ds[0])), float(zg.ds[0]));
+    min_dxyz = Real(m);
+  }
+  int i_size() const { return xg.size; }
+  int j_size() const { return yg.size; }
+  int k_size() const { return zg.size; }
+  int i_total() const { return xg.total; }
+  int j_total() const { return yg.total; }
+  int k_total() const { return zg.total; }
+  int i_margin() const { return xg.margin; }
+  int j_margin() const { return yg.margin; }
+  int k_margin() const { return zg.margin; }
+  const std::vector<Real> &x() const { return xg.s; }
+  const std::vector<Real> &y() const { return yg.s; }
+  const std::vector<Real> &z() const { return zg.s; }
+  std::size_t n_total() const {
+    return std::size_t(xg.total) * std::size_t(yg.total) * std::size_t(zg.total);
+  }
+};
+
+/// Device grid: the descriptor plus the device metric arrays.
+template <typename Real> class Grid {
+ public:
+  explicit Grid(const HostGrid<Real> &h)
+      : host_(h), dx_(h.xg.total), dy_(h.yg.total), dz_(h.zg.total), dxi_(h.xg.total),
+        dyi_(h.yg.total), dzi_(h.zg.total) {
+    dx_.upload(h.xg.ds.data(), h.xg.total);
+    dy_.upload(h.yg.ds.data(), h.yg.total);
+    dz_.upload(h.zg.ds.data(), h.zg.total);
+    dxi_.upload(h.xg.dsi.data(), h.xg.total);
+    dyi_.upload(h.yg.dsi.data(), h.yg.total);
+    dzi_.upload(h.zg.dsi.data(), h.zg.total);
+    d_.i_size = h.i_size(); d_.j_size = h.j_size(); d_.k_size = h.k_size();
+    d_.i_total = h.i_total(); d_.j_total = h.j_total(); d_.k_total = h.k_total();
+    d_.i_margin = h.i_margin(); d_.j_margin = h.j_margin(); d_.k_margin = h.k_margin();
+    d_.is = h.xg.stride; d_.js = h.yg.stride; d_.ks = h.zg.stride;
+    d_.min_dxyz = double(h.min_dxyz);
+    d_.dx = dx_.data(); d_.dy = dy_.data(); d_.dz = dz_.data();
+    d_.dxi = dxi_.data(); d_.dyi = dyi_.data(); d_.dzi = dzi_.data();
+  }
+  const misohip_grid *desc() const { return &d_; }
+  const HostGrid<Real> &host() const { return host_; }
+
+ private:
+  HostGrid<Real> host_;
+  DeviceArray<Real> dx_, dy_, dz_, dxi_, dyi_, dzi_;
+  misohip_grid d_{};
+};
+
+enum Field { RO = 0, VX, VY, VZ, BX, BY, BZ, EI, PH, N_FIELDS };
+
+/// Host fields in MISO order: (i, j, k) at (i * Jt + j) * Kt + k.
+template <typename Real> struct HostFields {
+  int i_total, j_total, k_total;
+  std::array<std::vector<Real>, N_FIELDS> q;
+  HostFields(int it, int jt, int kt) : i_total(it), j_total(jt), k_total(kt) {
+    for (auto &f : q) f.assign(std::size_t(it) * jt * kt, Real(0));
+  }
+  Real &operator()(int f, int i, int j, int k) {
+    return q[f][(std::size_t(i) * j_total + j) * k_total + k];
+  }
+  Real operator()(int f, int i, int j, int k) const {
+    return q[f][(std::size_t(i) * j_total + j) * k_total + k];
+  }
+};
+
+/// Device fields: nine x-fastest arrays.
+template <typename Real> class Fields {
+ public:
+  explicit Fields(const Grid<Real> &g) : grid_(&g) {
+    const std::size_t n = g.host().n_total();
+    for (int f = 0; f < N_FIELDS; ++f) {
+      a_[f] = DeviceArray<Real>(n);
+      d_.q[f] = a_[f].data();
+    }
+  }
+  const misohip_fields *desc() const { return &d_; }
+  Real *field(int f) const { return a_[f].data(); }
+
+  /// Fields<HIP>::copy_from(Fields<Host>) (mhd_fields.hpp:83-107)
+  void copy_from(const ExecContext<Real> &ctx, const HostFields<Real> &h) {
+    for (int f = 0; f < N_FIELDS; ++f)
+      MISO_HIP_CHECK(misohip_copy_to_device(ctx.handle(), grid_->desc(), dtype_of<Real>(),
+                                            h.q[f].data(), a_[f].data()));
+  }
+  /// Fields<Host>::copy_from(Fields<HIP>) (mhd_fields.hpp:148-172)
+  void copy_to(const ExecContext<Real> &ctx, HostFields<Real> &h) const {
+    for (int f = 0; f < N_FIELDS; ++f)
+      MISO_HIP_CHECK(misohip_copy_to_host(ctx.handle(), grid_->desc(), dtype_of<Real>(),
+                                          a_[f].data(), h.q[f].data()));
+  }
+
+ private:
+  const Grid<Real> *grid_;
+  std::array<DeviceArray<Real>, N_FIELDS> a_;
+  misohip_fields d_{};
+};
+
+/// Boundary descriptor: per axis periodic (the halo self-exchange), mirror
+/// with a sign per field (bc::symmetric, boundary_condition.hpp:108-136) or
+/// fixed (the ghost cells keep their values, EmptyBoundaryCondition on that
+/// axis, mhd_model_base.hpp:125).
+struct Boundary {
+  enum Kind { MIRROR = 0, PERIODIC = 1, FIXED = 2 };
+  misohip_bc d{};
+  Boundary(Kind x = PERIODIC, Kind y = PERIODIC, Kind z = PERIODIC) {
+    d.periodic[0] = x; d.periodic[1] = y; d.periodic[2] = z;
+    for (auto &a : d.sym_sign)
+      for (int &s : a) s = 1;
+  }
+  /// sign of field f when mirrored across the faces of axis a (+1 / -1)
+  Boundary &sign(int axis, int f, int s) {
+    d.sym_sign[axis][f] = s >= 0 ? 1 : -1;
+    return *this;
+  }
+};
+
+/// Source descriptors (the user Source functor's two forms here).
+struct NoSource {
+  void fill(misohip_params &p) const { p.src_kind = 0; p.grav_axis = 2; p.grav = 0.0; }
+};
+/// constant gravity: src.v[axis] = -g ro (demo/hd2d_rayleigh_taylor/src/main.cpp:109-112)
+struct Gravity {
+  int axis;
+  double g;
+  void fill(misohip_params &p) const { p.src_kind = 1; p.grav_axis = axis; p.grav = g; }
+};
+
+/// The ideal-gas EOS the step uses (eos::IdealEOS, eos.hpp:11-43).
+struct IdealEOS {
+  double gm;
+};
+
+/// Config keys the step reads (config_defaults.hpp:4-32).
+struct MHDConfig {
+  double cfl_number = 0.5;
+  double ep = 1.0, fh = 1.0, cs_fac = 1.0, ca_fac = 1.0, vv_fac = 1.0;
+};
+
+/// Integrator: cfl, update, apply_boundary_condition
+/// (mhd_integrator_host.hpp:13-357).  Owns qq_argm, qq_rslt and cc.
+template <typename Real> class Integrator {
+ public:
+  Integrator(const MHDConfig &cfg, const Grid<Real> &grid, ExecContext<Real> &ctx)
+      : cfg_(cfg), grid_(grid), ctx_(ctx), argm_(grid), rslt_(grid), cc_(grid.host().n_total()) {
+    if (cfg.ep < 0 || cfg.fh < 0)
+      throw std::runtime_error("artificial viscosity ep and fh must be >= 0");
+    w_.argm = *argm_.desc();
+    w_.rslt = *rslt_.desc();
+    w_.cc = cc_.data();
+  }
+
+  template <typename BC> void apply_boundary_condition(const BC &bc, Fields<Real> &qq) {
+    MISO_HIP_CHECK(misohip_ghost_fill(ctx_.handle(), grid_.desc(), dtype_of<Real>(), qq.desc(),
+                                      &bc.d));
+  }
+
+  template <typename EOS> Real cfl(const Fields<Real> &qq, const EOS &eos) {
+    misohip_params p = params(eos, NoSource{});
+    double dt = 0.0;
+    MISO_HIP_CHECK(misohip_cfl(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, qq.desc(), &dt));
+    return Real(dt);
+  }
+
+  template <typename EOS, typename BC, typename Source>
+  void update(Real dt, const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq) {
+    misohip_params p = params(eos, src);
+    MISO_HIP_CHECK(misohip_update(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
+                                  qq.desc(), &w_, double(dt)));
+  }
+
+  /// cfl + update in one call with one host synchronisation
+  /// (ModelBase::update, mhd_model_base.hpp:57-62); returns dt.
+  template <typename EOS, typename BC, typename Source>
+  Real step(const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq) {
+    misohip_params p = params(eos, src);
+    double dt = 0.0;
+    MISO_HIP_CHECK(misohip_step(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
+                                qq.desc(), &w_, &dt));
+    return Real(dt);
+  }
+
+ private:
+  template <typename EOS, typename Source> misohip_params params(const EOS &eos, const Source &src) {
+    misohip_params p{};
+    p.gm = eos.gm;
+    p.cfl_number = cfg_.cfl_number;
+    p.ep = cfg_.ep; p.fh = cfg_.fh;
+    p.cs_fac = cfg_.cs_fac; p.ca_fac = cfg_.ca_fac; p.vv_fac = cfg_.vv_fac;
+    src.fill(p);
+    return p;
+  }
+  MHDConfig cfg_;
+  const Grid<Real> &grid_;
+  ExecContext<Real> &ctx_;
+  Fields<Real> argm_, rslt_;
+  DeviceArray<Real> cc_;
+  misohip_work w_{};
+};
+
+/// mhd::MHD: grid, fields and integrator (mhd.hpp:15-50).
+template <typename Real> struct MHD {
+  Grid<Real> grid;
+  Fields<Real> qq;
+  ExecContext<Real> &exec_ctx;
+  Integrator<Real> integrator;
+
+  MHD(const MHDConfig &cfg, const HostGrid<Real> &grid_h, ExecContext<Real> &ctx)
+      : grid(grid_h), qq(grid), exec_ctx(ctx), integrator(cfg, grid, ctx) {}
+
+  /// ic(HostFields&, const HostGrid&) fills the host fields, then H2D + BC
+  template <typename IC, typename BC> void apply_initial_condition(const IC &ic, const BC &bc) {
+    const HostGrid<Real> &h = grid.host();
+    HostFields<Real> qq_h(h.i_total(), h.j_total(), h.k_total());
+    ic(qq_h, h);
+    qq.copy_from(exec_ctx, qq_h);
+    integrator.apply_boundary_condition(bc, qq);
+  }
+  template <typename EOS> Real cfl(const EOS &eos) { return integrator.cfl(qq, eos); }
+  template <typename EOS, typename BC, typename Source>
+  void update(Real dt, const EOS &eos, const BC &bc, const Source &src) {
+    integrator.update(dt, eos, bc, src, qq);
+  }
+};
+
+}  // namespace hip
+}  // namespace miso

@@ -1,0 +1,237 @@
+// Tests of the C++ host layer (include/miso_hip.hpp), built into a shared
+// library that pytest loads with ctypes, so every check runs inside the
+// test process (no child program is started from a process that has
+// touched the GPU).
+//
+//   cpp_test_hd1d_shock_tube   the reference's end-to-end test,
+//                              miso/tests/serial/test_hd1d_shock_tube.cpp:20-81:
+//                              Sod along x, y and z (FP32, 128 cells, gamma
+//                              1.4, mirror faces, cfl 0.5, run to t = 0.2),
+//                              L1 error of ro and vx below 5e-2 against the
+//                              exact Riemann solution, x/y/z agreement 1e-5
+//   cpp_run_steps              N steps of a given state through the C++
+//                              classes, for bitwise comparison with the oracle
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "miso_hip.hpp"
+
+using namespace miso;
+
+namespace {
+
+// Exact solution of the Riemann problem of the Euler equations (Toro,
+// "Riemann Solvers and Numerical Methods for Fluid Dynamics", ch. 4).
+struct ExactRiemann {
+  double gm, rl, ul, pl, rr, ur, pr, cl, cr, ps, us;
+  ExactRiemann(double gm_, double rl_, double ul_, double pl_, double rr_, double ur_, double pr_)
+      : gm(gm_), rl(rl_), ul(ul_), pl(pl_), rr(rr_), ur(ur_), pr(pr_) {
+    cl = std::sqrt(gm * pl / rl);
+    cr = std::sqrt(gm * pr / rr);
+    double p = 0.5 * (pl + pr);
+    for (int it = 0; it < 100; ++it) {
+      double fl, dfl, fr, dfr;
+      side(p, rl, pl, cl, fl, dfl);
+      side(p, rr, pr, cr, fr, dfr);
+      const double f = fl + fr + (ur - ul);
+      const double pn = std::fmax(1e-12, p - f / (dfl + dfr));
+      if (std::fabs(pn - p) < 1e-15 * (pn + p)) { p = pn; break; }
+      p = pn;
+    }
+    ps = p;
+    double fl, dfl, fr, dfr;
+    side(ps, rl, pl, cl, fl, dfl);
+    side(ps, rr, pr, cr, fr, dfr);
+    us = 0.5 * (ul + ur) + 0.5 * (fr - fl);
+  }
+  void side(double p, double rk, double pk, double ck, double &f, double &df) const {
+    if (p > pk) {  // shock
+      const double A = 2.0 / ((gm + 1.0) * rk), B = (gm - 1.0) / (gm + 1.0) * pk;
+      const double q = std::sqrt(A / (p + B));
+      f = (p - pk) * q;
+      df = q * (1.0 - 0.5 * (p - pk) / (p + B));
+    } else {  // rarefaction
+      const double e = (gm - 1.0) / (2.0 * gm);
+      f = 2.0 * ck / (gm - 1.0) * (std::pow(p / pk, e) - 1.0);
+      df = 1.0 / (rk * ck) * std::pow(p / pk, -(gm + 1.0) / (2.0 * gm));
+    }
+  }
+  // density and velocity at x/t = s
+  void sample(double s, double &r, double &u) const {
+    const double g1 = (gm - 1.0) / (gm + 1.0);
+    if (s <= us) {
+      if (ps > pl) {
+        const double sl = ul - cl * std::sqrt((gm + 1.0) / (2.0 * gm) * ps / pl + (gm - 1.0) / (2.0 * gm));
+        if (s <= sl) { r = rl; u = ul; }
+        else { r = rl * (ps / pl + g1) / (g1 * ps / pl + 1.0); u = us; }
+      } else {
+        const double shl = ul - cl;
+        const double csl = cl * std::pow(ps / pl, (gm - 1.0) / (2.0 * gm));
+        const double stl = us - csl;
+        if (s <= shl) { r = rl; u = ul; }
+        else if (s > stl) { r = rl * std::pow(ps / pl, 1.0 / gm); u = us; }
+        else {
+          const double c = 2.0 / (gm + 1.0) * (cl + 0.5 * (gm - 1.0) * (ul - s));
+          r = rl * std::pow(c / cl, 2.0 / (gm - 1.0));
+          u = 2.0 / (gm + 1.0) * (cl + 0.5 * (gm - 1.0) * ul + s);
+        }
+      }
+    } else {
+      if (ps > pr) {
+        const double sr = ur + cr * std::sqrt((gm + 1.0) / (2.0 * gm) * ps / pr + (gm - 1.0) / (2.0 * gm));
+        if (s >= sr) { r = rr; u = ur; }
+        else { r = rr * (ps / pr + g1) / (g1 * ps / pr + 1.0); u = us; }
+      } else {
+        const double shr = ur + cr;
+        const double csr = cr * std::pow(ps / pr, (gm - 1.0) / (2.0 * gm));
+        const double str = us + csr;
+        if (s >= shr) { r = rr; u = ur; }
+        else if (s <= str) { r = rr * std::pow(ps / pr, 1.0 / gm); u = us; }
+        else {
+          const double c = 2.0 / (gm + 1.0) * (cr - 0.5 * (gm - 1.0) * (ur - s));
+          r = rr * std::pow(c / cr, 2.0 / (gm - 1.0));
+          u = 2.0 / (gm + 1.0) * (-cr + 0.5 * (gm - 1.0) * ur + s);
+        }
+      }
+    }
+  }
+};
+
+using Real = float;
+
+struct SodRun {
+  std::vector<Real> ro, vn, s;  // interior density, normal velocity, coordinate
+  double time = 0.0;
+};
+
+// One shock tube along `axis` through the MISO-shaped classes.
+SodRun run_sod(int axis) {
+  const double gm = 1.4, rol = 1.0, prl = 1.0, ror = 0.125, prr = 0.1;
+  const int n = 128;
+  int sz[3] = {1, 1, 1};
+  sz[axis] = n;
+  hip::HostGrid<Real> hg(sz[0], sz[1], sz[2], 2, 0.f, 1.f, 0.f, 1.f, 0.f, 1.f);
+  hip::ExecContext<Real> ctx(0);
+  hip::MHDConfig cfg;  // config_defaults.hpp: cfl 0.5, AV defaults
+  hip::MHD<Real> mhd(cfg, hg, ctx);
+  hip::IdealEOS eos{gm};
+  hip::Boundary bc(hip::Boundary::MIRROR, hip::Boundary::MIRROR, hip::Boundary::MIRROR);
+  hip::NoSource src;
+  auto ic = [&](hip::HostFields<Real> &q, const hip::HostGrid<Real> &g) {
+    for (int i = 0; i < g.i_total(); ++i)
+      for (int j = 0; j < g.j_total(); ++j)
+        for (int k = 0; k < g.k_total(); ++k) {
+          const Real xyz = axis == 0 ? g.x()[i] : axis == 1 ? g.y()[j] : g.z()[k];
+          const bool left = xyz < Real(0.5);
+          q(hip::RO, i, j, k) = left ? Real(rol) : Real(ror);
+          q(hip::EI, i, j, k) = Real((left ? prl : prr) / (gm - 1.0) / (left ? rol : ror));
+        }
+  };
+  mhd.apply_initial_condition(ic, bc);
+  double time = 0.0;
+  const double tend = 0.2;
+  while (time < tend) {  // ModelBase::run / update (mhd_model_base.hpp:57-62,89-117)
+    const Real dt = mhd.cfl(eos);
+    mhd.update(dt, eos, bc, src);
+    time += double(Real(dt));
+  }
+  hip::HostFields<Real> out(hg.i_total(), hg.j_total(), hg.k_total());
+  mhd.qq.copy_to(ctx, out);
+  SodRun r;
+  r.time = time;
+  const auto &coord = axis == 0 ? hg.x() : axis == 1 ? hg.y() : hg.z();
+  const int m = 2;
+  for (int a = m; a < m + n; ++a) {
+    const int i = axis == 0 ? a : 0, j = axis == 1 ? a : 0, k = axis == 2 ? a : 0;
+    r.ro.push_back(out(hip::RO, i, j, k));
+    r.vn.push_back(out(hip::VX + axis, i, j, k));
+    r.s.push_back(coord[a]);
+  }
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Returns 0 when every check passes; msg receives a one-line summary.
+int cpp_test_hd1d_shock_tube(char *msg, int len) {
+  SodRun x = run_sod(0), y = run_sod(1), z = run_sod(2);
+  ExactRiemann ex(1.4, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1);
+  double dro = 0, dvx = 0, sro = 0, svx = 0;
+  for (std::size_t a = 0; a < x.ro.size(); ++a) {
+    double r, u;
+    ex.sample((double(x.s[a]) - 0.5) / x.time, r, u);
+    dro += std::fabs(x.ro[a] - r);
+    dvx += std::fabs(x.vn[a] - u);
+    sro += x.ro[a];
+    svx += x.vn[a];
+  }
+  double worst = 0.0;
+  for (std::size_t a = 0; a < x.ro.size(); ++a) {
+    const double d1 = std::fabs(double(x.ro[a]) - y.ro[a]) / (1.0 + std::fmax(std::fabs(x.ro[a]), std::fabs(y.ro[a])));
+    const double d2 = std::fabs(double(x.ro[a]) - z.ro[a]) / (1.0 + std::fmax(std::fabs(x.ro[a]), std::fabs(z.ro[a])));
+    worst = std::fmax(worst, std::fmax(d1, d2));
+  }
+  const bool ok = dro / sro < 5e-2 && dvx / svx < 5e-2 && worst <= 1e-5 && x.time == y.time &&
+                  x.time == z.time;
+  std::snprintf(msg, len, "t=%.6f L1(ro)=%.4e L1(vx)=%.4e xyz=%.3e p*=%.6f", x.time, dro / sro,
+                dvx / svx, worst, ex.ps);
+  return ok ? 0 : 1;
+}
+
+// Runs nsteps of ModelBase::update on a state given in MISO order (9 fields
+// of i_total*j_total*k_total values, ghosts included) through HostGrid,
+// MHD and Integrator; writes the final state and every dt.
+int cpp_run_steps(int fp64, int ni, int nj, int nk, const double *bounds, const int *kinds,
+                  const int *signs, double gm, double cfl_number, int src_kind, int grav_axis,
+                  double grav, int nsteps, const void *q_in, void *q_out, double *dts) {
+  auto run = [&](auto zero) -> int {
+    using R = decltype(zero);
+    hip::HostGrid<R> hg(ni, nj, nk, 2, R(bounds[0]), R(bounds[1]), R(bounds[2]), R(bounds[3]),
+                        R(bounds[4]), R(bounds[5]));
+    hip::ExecContext<R> ctx(0);
+    hip::MHDConfig cfg;
+    cfg.cfl_number = cfl_number;
+    hip::MHD<R> mhd(cfg, hg, ctx);
+    hip::Boundary bc{static_cast<hip::Boundary::Kind>(kinds[0]),
+                     static_cast<hip::Boundary::Kind>(kinds[1]),
+                     static_cast<hip::Boundary::Kind>(kinds[2])};
+    for (int a = 0; a < 3; ++a)
+      for (int f = 0; f < 9; ++f) bc.sign(a, f, signs[a * 9 + f]);
+    hip::IdealEOS eos{gm};
+    const std::size_t N = hg.n_total();
+    const R *src_q = static_cast<const R *>(q_in);
+    auto ic = [&](hip::HostFields<R> &q, const hip::HostGrid<R> &) {
+      for (int f = 0; f < 9; ++f) std::memcpy(q.q[f].data(), src_q + f * N, sizeof(R) * N);
+    };
+    mhd.apply_initial_condition(ic, bc);
+    for (int s = 0; s < nsteps; ++s) {
+      R dt;
+      if (src_kind == 1) {
+        hip::Gravity g{grav_axis, grav};
+        dt = mhd.cfl(eos);
+        mhd.update(dt, eos, bc, g);
+      } else {
+        dt = mhd.cfl(eos);
+        mhd.update(dt, eos, bc, hip::NoSource{});
+      }
+      dts[s] = double(dt);
+    }
+    hip::HostFields<R> out(hg.i_total(), hg.j_total(), hg.k_total());
+    mhd.qq.copy_to(ctx, out);
+    R *dst = static_cast<R *>(q_out);
+    for (int f = 0; f < 9; ++f) std::memcpy(dst + f * N, out.q[f].data(), sizeof(R) * N);
+    return 0;
+  };
+  try {
+    return fp64 ? run(double(0)) : run(float(0));
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "cpp_run_steps: %s\n", e.what());
+    return 2;
+  }
+}
+
+}  // extern "C"
