@@ -56,6 +56,12 @@ def parse():
     ap.add_argument("--rmhd-flag-tol", type=float, default=0.03)
     ap.add_argument("--rmhd-train-columns", type=int, default=100000)
     ap.add_argument("--rmhd-epochs", type=int, default=40)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--ens-runs", type=int, default=64)
+    ap.add_argument("--ens-seed", type=int, default=2026)
+    ap.add_argument("--ens-graph-steps", type=int, default=10)
+    ap.add_argument("--ens-replays", type=int, default=5)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of known size (HBM reference for the "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -128,6 +134,78 @@ def cpu_baseline(target_s: float):
         "sample": f"3D turbulence {n}^3 FP64 (same IC family), {steps} steps in {el:.1f} s, "
                   f"{threads} OpenMP threads",
     }
+
+
+BYTES_STEP_2D = {"f64": 1096, "f32": 548}     # algorithmic bytes per 2D cell-update
+
+
+def ensemble_leg(args):
+    """Batched ensemble: args.ens_runs independent 2D Kelvin-Helmholtz runs,
+    FP32, sizes drawn by seed from {128^2, 256^2, 512^2, 1024^2} (density
+    ratio 2, dv = 1, tanh width 0.02, seeded Bx in [0, 0.2], 1e-3 seeded vy
+    noise, gamma 1.4, p 2.5, x periodic, y mirror), advanced together by
+    replays of a hipGraph of args.ens_graph_steps batched steps; each run
+    keeps its own dt on the device.  Beside it, the same runs stepped one
+    after another through misohip_step (the unbatched path)."""
+    import numpy as np
+    import torch
+
+    from miso_amd import problems
+    from miso_amd.ensemble import KH_BC, Ensemble, kh_sizes
+    from miso_amd.mhd import ExecContext
+
+    ctx = ExecContext(0)
+    sizes = kh_sizes(args.ens_runs, args.ens_seed)
+
+    def init(m, g):
+        return problems.kelvin_helmholtz(g, seed=args.ens_seed + m, device="cpu")
+
+    t0 = time.perf_counter()
+    ens = Ensemble(ctx, [(n, n, 1) for n in sizes], init, dtype=np.float32)
+    setup_s = time.perf_counter() - t0
+    k = args.ens_graph_steps
+    ens.capture(k)
+    ens.step(k)  # warm-up replay
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(k * args.ens_replays)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    steps = k * args.ens_replays
+    cells = ens.cells
+    value = cells * steps / el
+    dt, tm, st, fb = ens.state()
+    # unbatched: every run through misohip_step, one after another
+    t0 = time.perf_counter()
+    seq_steps = 2
+    for _ in range(seq_steps):
+        for r in ens.runs:
+            r.integrator.step(KH_BC, r.qq)
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t0
+    out = {
+        "workload": f"{len(sizes)} independent 2D Kelvin-Helmholtz runs FP32, sizes drawn by seed "
+                    f"{args.ens_seed} from 128^2..1024^2, one hipGraph of {k} batched steps per "
+                    "replay, per-run dt on the device",
+        "runs": len(sizes),
+        "size_counts": {str(n): sizes.count(n) for n in sorted(set(sizes))},
+        "cells": cells,
+        "steps": steps,
+        "ms_per_step": 1e3 * el / steps,
+        "cell_updates_per_s": value,
+        "hbm_roofline_frac": value * BYTES_STEP_2D["f32"] / PEAK_HBM,
+        "unbatched_cell_updates_per_s": cells * seq_steps / seq,
+        "speedup_vs_unbatched": value / (cells * seq_steps / seq),
+        "exp_fallback": fb,
+        "dt_range": [float(dt.min()), float(dt.max())],
+        "setup_seconds": setup_s,
+    }
+    print(f"[ensemble] {out}", file=sys.stderr, flush=True)
+    ens.close()
+    del ens
+    ctx.close()
+    torch.cuda.empty_cache()
+    return out
 
 
 def rmhd_leg(args):
@@ -263,6 +341,9 @@ def main():
     if args.rmhd_only:
         print(json.dumps({"rmhd": rmhd_leg(args)}), flush=True)
         return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg(args)}), flush=True)
+        return
     import numpy as np
     import torch
 
@@ -357,6 +438,8 @@ def main():
     ctx.close()
     del m
     torch.cuda.empty_cache()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args)
     if not args.no_rmhd:
         out["rmhd"] = rmhd_leg(args)
     if not args.no_cpu:

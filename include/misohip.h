@@ -276,6 +276,35 @@ int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t);
 int misohip_profile_enable(misohip_ctx *ctx, int on);
 int misohip_profile_read(misohip_ctx *ctx, double *ms, int *launches, int nkinds);
 
+/* ---------------------------------------------------------------------
+ * Batched ensemble (no counterpart in the reference, which runs one model
+ * per process, miso/include/miso/mhd_model_base.hpp:89-117).  n independent
+ * runs of one dtype and the same active axes (x and y active; z active in
+ * all or none), margin 2, each with its own grid, fields, workspace and dt,
+ * advanced together: one launch per phase covers every run, the step
+ * scalars (dt, ch2, damp) stay on the device, and a captured hipGraph
+ * replays whole steps.  Every run computes exactly what misohip_step
+ * computes for it alone.  The handle records the pointers it is given; the
+ * caller keeps those buffers alive until misohip_batch_destroy.  The handle
+ * owns one stream (default priority) for capture and replay, ordered
+ * against the context's stream with events. */
+typedef struct misohip_batch misohip_batch;
+int misohip_batch_create(misohip_ctx *ctx, misohip_dtype dt, int n, const misohip_grid *grids,
+                         const misohip_fields *qq, const misohip_work *work,
+                         const misohip_params *p, const misohip_bc *bc, misohip_batch **out);
+int misohip_batch_destroy(misohip_batch *b);
+/* Capture steps_per_graph steps (1..1000) into one graph; later
+ * misohip_batch_step calls replay it. */
+int misohip_batch_capture(misohip_ctx *ctx, misohip_batch *b, int steps_per_graph);
+/* Advance every run by nsteps (a multiple of the captured step count when a
+ * graph exists); asynchronous. */
+int misohip_batch_step(misohip_ctx *ctx, misohip_batch *b, int nsteps);
+/* Last dt, simulated time and step count of each run (synchronises); any
+ * output may be NULL.  *exp_fallback != 0 reports that a GLM damping factor
+ * was evaluated on the device rather than taken from the host table. */
+int misohip_batch_state(misohip_ctx *ctx, misohip_batch *b, double *dt, double *time,
+                        long *steps, int *exp_fallback);
+
 #ifdef __cplusplus
 }
 #endif

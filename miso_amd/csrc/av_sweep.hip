@@ -38,13 +38,13 @@ __device__ __forceinline__ void flux_pair(const R v[5], R cdw, R cup, R ep, R fh
 // DIR: sweep direction.  The launch covers interior x-columns; for DIR == 0
 // it also covers the y/z ring where only cc is produced.
 template <typename R, int DIR>
-__global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> rslt,
-                                                       R *__restrict__ cc_store,
-                                                       const R *__restrict__ cc_in,
-                                                       const R *__restrict__ dd_arr, Shape sh,
-                                                       int j_lo, int k_lo, int j_n, int k_n,
-                                                       AvScalars sc, int nbx, int nby, int nb) {
-  const int V = xcd_virtual_block(blockIdx.x, nb);
+__device__ __forceinline__ void av_generic_block(const Fld<const R> &qq, const Fld<R> &rslt,
+                                                 R *__restrict__ cc_store,
+                                                 const R *__restrict__ cc_in,
+                                                 const R *__restrict__ dd_arr, const Shape &sh,
+                                                 int j_lo, int k_lo, int j_n, int k_n,
+                                                 const AvScalars &sc, const R dt, int nbx,
+                                                 int nby, int V) {
   const int blk_x = V % nbx, rest = V / nbx;
   const int i = blk_x * 64 + (threadIdx.x & 63) + sh.mi;
   const int j = (rest % nby) * 4 + (threadIdx.x >> 6) + j_lo;
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> r
     ccr = cc_in[c + s];
   }
   const R cdw = R(0.5) * (ccl + ccc), cup = R(0.5) * (ccc + ccr);
-  const R ep = R(sc.ep), fh = R(sc.fh), dt = R(sc.dt);
+  const R ep = R(sc.ep), fh = R(sc.fh);
   const int pos = DIR == 0 ? i : (DIR == 1 ? j : k);
   const R dd = dd_arr[s ? pos : 0];
 
@@ -123,6 +123,17 @@ __global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> r
   rslt.q[8][c] = bn[3];
 }
 
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                       R *__restrict__ cc_store,
+                                                       const R *__restrict__ cc_in,
+                                                       const R *__restrict__ dd_arr, Shape sh,
+                                                       int j_lo, int k_lo, int j_n, int k_n,
+                                                       AvScalars sc, int nbx, int nby, int nb) {
+  av_generic_block<R, DIR>(qq, rslt, cc_store, cc_in, dd_arr, sh, j_lo, k_lo, j_n, k_n, sc,
+                           R(sc.dt), nbx, nby, xcd_virtual_block(blockIdx.x, nb));
+}
+
 // ---------------------------------------------------------------------------
 // y and z sweeps that walk along the sweep axis.  Lanes run along x
 // (coalesced); each thread walks `chunk` cells along DIR keeping a window of
@@ -158,7 +169,7 @@ __device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R
 // (mhd_artificial_viscosity_host.hpp:131-344)
 template <typename R>
 __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9], const R fup[9],
-                                              R dd, R dt, Fld<R> &out, int c) {
+                                              R dd, R dt, const Fld<R> &out, int c) {
   const R ro_n = a.v[0] - (fup[0] - fdw[0]) * dd * dt;
   const R vxn = (a.v[1] - (fup[1] - fdw[1]) * dd * dt) / ro_n;
   const R vyn = (a.v[2] - (fup[2] - fdw[2]) * dd * dt) / ro_n;
@@ -190,12 +201,11 @@ __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9
 }
 
 template <typename R, int DIR>
-__global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rslt,
-                                                     const R *__restrict__ cc,
-                                                     const R *__restrict__ dd_arr, Shape sh,
-                                                     int chunk, int nchunks, int nbx, int nbo,
-                                                     int nb, AvScalars sc) {
-  const int V = xcd_virtual_block(blockIdx.x, nb);
+__device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<R> &rslt,
+                                              const R *__restrict__ cc,
+                                              const R *__restrict__ dd_arr, const Shape &sh,
+                                              int chunk, int nchunks, int nbx,
+                                              const AvScalars &sc, const R dt, int V) {
   const int ch = V % nchunks, r = V / nchunks;
   const int bx = r % nbx, bo = r / nbx;
   const int i = bx * 64 + (threadIdx.x & 63) + sh.mi;
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rs
   const int len = min(chunk, n_dir - p0);
   if (len <= 0) return;
   const int s = sh.stride(DIR);
-  const R ep = R(sc.ep), fh = R(sc.fh), dt = R(sc.dt);
+  const R ep = R(sc.ep), fh = R(sc.fh);
   int c = sh.idx(i, j, k);
 
   // window: a0 = p, a1 = p+1 (and p+2 loaded per step); slopes at p; flux p-1/2
@@ -264,6 +274,17 @@ __global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rs
   }
 }
 
+template <typename R, int DIR>
+__global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                     const R *__restrict__ cc,
+                                                     const R *__restrict__ dd_arr, Shape sh,
+                                                     int chunk, int nchunks, int nbx, int nbo,
+                                                     int nb, AvScalars sc) {
+  (void)nbo;
+  av_walk_block<R, DIR>(qq, rslt, cc, dd_arr, sh, chunk, nchunks, nbx, sc, R(sc.dt),
+                        xcd_virtual_block(blockIdx.x, nb));
+}
+
 // ---------------------------------------------------------------------------
 // x sweep with lane shuffles: a wave covers x cells [i0-2, i0+62) and
 // outputs [i0, i0+60); every lane evaluates cc, the slope and the right
@@ -280,12 +301,11 @@ __device__ __forceinline__ R lane_dn(R v) {  // value of lane + 1
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void av_xshfl_kernel(Fld<const R> qq, Fld<R> rslt,
-                                                      R *__restrict__ cc_store,
-                                                      const R *__restrict__ dxi, Shape sh,
-                                                      int j_lo, int k_lo, int j_n, int k_n,
-                                                      int nseg, int nb, AvScalars sc) {
-  const int V = xcd_virtual_block(blockIdx.x, nb);
+__device__ __forceinline__ void av_xshfl_block(const Fld<const R> &qq, const Fld<R> &rslt,
+                                               R *__restrict__ cc_store,
+                                               const R *__restrict__ dxi, const Shape &sh,
+                                               int j_lo, int k_lo, int j_n, int k_n, int nseg,
+                                               const AvScalars &sc, const R dt, int V) {
   const int seg = V % nseg, r = V / nseg;
   const int lane = threadIdx.x & 63;
   const int row = r * 4 + (threadIdx.x >> 6);  // (j, k) row, j fastest
@@ -303,7 +323,7 @@ __global__ __launch_bounds__(256) void av_xshfl_kernel(Fld<const R> qq, Fld<R> r
   if (ring) return;  // uniform per wave
   AvCons<R> a;
   load_cons(qq, c, a);
-  const R ep = R(sc.ep), fh = R(sc.fh), dt = R(sc.dt);
+  const R ep = R(sc.ep), fh = R(sc.fh);
   const R cup = R(0.5) * (ccv + lane_dn(ccv));
   R fdw[9], fup[9];
 #pragma unroll
@@ -315,6 +335,16 @@ __global__ __launch_bounds__(256) void av_xshfl_kernel(Fld<const R> qq, Fld<R> r
     fdw[f] = lane_up(fup[f]);
   }
   if (out_lane) av_cell_update(a, fdw, fup, dxi[i], dt, rslt, c);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void av_xshfl_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                      R *__restrict__ cc_store,
+                                                      const R *__restrict__ dxi, Shape sh,
+                                                      int j_lo, int k_lo, int j_n, int k_n,
+                                                      int nseg, int nb, AvScalars sc) {
+  av_xshfl_block<R>(qq, rslt, cc_store, dxi, sh, j_lo, k_lo, j_n, k_n, nseg, sc, R(sc.dt),
+                    xcd_virtual_block(blockIdx.x, nb));
 }
 
 // Characteristic speed over every cell (misohip_char_vel, kept for the
@@ -400,6 +430,82 @@ hipError_t launch_char_vel(hipStream_t st, const Fld<const R> &q, R *cc, int n, 
   hipLaunchKernelGGL((char_vel_kernel<R>), dim3((n + 255) / 256), dim3(256), 0, st, q, cc, n, sc);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- batches
+template <typename R> __device__ __forceinline__ Fld<const R> av_const(const Fld<R> &f) {
+  Fld<const R> o;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) o.q[m] = f.q[m];
+  return o;
+}
+
+// One AV sweep of every instance of a batch: MODE 0 the x-shuffle sweep, 1
+// the walking y/z sweep, 2 the generic kernel (degenerate direction).  The
+// block decomposition per instance is the single-run launcher's.
+template <typename R, int DIR, int MODE>
+__global__ __launch_bounds__(256) void batch_av_kernel(const BatchInst<R> *__restrict__ inst,
+                                                       const BatchDyn<R> *__restrict__ dyn,
+                                                       const int *__restrict__ start, int n,
+                                                       int total, int in_set, int out_set,
+                                                       AvScalars sc) {
+  const int V = xcd_virtual_block(blockIdx.x, total);
+  const int m = batch_find(start, n, V);
+  const int b = V - start[m];
+  const BatchInst<R> &I = inst[m];
+  const Shape sh = I.sh;
+  const R dt = dyn[m].dt;
+  const Fld<const R> q = av_const(I.st[in_set]);
+  const Fld<R> out = I.st[out_set];
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  if (MODE == 0) {
+    const int j_lo = sh.mj - sh.js, j_n = nj + 2 * sh.js;
+    const int k_lo = sh.mk - sh.ks, k_n = nk + 2 * sh.ks;
+    av_xshfl_block<R>(q, out, I.cc, I.dxi, sh, j_lo, k_lo, j_n, k_n, (ni + 59) / 60, sc, dt, b);
+  } else if (MODE == 1) {
+    const int n_dir = DIR == 1 ? nj : nk;
+    const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
+    av_walk_block<R, DIR>(q, out, I.cc, DIR == 1 ? I.dyi : I.dzi, sh, chunk,
+                          (n_dir + chunk - 1) / chunk, (ni + 63) / 64, sc, dt, b);
+  } else {
+    const R *dd = DIR == 0 ? I.dxi : (DIR == 1 ? I.dyi : I.dzi);
+    av_generic_block<R, DIR>(q, out, I.cc, I.cc, dd, sh, sh.mj, sh.mk, nj, nk, sc, dt,
+                             (ni + 63) / 64, (nj + 3) / 4, b);
+  }
+}
+
+template <typename R, int DIR, int MODE>
+static hipError_t launch_batch_av_t(hipStream_t st, const BatchInst<R> *inst,
+                                    const BatchDyn<R> *dyn, const int *start, int n, int total,
+                                    int in_set, int out_set, const AvScalars &sc) {
+  hipLaunchKernelGGL((batch_av_kernel<R, DIR, MODE>), dim3(total), dim3(256), 0, st, inst, dyn,
+                     start, n, total, in_set, out_set, sc);
+  return hipGetLastError();
+}
+
+// mode: 0 x-shuffle (dir 0), 1 walk (dir 1, 2), 2 degenerate (dir 1, 2)
+template <typename R>
+hipError_t launch_batch_av(hipStream_t st, int dir, int mode, const BatchInst<R> *inst,
+                           const BatchDyn<R> *dyn, const int *start, int n, int total, int in_set,
+                           int out_set, const AvScalars &sc) {
+  if (dir == 0 && mode == 0)
+    return launch_batch_av_t<R, 0, 0>(st, inst, dyn, start, n, total, in_set, out_set, sc);
+  if (dir == 1 && mode == 1)
+    return launch_batch_av_t<R, 1, 1>(st, inst, dyn, start, n, total, in_set, out_set, sc);
+  if (dir == 1 && mode == 2)
+    return launch_batch_av_t<R, 1, 2>(st, inst, dyn, start, n, total, in_set, out_set, sc);
+  if (dir == 2 && mode == 1)
+    return launch_batch_av_t<R, 2, 1>(st, inst, dyn, start, n, total, in_set, out_set, sc);
+  if (dir == 2 && mode == 2)
+    return launch_batch_av_t<R, 2, 2>(st, inst, dyn, start, n, total, in_set, out_set, sc);
+  return hipErrorInvalidValue;
+}
+
+template hipError_t launch_batch_av<double>(hipStream_t, int, int, const BatchInst<double> *,
+                                            const BatchDyn<double> *, const int *, int, int, int,
+                                            int, const AvScalars &);
+template hipError_t launch_batch_av<float>(hipStream_t, int, int, const BatchInst<float> *,
+                                           const BatchDyn<float> *, const int *, int, int, int,
+                                           int, const AvScalars &);
 
 template hipError_t launch_av_sweep<double>(hipStream_t, int, const Fld<const double> &,
                                             const Fld<double> &, double *, const double *,

@@ -42,10 +42,8 @@ __device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool
 // Flattened over the shell: x slabs (all j, k), then y slabs (interior i,
 // all k), then z slabs (interior i, j).  The three sets partition the ghosts.
 template <typename R>
-__global__ __launch_bounds__(256) void ghost_fill_kernel(Fld<R> qq, Shape sh, BcDev bc, long n_x,
-                                                         long n_y, long n_tot) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tot) return;
+__device__ __forceinline__ void ghost_cell(const Fld<R> &qq, const Shape &sh, const BcDev &bc,
+                                           long n_x, long n_y, long t) {
   int i, j, k;
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
   if (t < n_x) {
@@ -84,6 +82,21 @@ __global__ __launch_bounds__(256) void ghost_fill_kernel(Fld<R> qq, Shape sh, Bc
     if (mz) v = R(bc.sign[2][f]) * v;
     qq.q[f][dst] = v;
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void ghost_fill_kernel(Fld<R> qq, Shape sh, BcDev bc, long n_x,
+                                                         long n_y, long n_tot) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tot) return;
+  ghost_cell<R>(qq, sh, bc, n_x, n_y, t);
+}
+
+__host__ __device__ inline void ghost_counts(const Shape &sh, long &n_x, long &n_y, long &n_tot) {
+  const long ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
+  n_x = 2L * sh.mi * sh.Jt * sh.Kt;
+  n_y = 2L * sh.mj * ni * sh.Kt;
+  n_tot = n_x + n_y + 2L * sh.mk * ni * nj;
 }
 
 template <typename R>
@@ -165,6 +178,152 @@ hipError_t launch_cfl(hipStream_t st, const Fld<const R> &q, const R *dx, const 
                      gm, cfl, slot);
   return hipGetLastError();
 }
+
+// ---------------------------------------------------------------- batches
+// Ghost fill of field set `set` of every instance; 256 ghost cells per block.
+template <typename R>
+__global__ __launch_bounds__(256) void batch_ghost_kernel(const BatchInst<R> *__restrict__ inst,
+                                                          const int *__restrict__ start, int n,
+                                                          int set, BcDev bc) {
+  const int m = batch_find(start, n, blockIdx.x);
+  const BatchInst<R> &I = inst[m];
+  const Shape sh = I.sh;
+  long n_x, n_y, n_tot;
+  ghost_counts(sh, n_x, n_y, n_tot);
+  const long t = (long)(blockIdx.x - start[m]) * blockDim.x + threadIdx.x;
+  if (t >= n_tot) return;
+  ghost_cell<R>(I.st[set], sh, bc, n_x, n_y, t);
+}
+
+__global__ void batch_cfl_init_kernel(unsigned *bits, int stride_words, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) bits[(long)m * stride_words] = __float_as_uint(1.e10f);
+}
+
+// CFL of every instance of the qq set: the blocks of an instance stride over
+// its interior; one atomicMin per block into the instance's slot.
+template <typename R>
+__global__ __launch_bounds__(256) void batch_cfl_kernel(const BatchInst<R> *__restrict__ inst,
+                                                        BatchDyn<R> *__restrict__ dyn,
+                                                        const int *__restrict__ start, int n,
+                                                        double gm, double cfl) {
+  const int m = batch_find(start, n, blockIdx.x);
+  const BatchInst<R> &I = inst[m];
+  const Shape sh = I.sh;
+  const Fld<const R> q = [&] {
+    Fld<const R> o;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) o.q[f] = I.st[0].q[f];
+    return o;
+  }();
+  const int nblk = start[m + 1] - start[m];
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  const long ncell = (long)ni * nj * nk;
+  unsigned best = __float_as_uint(1.e10f);
+  for (long t = (long)(blockIdx.x - start[m]) * blockDim.x + threadIdx.x; t < ncell;
+       t += (long)nblk * blockDim.x) {
+    const int i = sh.mi + (int)(t % ni);
+    const long r = t / ni;
+    const int j = sh.mj + (int)(r % nj);
+    const int k = sh.mk + (int)(r / nj);
+    const R hmin = nmin3(I.dx[i], I.dy[j], I.dz[k]);
+    best = min(best, __float_as_uint(cfl_candidate(q, sh.idx(i, j, k), R(gm), R(cfl), hmin)));
+  }
+  best = wave_umin(best);
+  __shared__ unsigned red[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) red[wv] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = min(b, red[w]);
+    atomicMin(&dyn[m].cfl_bits, b);
+  }
+}
+
+template <typename R> __device__ __forceinline__ long long key_bits(R x) {
+  if (sizeof(R) == 8) return (long long)__double_as_longlong(double(x));
+  return (long long)__float_as_uint(float(x));
+}
+
+// Step scalars of every instance from its CFL result, with the host's
+// arithmetic: dt = (Real)(float)min, ch = 0.8 cfl min_dxyz / dt evaluated in
+// double then rounded, tau = 2 dt, stage steps dt/4, dt/3, dt/2, dt, and
+// damp = exp(-dt_s / tau) from the table of host-library values.
+template <typename R>
+__global__ void batch_params_kernel(const BatchInst<R> *__restrict__ inst,
+                                    BatchDyn<R> *__restrict__ dyn, int n, double cfl_number,
+                                    DampTable<R> tab) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  BatchDyn<R> &D = dyn[m];
+  const R dt = R(__uint_as_float(D.cfl_bits));
+  const R cfl = R(cfl_number), hmin = R(inst[m].min_dxyz);
+  const R ch_divb = R(0.8 * double(cfl) * double(hmin) / double(dt));
+  const R tau = R(2.0 * double(dt));
+  const R dts[4] = {R(double(dt) / 4.0), R(double(dt) / 3.0), R(double(dt) / 2.0), dt};
+  D.dt = dt;
+  D.ch2 = ch_divb * ch_divb;
+  const R exact_q[4] = {R(-0.125), R(0), R(-0.25), R(-0.5)};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    D.dts[s] = dts[s];
+    const R q = -dts[s] / tau;
+    R v;
+    bool found = false;
+    if (s != 1) {
+      if (q == exact_q[s]) { v = tab.exact[s]; found = true; }
+    } else {
+      const long long off = key_bits(q) - key_bits(tab.q0);
+      if (off >= -16 && off <= 16) { v = tab.third[off + 16]; found = true; }
+    }
+    if (!found) {
+      v = R(exp(double(q)));
+      D.exp_fallback = 1;
+    }
+    D.damp[s] = v;
+  }
+  D.time += double(dt);
+  D.steps += 1;
+}
+
+template <typename R>
+hipError_t launch_batch_ghost(hipStream_t st, const BatchInst<R> *inst, const int *start, int n,
+                              int total, int set, const BcDev &bc) {
+  hipLaunchKernelGGL((batch_ghost_kernel<R>), dim3(total), dim3(256), 0, st, inst, start, n, set,
+                     bc);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_batch_cfl(hipStream_t st, const BatchInst<R> *inst, BatchDyn<R> *dyn,
+                            const int *start, int n, int total, double gm, double cfl) {
+  static_assert(sizeof(BatchDyn<R>) % sizeof(unsigned) == 0, "slot stride");
+  hipLaunchKernelGGL(batch_cfl_init_kernel, dim3((n + 63) / 64), dim3(64), 0, st,
+                     &dyn[0].cfl_bits, (int)(sizeof(BatchDyn<R>) / sizeof(unsigned)), n);
+  hipLaunchKernelGGL((batch_cfl_kernel<R>), dim3(total), dim3(256), 0, st, inst, dyn, start, n,
+                     gm, cfl);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_batch_params(hipStream_t st, const BatchInst<R> *inst, BatchDyn<R> *dyn, int n,
+                               double cfl_number, const DampTable<R> &tab) {
+  hipLaunchKernelGGL((batch_params_kernel<R>), dim3((n + 63) / 64), dim3(64), 0, st, inst, dyn, n,
+                     cfl_number, tab);
+  return hipGetLastError();
+}
+
+#define MISOHIP_BATCH_INST(R)                                                                     \
+  template hipError_t launch_batch_ghost<R>(hipStream_t, const BatchInst<R> *, const int *, int, \
+                                            int, int, const BcDev &);                             \
+  template hipError_t launch_batch_cfl<R>(hipStream_t, const BatchInst<R> *, BatchDyn<R> *,      \
+                                          const int *, int, int, double, double);                 \
+  template hipError_t launch_batch_params<R>(hipStream_t, const BatchInst<R> *, BatchDyn<R> *,   \
+                                             int, double, const DampTable<R> &);
+MISOHIP_BATCH_INST(double)
+MISOHIP_BATCH_INST(float)
+#undef MISOHIP_BATCH_INST
 
 // ---------------------------------------------------------- transposes
 // MISO order (i, j, k) k-fastest  <->  device order x-fastest.  For each j

@@ -58,6 +58,66 @@ hipError_t launch_rt_source(hipStream_t, const double *, const double *, double 
 hipError_t launch_rt_heat(hipStream_t, const double *, double *, const double *, const Shape &,
                           double);
 
+// ---------------------------------------------------------------- batches
+// One instance of a batched launch: its three field sets (0 = qq, 1 = argm,
+// 2 = rslt), frozen AV speed, metric and shape.  Stored in device memory;
+// a block finds its instance by a binary search over per-phase block
+// offsets.
+template <typename R> struct BatchInst {
+  Fld<R> st[3];
+  R *cc;
+  const R *dx, *dy, *dz, *dxi, *dyi, *dzi;
+  Shape sh;
+  double min_dxyz;
+};
+
+// Per-instance step scalars, produced on the device from the CFL result.
+template <typename R> struct BatchDyn {
+  unsigned cfl_bits;  // float bits of the CFL minimum
+  int exp_fallback;   // set if damp had to be evaluated on the device
+  R dt;
+  R dts[4];
+  R ch2;
+  R damp[4];
+  double time;        // simulated time of the instance
+  long steps;
+};
+
+// exp(-dt_s/tau) exactly as the host library evaluates it, tabulated for
+// the quotients the four stages can produce (stage fractions 1/4, 1/2 and 1
+// give -1/8, -1/4, -1/2 exactly; 1/3 gives a value within a few ulps of
+// -1/6).
+template <typename R> struct DampTable {
+  R exact[4];      // stages 0, 2, 3 (index by stage; [1] unused)
+  R third[33];     // stage 1: quotient bits q0 - 16 .. q0 + 16
+  R q0;            // the quotient nearest -1/6
+};
+
+__device__ __forceinline__ int batch_find(const int *__restrict__ start, int n, int b) {
+  int lo = 0, hi = n - 1;  // start[0] = 0 <= b < start[n]
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename R>
+hipError_t launch_batch_rk(hipStream_t, const BatchInst<R> *, const BatchDyn<R> *, const int *,
+                           int, int, int, int, int, const RkScalars &, int);
+template <typename R>
+hipError_t launch_batch_ghost(hipStream_t, const BatchInst<R> *, const int *, int, int, int,
+                              const BcDev &);
+template <typename R>
+hipError_t launch_batch_cfl(hipStream_t, const BatchInst<R> *, BatchDyn<R> *, const int *, int,
+                            int, double, double);
+template <typename R>
+hipError_t launch_batch_params(hipStream_t, const BatchInst<R> *, BatchDyn<R> *, int, double,
+                               const DampTable<R> &);
+template <typename R>
+hipError_t launch_batch_av(hipStream_t, int, int, const BatchInst<R> *, const BatchDyn<R> *,
+                           const int *, int, int, int, int, const AvScalars &);
+
 template <typename R>
 hipError_t launch_rk_stage(hipStream_t, const Fld<const R> &, const Fld<const R> &,
                            const Fld<R> &, const R *, const R *, const R *, const Shape &,

@@ -102,20 +102,22 @@ __device__ __forceinline__ void derivs_or_zero(const Fld<const R> &a, int c, int
 
 // Energy recovery is carried in double for both storage types ("pressure
 // recovery from total energy" is the cancellation-prone step).
+// One 64 x 4 block of cells (x-tile blk_x, y-rows 4*blk_y.., plane blk_z);
+// the stage step dt, ch2 and damp come as arguments so that a batched launch
+// can take them per instance from device memory.
 template <typename R, bool HX, bool HY, bool HZ>
-__global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<const R> argm,
-                                                       Fld<R> rslt, const R *__restrict__ dxi,
-                                                       const R *__restrict__ dyi,
-                                                       const R *__restrict__ dzi, Shape sh,
-                                                       RkScalars sc, Pencils pc) {
-  int blk_x, blk_y, blk_z;
-  if (!pc.decode(xcd_virtual_block(blockIdx.x, pc.nb), blk_x, blk_y, blk_z)) return;
+__device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<const R> &argm,
+                                         const Fld<R> &rslt, const R *__restrict__ dxi,
+                                         const R *__restrict__ dyi, const R *__restrict__ dzi,
+                                         const Shape &sh, const RkScalars &sc, const R dt,
+                                         const R ch2, const R damp, int blk_x, int blk_y,
+                                         int blk_z) {
   const int i = blk_x * 64 + (threadIdx.x & 63) + sh.mi;
   const int j = blk_y * 4 + (threadIdx.x >> 6) + sh.mj;
   const int k = blk_z + sh.mk;
   if (i >= sh.It - sh.mi || j >= sh.Jt - sh.mj) return;
   const int c = sh.idx(i, j, k);
-  const R gm1 = R(sc.gm) - R(1), dt = R(sc.dt);
+  const R gm1 = R(sc.gm) - R(1);
   const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
   R S[17];
 
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<co
   const R by = o_by + dt * a_by;
   const R bz = o_bz + dt * a_bz;
   // GLM divergence cleaning; damp = exp(-dt/tau) from the host
-  const R ph = (o_ph + dt * R(sc.ch2) * a_ph) * R(sc.damp);
+  const R ph = (o_ph + dt * ch2 * a_ph) * damp;
   // energy: total energy of orgn, flux divergence, recovery of ei
   const double eo_ro = o_ro;
   const double Et = +eo_ro * double(o_ei) +
@@ -219,6 +221,18 @@ __global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<co
   rslt.q[6][c] = bz;
   rslt.q[7][c] = R(ei);
   rslt.q[8][c] = ph;
+}
+
+template <typename R, bool HX, bool HY, bool HZ>
+__global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<const R> argm,
+                                                       Fld<R> rslt, const R *__restrict__ dxi,
+                                                       const R *__restrict__ dyi,
+                                                       const R *__restrict__ dzi, Shape sh,
+                                                       RkScalars sc, Pencils pc) {
+  int blk_x, blk_y, blk_z;
+  if (!pc.decode(xcd_virtual_block(blockIdx.x, pc.nb), blk_x, blk_y, blk_z)) return;
+  rk_block<R, HX, HY, HZ>(orgn, argm, rslt, dxi, dyi, dzi, sh, sc, R(sc.dt), R(sc.ch2),
+                          R(sc.damp), blk_x, blk_y, blk_z);
 }
 
 template <typename R, bool HX, bool HY, bool HZ>
@@ -250,6 +264,69 @@ hipError_t launch_rk_stage(hipStream_t st, const Fld<const R> &o, const Fld<cons
     default: return launch_rk_t<R, false, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
   }
 }
+
+// ---------------------------------------------------------------- batches
+template <typename R> __device__ __forceinline__ Fld<const R> as_const(const Fld<R> &f) {
+  Fld<const R> o;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) o.q[m] = f.q[m];
+  return o;
+}
+
+// One RK stage of every instance of a batch in one launch: orgn = qq, argm =
+// field set in_set, rslt = out_set; the stage step, ch2 and damp are the
+// instance's own (device memory).  Blocks run 64 x 4 cells, x fastest.
+template <typename R, bool HX, bool HY, bool HZ>
+__global__ __launch_bounds__(256) void batch_rk_kernel(const BatchInst<R> *__restrict__ inst,
+                                                       const BatchDyn<R> *__restrict__ dyn,
+                                                       const int *__restrict__ start, int n,
+                                                       int total, int stage, int in_set,
+                                                       int out_set, RkScalars sc) {
+  const int V = xcd_virtual_block(blockIdx.x, total);
+  const int m = batch_find(start, n, V);
+  const int b = V - start[m];
+  const BatchInst<R> &I = inst[m];
+  const Shape sh = I.sh;
+  const int nbx = (sh.It - 2 * sh.mi + 63) / 64, nby = (sh.Jt - 2 * sh.mj + 3) / 4;
+  const int blk_x = b % nbx, r = b / nbx;
+  const BatchDyn<R> &D = dyn[m];
+  rk_block<R, HX, HY, HZ>(as_const(I.st[0]), as_const(I.st[in_set]), I.st[out_set], I.dxi, I.dyi,
+                          I.dzi, sh, sc, D.dts[stage], D.ch2, D.damp[stage], blk_x, r % nby,
+                          r / nby);
+}
+
+template <typename R, bool HX, bool HY, bool HZ>
+static hipError_t launch_batch_rk_t(hipStream_t st, const BatchInst<R> *inst,
+                                    const BatchDyn<R> *dyn, const int *start, int n, int total,
+                                    int stage, int in_set, int out_set, const RkScalars &sc) {
+  hipLaunchKernelGGL((batch_rk_kernel<R, HX, HY, HZ>), dim3(total), dim3(256), 0, st, inst, dyn,
+                     start, n, total, stage, in_set, out_set, sc);
+  return hipGetLastError();
+}
+
+// key = (is ? 4 : 0) | (js ? 2 : 0) | (ks ? 1 : 0), the same for every instance
+template <typename R>
+hipError_t launch_batch_rk(hipStream_t st, const BatchInst<R> *inst, const BatchDyn<R> *dyn,
+                           const int *start, int n, int total, int stage, int in_set,
+                           int out_set, const RkScalars &sc, int key) {
+  switch (key) {
+    case 7: return launch_batch_rk_t<R, true, true, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 6: return launch_batch_rk_t<R, true, true, false>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 5: return launch_batch_rk_t<R, true, false, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 3: return launch_batch_rk_t<R, false, true, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 4: return launch_batch_rk_t<R, true, false, false>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 2: return launch_batch_rk_t<R, false, true, false>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 1: return launch_batch_rk_t<R, false, false, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_batch_rk<double>(hipStream_t, const BatchInst<double> *,
+                                            const BatchDyn<double> *, const int *, int, int, int,
+                                            int, int, const RkScalars &, int);
+template hipError_t launch_batch_rk<float>(hipStream_t, const BatchInst<float> *,
+                                           const BatchDyn<float> *, const int *, int, int, int,
+                                           int, int, const RkScalars &, int);
 
 template hipError_t launch_rk_stage<double>(hipStream_t, const Fld<const double> &,
                                             const Fld<const double> &, const Fld<double> &,

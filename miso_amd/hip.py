@@ -106,6 +106,13 @@ SIGNATURES = {
                          C.c_void_p, C.c_void_p, C.c_void_p],
     "misohip_mlp_residual": [C.c_void_p, C.c_void_p, _P(MisohipGrid), C.c_void_p, C.c_void_p,
                              C.c_void_p, C.c_double],
+    "misohip_batch_create": [C.c_void_p, C.c_int, C.c_int, _P(MisohipGrid), _P(MisohipFields),
+                             _P(MisohipWork), _P(MisohipParams), _P(MisohipBc), _P(C.c_void_p)],
+    "misohip_batch_destroy": [C.c_void_p],
+    "misohip_batch_capture": [C.c_void_p, C.c_void_p, C.c_int],
+    "misohip_batch_step": [C.c_void_p, C.c_void_p, C.c_int],
+    "misohip_batch_state": [C.c_void_p, C.c_void_p, _P(C.c_double), _P(C.c_double),
+                            _P(C.c_long), _P(C.c_int)],
     "misohip_set_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_get_tiles": [C.c_void_p, _P(MisohipTiles)],
     "misohip_profile_enable": [C.c_void_p, C.c_int],
