@@ -299,6 +299,9 @@ int misohip_batch_capture(misohip_ctx *ctx, misohip_batch *b, int steps_per_grap
 /* Advance every run by nsteps (a multiple of the captured step count when a
  * graph exists); asynchronous. */
 int misohip_batch_step(misohip_ctx *ctx, misohip_batch *b, int nsteps);
+/* Per-run walk length (cells per thread, 4..4096) of the y/z AV sweeps,
+ * as a tile autotuner picks it per run size; drops a captured graph. */
+int misohip_batch_set_chunks(misohip_ctx *ctx, misohip_batch *b, const int *chunks);
 /* Last dt, simulated time and step count of each run (synchronises); any
  * output may be NULL.  *exp_fallback != 0 reports that a GLM damping factor
  * was evaluated on the device rather than taken from the host table. */

@@ -208,8 +208,10 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
                                               const AvScalars &sc, const R dt, int V) {
   const int ch = V % nchunks, r = V / nchunks;
   const int bx = r % nbx, bo = r / nbx;
-  const int i = bx * 64 + (threadIdx.x & 63) + sh.mi;
-  const int o = bo * 4 + (threadIdx.x >> 6);  // the other transverse axis
+  const int wx = walk_waves_x(DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj);
+  const int wave = threadIdx.x >> 6;
+  const int i = (bx * wx + wave % wx) * 64 + (threadIdx.x & 63) + sh.mi;
+  const int o = bo * (4 / wx) + wave / wx;  // the other transverse axis
   if (i >= sh.It - sh.mi) return;
   int j, k, m_dir, n_dir;
   if (DIR == 1) {
@@ -384,7 +386,9 @@ static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld
   const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
   const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
   const int nchunks = (n_dir + chunk - 1) / chunk;
-  const int nbx = (sh.It - 2 * sh.mi + 63) / 64, nbo = (n_oth + 3) / 4;
+  const int wx = walk_waves_x(n_oth);
+  const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
+  const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
   const long nbl = (long)nchunks * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   hipLaunchKernelGGL((av_walk_kernel<R, DIR>), dim3((unsigned)nbl), dim3(256), 0, st, q, r, cc_in,
@@ -463,9 +467,10 @@ __global__ __launch_bounds__(256) void batch_av_kernel(const BatchInst<R> *__res
     av_xshfl_block<R>(q, out, I.cc, I.dxi, sh, j_lo, k_lo, j_n, k_n, (ni + 59) / 60, sc, dt, b);
   } else if (MODE == 1) {
     const int n_dir = DIR == 1 ? nj : nk;
-    const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
+    const int chunk = I.chunk;
+    const int wx = walk_waves_x(DIR == 1 ? nk : nj);
     av_walk_block<R, DIR>(q, out, I.cc, DIR == 1 ? I.dyi : I.dzi, sh, chunk,
-                          (n_dir + chunk - 1) / chunk, (ni + 63) / 64, sc, dt, b);
+                          (n_dir + chunk - 1) / chunk, ((ni + 63) / 64 + wx - 1) / wx, sc, dt, b);
   } else {
     const R *dd = DIR == 0 ? I.dxi : (DIR == 1 ? I.dyi : I.dzi);
     av_generic_block<R, DIR>(q, out, I.cc, I.cc, dd, sh, sh.mj, sh.mk, nj, nk, sc, dt,

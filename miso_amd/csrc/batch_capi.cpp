@@ -35,7 +35,7 @@ template <typename R> struct BatchT {
   BatchDyn<R> *d_dyn = nullptr;
   int *d_start = nullptr;  // P_COUNT arrays of n + 1 block offsets
   DampTable<R> tab{};
-  int chunk = 64;
+  std::vector<BatchInst<R>> h_inst;  // host copy of the instance table
 };
 
 }  // namespace
@@ -71,12 +71,18 @@ void phase_blocks(const Shape &sh, int chunk, int mode_y, int mode_z, long out[P
   out[P_CFL] = c < 1 ? 1 : (c > 512 ? 512 : c);
   const long j_n = nj + 2 * sh.js, k_n = nk + 2 * sh.ks;
   out[P_AVX] = ((ni + 59) / 60) * ((j_n * k_n + 3) / 4);
+  // walking sweeps: chunks x x-blocks x other-axis blocks (walk_waves_x)
+  auto walk = [&](long n_dir, long n_oth) {
+    const long wx = walk_waves_x((int)n_oth), wo = 4 / wx;
+    return ((n_dir + chunk - 1) / chunk) * (((ni + 63) / 64 + wx - 1) / wx) *
+           ((n_oth + wo - 1) / wo);
+  };
   if (mode_y == 1)
-    out[P_AVY] = ((nj + chunk - 1) / chunk) * ((ni + 63) / 64) * ((nk + 3) / 4);
+    out[P_AVY] = walk(nj, nk);
   else
     out[P_AVY] = ((ni + 63) / 64) * ((nj + 3) / 4) * nk;
   if (mode_z == 1)
-    out[P_AVZ] = ((nk + chunk - 1) / chunk) * ((ni + 63) / 64) * ((nj + 3) / 4);
+    out[P_AVZ] = walk(nk, nj);
   else
     out[P_AVZ] = ((ni + 63) / 64) * ((nj + 3) / 4) * nk;
 }
@@ -96,6 +102,29 @@ template <typename R> void fill_table(DampTable<R> &t) {
   }
 }
 
+// Block offsets of every phase from the instance table, and both tables
+// to the device (synchronous).
+template <typename R> int upload_layout(BatchT<R> *b) {
+  const int n = b->n;
+  std::vector<int> start((size_t)P_COUNT * (n + 1), 0);
+  long tot[P_COUNT] = {};
+  for (int m = 0; m < n; ++m) {
+    long blk[P_COUNT];
+    phase_blocks(b->h_inst[m].sh, b->h_inst[m].chunk, b->mode_y, b->mode_z, blk);
+    for (int ph = 0; ph < P_COUNT; ++ph) {
+      tot[ph] += blk[ph];
+      if (tot[ph] > 2147483647L) return MISOHIP_E_RANGE;
+      start[(size_t)ph * (n + 1) + m + 1] = (int)tot[ph];
+    }
+  }
+  for (int ph = 0; ph < P_COUNT; ++ph) b->total[ph] = (int)tot[ph];
+  MISOHIP_TRY(hipMemcpy(b->d_inst, b->h_inst.data(), sizeof(BatchInst<R>) * n,
+                        hipMemcpyHostToDevice));
+  MISOHIP_TRY(hipMemcpy(b->d_start, start.data(), sizeof(int) * start.size(),
+                        hipMemcpyHostToDevice));
+  return MISOHIP_OK;
+}
+
 template <typename R>
 int create_t(misohip_ctx *ctx, int n, const misohip_grid *grids, const misohip_fields *qq,
              const misohip_work *work, const misohip_params *p, const misohip_bc *bc,
@@ -105,16 +134,15 @@ int create_t(misohip_ctx *ctx, int n, const misohip_grid *grids, const misohip_f
   if (!b) return MISOHIP_E_NOMEM;
   b->n = n;
   b->p = *p;
-  b->chunk = ctx->tile.sweep_chunk < 4 ? 4 : ctx->tile.sweep_chunk;
+  const int chunk = ctx->tile.sweep_chunk < 4 ? 4 : ctx->tile.sweep_chunk;
   BcDev bd;
   for (int a = 0; a < 3; ++a) {
     bd.periodic[a] = bc->periodic[a] == 2 ? 2 : (bc->periodic[a] ? 1 : 0);
     for (int f = 0; f < 9; ++f) bd.sign[a][f] = bc->sym_sign[a][f] >= 0 ? 1.0f : -1.0f;
   }
   b->bc = bd;
-  std::vector<BatchInst<R>> inst(n);
-  std::vector<int> start((size_t)P_COUNT * (n + 1), 0);
-  long tot[P_COUNT] = {};
+  std::vector<BatchInst<R>> &inst = b->h_inst;
+  inst.resize(n);
   for (int m = 0; m < n; ++m) {
     const misohip_grid *g = &grids[m];
     const int key = (g->is ? 4 : 0) | (g->js ? 2 : 0) | (g->ks ? 1 : 0);
@@ -143,27 +171,23 @@ int create_t(misohip_ctx *ctx, int n, const misohip_grid *grids, const misohip_f
     I.dyi = static_cast<const R *>(g->dyi); I.dzi = static_cast<const R *>(g->dzi);
     I.sh = shape_of(g);
     I.min_dxyz = g->min_dxyz;
-    long blk[P_COUNT];
-    phase_blocks(I.sh, b->chunk, b->mode_y, b->mode_z, blk);
-    for (int ph = 0; ph < P_COUNT; ++ph) {
-      tot[ph] += blk[ph];
-      if (tot[ph] > 2147483647L) {
-        delete b;
-        return MISOHIP_E_RANGE;
-      }
-      start[(size_t)ph * (n + 1) + m + 1] = (int)tot[ph];
-    }
+    I.chunk = chunk;
   }
-  for (int ph = 0; ph < P_COUNT; ++ph) b->total[ph] = (int)tot[ph];
   fill_table(b->tab);
   hipError_t e = hipMalloc(reinterpret_cast<void **>(&b->d_inst), sizeof(BatchInst<R>) * n);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&b->d_dyn), sizeof(BatchDyn<R>) * n);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&b->d_start), sizeof(int) * start.size());
   if (e == hipSuccess)
-    e = hipMemcpy(b->d_inst, inst.data(), sizeof(BatchInst<R>) * n, hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(b->d_start, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice);
+    e = hipMalloc(reinterpret_cast<void **>(&b->d_start), sizeof(int) * P_COUNT * (n + 1));
   if (e == hipSuccess) e = hipMemset(b->d_dyn, 0, sizeof(BatchDyn<R>) * n);
+  int rc = MISOHIP_OK;
+  if (e == hipSuccess) rc = upload_layout(b);
+  if (rc) {
+    (void)hipFree(b->d_inst);
+    (void)hipFree(b->d_dyn);
+    (void)hipFree(b->d_start);
+    delete b;
+    return rc;
+  }
   if (e != hipSuccess) {
     (void)hipFree(b->d_inst);
     (void)hipFree(b->d_dyn);
@@ -208,7 +232,7 @@ template <typename R> hipError_t step_t(hipStream_t st, BatchT<R> *b) {
   AvScalars av{};
   av.ep = (R)b->p.ep; av.fh = (R)b->p.fh; av.gm = (R)b->p.gm;
   av.cs_fac = (R)b->p.cs_fac; av.ca_fac = (R)b->p.ca_fac; av.vv_fac = (R)b->p.vv_fac;
-  av.chunk = b->chunk;
+  av.chunk = 0;  // per instance (BatchInst::chunk)
   // x: argm(1) -> rslt(2); y: rslt(2) -> argm(1); z: argm(1) -> qq(0)
   const int av_in[3] = {1, 2, 1}, av_out[3] = {2, 1, 0};
   const int mode[3] = {0, b->mode_y, b->mode_z};
@@ -328,6 +352,34 @@ int misohip_batch_step(misohip_ctx *ctx, misohip_batch *b, int nsteps) {
     return MISOHIP_OK;
   }
   for (int s = 0; s < nsteps; ++s) MISOHIP_TRY(step_any(ctx->stream, b));
+  return MISOHIP_OK;
+}
+
+// Per-run walk length of the y/z sweeps (>= 4), e.g. chosen by a tile
+// autotuner for each run size; drops a captured graph.
+int misohip_batch_set_chunks(misohip_ctx *ctx, misohip_batch *b, const int *chunks) {
+  if (!ctx || !b || !chunks) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+  MISOHIP_TRY(hipStreamSynchronize(b->stream));
+  auto apply = [&](auto *t) -> int {
+    for (int m = 0; m < t->n; ++m)
+      if (chunks[m] < 4 || chunks[m] > 4096) return MISOHIP_E_ARG;
+    for (int m = 0; m < t->n; ++m) t->h_inst[m].chunk = chunks[m];
+    return upload_layout(t);
+  };
+  const int rc = b->dtype == MISOHIP_F64 ? apply(static_cast<BatchT<double> *>(b->impl))
+                                         : apply(static_cast<BatchT<float> *>(b->impl));
+  if (rc) return rc;
+  if (b->exec) {
+    MISOHIP_TRY(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  if (b->graph) {
+    MISOHIP_TRY(hipGraphDestroy(b->graph));
+    b->graph = nullptr;
+  }
+  b->steps_per_graph = 0;
   return MISOHIP_OK;
 }
 

@@ -58,6 +58,11 @@ hipError_t launch_rt_source(hipStream_t, const double *, const double *, double 
 hipError_t launch_rt_heat(hipStream_t, const double *, double *, const double *, const Shape &,
                           double);
 
+// Waves of a walking-sweep block laid along x: all four when the other
+// transverse axis has fewer than four cells (2D runs), else one (the four
+// waves then take four rows of that axis).
+__host__ __device__ inline int walk_waves_x(int n_other) { return n_other < 4 ? 4 : 1; }
+
 // ---------------------------------------------------------------- batches
 // One instance of a batched launch: its three field sets (0 = qq, 1 = argm,
 // 2 = rslt), frozen AV speed, metric and shape.  Stored in device memory;
@@ -69,6 +74,7 @@ template <typename R> struct BatchInst {
   const R *dx, *dy, *dz, *dxi, *dyi, *dzi;
   Shape sh;
   double min_dxyz;
+  int chunk;  // cells walked per thread by the y/z sweeps of this instance
 };
 
 // Per-instance step scalars, produced on the device from the CFL result.

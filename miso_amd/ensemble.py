@@ -59,6 +59,11 @@ class Ensemble:
     def cells(self) -> int:
         return sum(r.grid.n_interior for r in self.runs)
 
+    def set_chunks(self, chunks) -> None:
+        """Walk length of the y/z sweeps per run (drops a captured graph)."""
+        arr = (C.c_int * self.n)(*[int(c) for c in chunks])
+        hip.call("misohip_batch_set_chunks", self.ctx.handle, self.h, arr)
+
     def capture(self, steps_per_graph: int) -> None:
         hip.call("misohip_batch_capture", self.ctx.handle, self.h, int(steps_per_graph))
 

@@ -111,6 +111,7 @@ SIGNATURES = {
     "misohip_batch_destroy": [C.c_void_p],
     "misohip_batch_capture": [C.c_void_p, C.c_void_p, C.c_int],
     "misohip_batch_step": [C.c_void_p, C.c_void_p, C.c_int],
+    "misohip_batch_set_chunks": [C.c_void_p, C.c_void_p, _P(C.c_int)],
     "misohip_batch_state": [C.c_void_p, C.c_void_p, _P(C.c_double), _P(C.c_double),
                             _P(C.c_long), _P(C.c_int)],
     "misohip_set_tiles": [C.c_void_p, _P(MisohipTiles)],
