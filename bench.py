@@ -45,6 +45,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--no-rmhd", action="store_true", help="skip the radiative-MHD leg")
     ap.add_argument("--rmhd-only", action="store_true", help="run only the radiative-MHD leg")
     ap.add_argument("--rmhd-n", type=int, default=384, help="learned-RT box, cells per side")
@@ -163,6 +164,12 @@ def ensemble_leg(args):
     t0 = time.perf_counter()
     ens = Ensemble(ctx, [(n, n, 1) for n in sizes], init, dtype=np.float32)
     setup_s = time.perf_counter() - t0
+    tuned = None
+    if not args.no_tune:
+        from miso_amd.autotune import summary, tune_ensemble
+        tuned = summary(tune_ensemble(
+            ctx, ens, lambda s: Ensemble(ctx, [s, s], init, dtype=np.float32)))
+        tuned.pop("times_ms", None)
     k = args.ens_graph_steps
     ens.capture(k)
     ens.step(k)  # warm-up replay
@@ -197,6 +204,7 @@ def ensemble_leg(args):
         "unbatched_cell_updates_per_s": cells * seq_steps / seq,
         "speedup_vs_unbatched": value / (cells * seq_steps / seq),
         "exp_fallback": fb,
+        "tiles": tuned,
         "dt_range": [float(dt.min()), float(dt.max())],
         "setup_seconds": setup_s,
     }
@@ -362,6 +370,11 @@ def main():
     del q0
     torch.cuda.synchronize()
     m.integrator.apply_boundary_condition(bc, m.qq)
+    tiles = None
+    if not args.no_tune:
+        from miso_amd.autotune import summary, tune_single
+        tiles = summary(tune_single(ctx, m, bc, steps=1))
+        print(f"[tune] {tiles}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         m.integrator.step(bc, m.qq)
@@ -433,6 +446,7 @@ def main():
             "traffic_source": f"profiles/mhd_step_turbulence_{n}cubed_{args.dtype}.md",
         },
         "kernels": kernels,
+        "tiles": tiles,
         "dt_last": dts[-1],
     }
     ctx.close()

@@ -1,0 +1,175 @@
+"""Tile autotuner (no counterpart in the reference).
+
+Picks, per device, kernel and problem size, the tile shapes the HIP kernels
+expose, by timing the candidates on the real state:
+
+- single runs (`tune_single`): the RK stage's pencil shape
+  (`rk_pencil_bx` x `rk_pencil_by` blocks of 64 x 4 cells walked along z)
+  and the walk length of the y/z AV sweeps (`sweep_chunk`), timed with the
+  library's own HIP events (misohip_profile_*) over a few steps;
+- batches (`tune_ensemble`): the y/z sweep walk length per run size, timed
+  on a probe batch of that size.
+
+The state is restored after tuning, so tuning changes no result (every
+candidate computes the same values; only the launch shape differs).
+Choices are cached in a JSON file ($MISO_AMD_TILE_CACHE, default
+~/.cache/miso_amd/tiles.json) keyed by device, kernel and size.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import time
+
+from . import hip
+
+RK_PENCILS = ((1, 4), (1, 8), (2, 4), (2, 8), (2, 16), (4, 8), (4, 16))
+CHUNKS = (16, 32, 64, 128, 256)
+
+
+def _device_key() -> str:
+    import torch
+    p = torch.cuda.get_device_properties(0)
+    return f"{p.name}|{getattr(p, 'gcnArchName', '')}|{p.multi_processor_count}"
+
+
+class TileCache:
+    def __init__(self, path: str | None = None):
+        self.path = path or os.environ.get("MISO_AMD_TILE_CACHE") or os.path.join(
+            os.path.expanduser("~"), ".cache", "miso_amd", "tiles.json")
+        self.data = {}
+        try:
+            with open(self.path) as f:
+                self.data = json.load(f)
+        except (OSError, ValueError):
+            self.data = {}
+
+    def get(self, key):
+        return self.data.get(key)
+
+    def put(self, key, value) -> None:
+        self.data[key] = value
+        try:
+            os.makedirs(os.path.dirname(self.path), exist_ok=True)
+            with open(self.path, "w") as f:
+                json.dump(self.data, f, indent=1, sort_keys=True)
+        except OSError:
+            pass  # an unwritable cache only costs a re-tune
+
+
+def get_tiles(ctx):
+    t = hip.MisohipTiles()
+    hip.call("misohip_get_tiles", ctx.handle, C.byref(t))
+    return t
+
+
+def set_tiles(ctx, pencil_bx: int, pencil_by: int, chunk: int) -> None:
+    t = hip.MisohipTiles()
+    t.rk_pencil_bx, t.rk_pencil_by, t.sweep_chunk = int(pencil_bx), int(pencil_by), int(chunk)
+    hip.call("misohip_set_tiles", ctx.handle, C.byref(t))
+
+
+def _profile(ctx, fn):
+    """Run fn with the library's event timing on; returns {kind: ms per launch}."""
+    L = hip.lib()
+    nk = len(hip.KIND_NAMES)
+    hip.check(L.misohip_profile_enable(ctx.handle, 1))
+    fn()
+    ms = (C.c_double * nk)()
+    cnt = (C.c_int * nk)()
+    hip.check(L.misohip_profile_read(ctx.handle, ms, cnt, nk))
+    hip.check(L.misohip_profile_enable(ctx.handle, 0))
+    return {hip.KIND_NAMES[k]: ms[k] / cnt[k] for k in range(nk) if cnt[k]}
+
+
+def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNKS,
+                cache: TileCache | None = None, force: bool = False):
+    """Choose and set the RK pencil and sweep chunk for `model` (a mhd.MHD);
+    returns {"rk_pencil": (bx, by), "sweep_chunk": c, "times_ms": {...},
+    "seconds": tuning time, "cached": bool}."""
+    import torch
+    cache = cache or TileCache()
+    g = model.grid
+    key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}"
+    hit = None if force else cache.get(key)
+    if hit:
+        set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"])
+        return dict(hit, cached=True, seconds=0.0)
+    t0 = time.perf_counter()
+    saved = model.qq.data.clone()
+    base = get_tiles(ctx)
+    integ = model.integrator
+
+    def run():
+        for _ in range(steps):
+            integ.step(bc, model.qq)
+        model.qq.data.copy_(saved)
+
+    run()  # warm-up (code objects, caches)
+    rk_times = {}
+    for bx, by in pencils:
+        set_tiles(ctx, bx, by, base.sweep_chunk)
+        rk_times[(bx, by)] = _profile(ctx, run).get("rk_stage", float("inf"))
+    best_p = min(rk_times, key=rk_times.get)
+    sweep_times = {}
+    active = [k for k, s in (("av_y", g.j_size), ("av_z", g.k_size)) if s > 1]
+    for c in chunks:
+        set_tiles(ctx, best_p[0], best_p[1], c)
+        t = _profile(ctx, run)
+        sweep_times[c] = sum(t.get(k, 0.0) for k in active)
+    best_c = min(sweep_times, key=sweep_times.get)
+    set_tiles(ctx, best_p[0], best_p[1], best_c)
+    torch.cuda.synchronize()
+    out = {"rk_pencil": list(best_p), "sweep_chunk": int(best_c),
+           "times_ms": {"rk_stage": {f"{a}x{b}": v for (a, b), v in rk_times.items()},
+                        "av_yz": {str(k): v for k, v in sweep_times.items()}}}
+    cache.put(key, out)
+    return dict(out, cached=False, seconds=time.perf_counter() - t0)
+
+
+def tune_ensemble(ctx, ens, make_probe, chunks=CHUNKS, reps: int = 3,
+                  cache: TileCache | None = None, force: bool = False):
+    """Choose the y/z sweep walk length per run size of `ens` (an
+    ensemble.Ensemble) and apply it.  make_probe(size) builds a small
+    Ensemble of runs of that size (the probe is timed, the real runs are
+    untouched).  Returns {size: chunk}, the timings and the tuning time."""
+    import torch
+    cache = cache or TileCache()
+    t0 = time.perf_counter()
+    sizes = sorted({(r.grid.i_size, r.grid.j_size, r.grid.k_size) for r in ens.runs})
+    choice, times = {}, {}
+    for s in sizes:
+        key = f"{_device_key()}|batch|{s[0]}x{s[1]}x{s[2]}|{ens.dtype.name}"
+        hit = None if force else cache.get(key)
+        if hit:
+            choice[s] = hit["chunk"]
+            continue
+        probe = make_probe(s)
+        tt = {}
+        for c in chunks:
+            probe.set_chunks([c] * probe.n)
+            probe.step(1)  # warm-up
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            probe.step(reps)
+            e1.record()
+            torch.cuda.synchronize()
+            tt[c] = e0.elapsed_time(e1) / reps
+        probe.close()
+        best = min(tt, key=tt.get)
+        choice[s] = int(best)
+        times[f"{s[0]}x{s[1]}x{s[2]}"] = {str(k): v for k, v in tt.items()}
+        cache.put(key, {"chunk": int(best), "times_ms": times[f"{s[0]}x{s[1]}x{s[2]}"]})
+    ens.set_chunks([choice[(r.grid.i_size, r.grid.j_size, r.grid.k_size)] for r in ens.runs])
+    return {"chunks": {f"{a}x{b}x{c}": v for (a, b, c), v in choice.items()}, "times_ms": times,
+            "seconds": time.perf_counter() - t0}
+
+
+def summary(d) -> dict:
+    """JSON-safe copy of a tuning result."""
+    return json.loads(json.dumps(d, default=lambda o: list(o) if isinstance(o, tuple) else str(o)))
+
+
+__all__ = ["TileCache", "tune_single", "tune_ensemble", "set_tiles", "get_tiles", "summary",
+           "RK_PENCILS", "CHUNKS"]

@@ -1,0 +1,63 @@
+"""Tile autotuner: the cache (CPU); on the GPU, tuning restores the state,
+picks a candidate, and a tuned launch shape computes the same bits as the
+default one."""
+import numpy as np
+import pytest
+
+from miso_amd.autotune import TileCache
+
+
+def test_tile_cache_round_trip(tmp_path):
+    p = str(tmp_path / "t.json")
+    c = TileCache(p)
+    assert c.get("k") is None
+    c.put("k", {"chunk": 64})
+    assert TileCache(p).get("k") == {"chunk": 64}
+
+
+@pytest.mark.gpu
+def test_tune_single_keeps_results(ctx, tmp_path):
+    import torch
+    from miso_amd import mhd, problems
+    from miso_amd.autotune import set_tiles, tune_single
+    g = mhd.Grid(48, 40, 36, 2, device="cuda:0")
+    bc = mhd.BoundaryCondition((True, True, True))
+    cfg = mhd.Config({"mhd": {"cfl_number": 0.4}})
+    m = mhd.MHD(cfg, g, ctx, 5.0 / 3.0)
+    m.qq.load_ijk(problems.turbulence(g, device="cpu"))
+    m.integrator.apply_boundary_condition(bc, m.qq)
+    before = m.qq.data.clone()
+    res = tune_single(ctx, m, bc, steps=1, cache=TileCache(str(tmp_path / "c.json")))
+    assert torch.equal(m.qq.data, before)
+    assert res["sweep_chunk"] in (16, 32, 64, 128, 256) and len(res["rk_pencil"]) == 2
+    m.integrator.step(bc, m.qq)
+    tuned = m.qq.data.clone()
+    m.qq.data.copy_(before)
+    set_tiles(ctx, 2, 8, 64)
+    m.integrator.step(bc, m.qq)
+    assert torch.equal(m.qq.data, tuned)
+    # a second call hits the cache
+    assert tune_single(ctx, m, bc, cache=TileCache(str(tmp_path / "c.json")))["cached"]
+
+
+@pytest.mark.gpu
+def test_tune_ensemble_keeps_results(ctx, tmp_path):
+    import torch
+    from miso_amd import problems
+    from miso_amd.autotune import tune_ensemble
+    from miso_amd.ensemble import Ensemble
+
+    def init(m, g):
+        return problems.kelvin_helmholtz(g, seed=4 + m, device="cpu")
+    shapes = [(64, 64, 1), (128, 128, 1), (64, 64, 1)]
+    a = Ensemble(ctx, shapes, init, dtype=np.float32)
+    b = Ensemble(ctx, shapes, init, dtype=np.float32)
+    res = tune_ensemble(ctx, b, lambda s: Ensemble(ctx, [s, s], init, dtype=np.float32),
+                        cache=TileCache(str(tmp_path / "e.json")))
+    assert set(res["chunks"]) == {"64x64x1", "128x128x1"}
+    a.step(2)
+    b.step(2)
+    for m in range(len(shapes)):
+        assert torch.equal(a.runs[m].qq.data, b.runs[m].qq.data)
+    a.close()
+    b.close()
