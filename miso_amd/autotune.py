@@ -4,8 +4,9 @@ Picks, per device, kernel and problem size, the tile shapes the HIP kernels
 expose, by timing the candidates on the real state:
 
 - single runs (`tune_single`): the RK stage's pencil shape
-  (`rk_pencil_bx` x `rk_pencil_by` blocks of 64 x 4 cells walked along z)
-  and the walk length of the y/z AV sweeps (`sweep_chunk`), timed with the
+  (`rk_pencil_bx` x `rk_pencil_by` blocks of 64 x 4 cells walked along z),
+  its load-phase variant (`rk_variant`, 3D) and the walk length of the y/z
+  AV sweeps (`sweep_chunk`), timed with the
   library's own HIP events (misohip_profile_*) over a few steps;
 - batches (`tune_ensemble`): the y/z sweep walk length per run size, timed
   on a probe batch of that size.
@@ -64,9 +65,10 @@ def get_tiles(ctx):
     return t
 
 
-def set_tiles(ctx, pencil_bx: int, pencil_by: int, chunk: int) -> None:
+def set_tiles(ctx, pencil_bx: int, pencil_by: int, chunk: int, variant: int = 0) -> None:
     t = hip.MisohipTiles()
     t.rk_pencil_bx, t.rk_pencil_by, t.sweep_chunk = int(pencil_bx), int(pencil_by), int(chunk)
+    t.rk_variant = int(variant)
     hip.call("misohip_set_tiles", ctx.handle, C.byref(t))
 
 
@@ -94,7 +96,7 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
     key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}"
     hit = None if force else cache.get(key)
     if hit:
-        set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"])
+        set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"], hit.get("rk_variant", 0))
         return dict(hit, cached=True, seconds=0.0)
     t0 = time.perf_counter()
     saved = model.qq.data.clone()
@@ -108,21 +110,23 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
 
     run()  # warm-up (code objects, caches)
     rk_times = {}
-    for bx, by in pencils:
-        set_tiles(ctx, bx, by, base.sweep_chunk)
-        rk_times[(bx, by)] = _profile(ctx, run).get("rk_stage", float("inf"))
+    variants = (0, 1, 2) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
+    for var in variants:
+        for bx, by in pencils:
+            set_tiles(ctx, bx, by, base.sweep_chunk, var)
+            rk_times[(bx, by, var)] = _profile(ctx, run).get("rk_stage", float("inf"))
     best_p = min(rk_times, key=rk_times.get)
     sweep_times = {}
     active = [k for k, s in (("av_y", g.j_size), ("av_z", g.k_size)) if s > 1]
     for c in chunks:
-        set_tiles(ctx, best_p[0], best_p[1], c)
+        set_tiles(ctx, best_p[0], best_p[1], c, best_p[2])
         t = _profile(ctx, run)
         sweep_times[c] = sum(t.get(k, 0.0) for k in active)
     best_c = min(sweep_times, key=sweep_times.get)
-    set_tiles(ctx, best_p[0], best_p[1], best_c)
+    set_tiles(ctx, best_p[0], best_p[1], best_c, best_p[2])
     torch.cuda.synchronize()
-    out = {"rk_pencil": list(best_p), "sweep_chunk": int(best_c),
-           "times_ms": {"rk_stage": {f"{a}x{b}": v for (a, b), v in rk_times.items()},
+    out = {"rk_pencil": list(best_p[:2]), "rk_variant": int(best_p[2]), "sweep_chunk": int(best_c),
+           "times_ms": {"rk_stage": {f"{a}x{b}v{v}": t for (a, b, v), t in rk_times.items()},
                         "av_yz": {str(k): v for k, v in sweep_times.items()}}}
     cache.put(key, out)
     return dict(out, cached=False, seconds=time.perf_counter() - t0)

@@ -62,8 +62,18 @@ __device__ __forceinline__ void point_terms(const R f[9], R gm1, R o[17]) {
   o[16] = vb * bd;
 }
 
+// Register fence: the 17 running sums are materialised and `idx` becomes
+// opaque, so loads addressed from it cannot be hoisted above this point.
+template <typename R> __device__ __forceinline__ void fence17(int &idx, R S[17]) {
+  asm volatile("" : "+v"(idx), "+v"(S[0]), "+v"(S[1]), "+v"(S[2]), "+v"(S[3]), "+v"(S[4]),
+               "+v"(S[5]), "+v"(S[6]), "+v"(S[7]), "+v"(S[8]), "+v"(S[9]), "+v"(S[10]),
+               "+v"(S[11]), "+v"(S[12]), "+v"(S[13]), "+v"(S[14]), "+v"(S[15]), "+v"(S[16]));
+}
+
 // All 17 derivatives along D at linear index c (stride s, inverse spacing dd).
-template <typename R, int D>
+// SPLIT: fence after each stencil point (9 loads in flight per wave instead
+// of 36, fewer registers, more waves per SIMD); the arithmetic is the same.
+template <typename R, int D, int SPLIT>
 __device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, R dd, R gm1,
                                            R S[17]) {
   R f[9], t[17];
@@ -72,16 +82,19 @@ __device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, 
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = -t[n];
+  if (SPLIT) fence17(c, S);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + s];
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(8), t[n], S[n]);
+  if (SPLIT) fence17(c, S);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - s];
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(-8), t[n], S[n]);
+  if (SPLIT) fence17(c, S);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - 2 * s];
   point_terms<R, D>(f, gm1, t);
@@ -89,11 +102,11 @@ __device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, 
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
 }
 
-template <typename R, int D, bool ACTIVE>
+template <typename R, int D, bool ACTIVE, int SPLIT = 0>
 __device__ __forceinline__ void derivs_or_zero(const Fld<const R> &a, int c, int s, R dd, R gm1,
                                                R S[17]) {
   if (ACTIVE) {
-    dir_derivs<R, D>(a, c, s, dd, gm1, S);
+    dir_derivs<R, D, SPLIT>(a, c, s, dd, gm1, S);
   } else {
 #pragma unroll
     for (int n = 0; n < 17; ++n) S[n] = R(0);
@@ -105,7 +118,7 @@ __device__ __forceinline__ void derivs_or_zero(const Fld<const R> &a, int c, int
 // One 64 x 4 block of cells (x-tile blk_x, y-rows 4*blk_y.., plane blk_z);
 // the stage step dt, ch2 and damp come as arguments so that a batched launch
 // can take them per instance from device memory.
-template <typename R, bool HX, bool HY, bool HZ>
+template <typename R, bool HX, bool HY, bool HZ, int SPLIT = 0>
 __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<const R> &argm,
                                          const Fld<R> &rslt, const R *__restrict__ dxi,
                                          const R *__restrict__ dyi, const R *__restrict__ dzi,
@@ -122,7 +135,7 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   R S[17];
 
   // ---- x: start every running sum
-  derivs_or_zero<R, 0, HX>(argm, c, 1, HX ? dxi[i] : R(0), gm1, S);
+  derivs_or_zero<R, 0, HX, SPLIT>(argm, c, 1, HX ? dxi[i] : R(0), gm1, S);
   R a_ro = -S[0];
   R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
   R a_vy = -S[2], t_vy = S[7];
@@ -141,7 +154,7 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
                "+v"(a_bz), "+v"(k_bz), "+v"(ph_bx), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
 
   // ---- y
-  derivs_or_zero<R, 1, HY>(argm, cy, sh.It, HY ? dyi[j] : R(0), gm1, S);
+  derivs_or_zero<R, 1, HY, SPLIT>(argm, cy, sh.It, HY ? dyi[j] : R(0), gm1, S);
   a_ro = a_ro - S[0];
   a_vx = a_vx - S[1];
   t_vx = t_vx + S[6];
@@ -165,7 +178,7 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
                "+v"(a_bz), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
 
   // ---- z
-  derivs_or_zero<R, 2, HZ>(argm, cz, sh.It * sh.Jt, HZ ? dzi[k] : R(0), gm1, S);
+  derivs_or_zero<R, 2, HZ, SPLIT>(argm, cz, sh.It * sh.Jt, HZ ? dzi[k] : R(0), gm1, S);
   a_ro = a_ro - S[0];
   a_vx = a_vx - S[1];
   t_vx = t_vx + S[6];
@@ -223,7 +236,7 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   rslt.q[8][c] = ph;
 }
 
-template <typename R, bool HX, bool HY, bool HZ>
+template <typename R, bool HX, bool HY, bool HZ, int SPLIT>
 __global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<const R> argm,
                                                        Fld<R> rslt, const R *__restrict__ dxi,
                                                        const R *__restrict__ dyi,
@@ -231,8 +244,21 @@ __global__ __launch_bounds__(256) void rk_stage_kernel(Fld<const R> orgn, Fld<co
                                                        RkScalars sc, Pencils pc) {
   int blk_x, blk_y, blk_z;
   if (!pc.decode(xcd_virtual_block(blockIdx.x, pc.nb), blk_x, blk_y, blk_z)) return;
-  rk_block<R, HX, HY, HZ>(orgn, argm, rslt, dxi, dyi, dzi, sh, sc, R(sc.dt), R(sc.ch2),
+  rk_block<R, HX, HY, HZ, SPLIT>(orgn, argm, rslt, dxi, dyi, dzi, sh, sc, R(sc.dt), R(sc.ch2),
                           R(sc.damp), blk_x, blk_y, blk_z);
+}
+
+// The per-point-fence variant held to 128 registers (4 waves per SIMD; the
+// compiler spills a few dwords to scratch).
+template <typename R, bool HX, bool HY, bool HZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void
+rk_stage_kernel_w4(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt, const R *__restrict__ dxi,
+                   const R *__restrict__ dyi, const R *__restrict__ dzi, Shape sh, RkScalars sc,
+                   Pencils pc) {
+  int blk_x, blk_y, blk_z;
+  if (!pc.decode(xcd_virtual_block(blockIdx.x, pc.nb), blk_x, blk_y, blk_z)) return;
+  rk_block<R, HX, HY, HZ, 1>(orgn, argm, rslt, dxi, dyi, dzi, sh, sc, R(sc.dt), R(sc.ch2),
+                             R(sc.damp), blk_x, blk_y, blk_z);
 }
 
 template <typename R, bool HX, bool HY, bool HZ>
@@ -243,8 +269,15 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
   Pencils pc;
   if (!pc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.pencil_bx, sc.pencil_by))
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ>), dim3(pc.nb), dim3(256), 0, st, o, a, r,
-                     dxi, dyi, dzi, sh, sc, pc);
+  if (HX && HY && HZ && sc.variant == 1)
+    hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 1>), dim3(pc.nb), dim3(256), 0, st, o, a,
+                       r, dxi, dyi, dzi, sh, sc, pc);
+  else if (HX && HY && HZ && sc.variant == 2)
+    hipLaunchKernelGGL((rk_stage_kernel_w4<R, HX, HY, HZ>), dim3(pc.nb), dim3(256), 0, st, o, a,
+                       r, dxi, dyi, dzi, sh, sc, pc);
+  else
+    hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 0>), dim3(pc.nb), dim3(256), 0, st, o, a,
+                       r, dxi, dyi, dzi, sh, sc, pc);
   return hipGetLastError();
 }
 
