@@ -267,8 +267,8 @@ template <typename Real> class Fields {
 
 /// Boundary descriptor: per axis periodic (the halo self-exchange), mirror
 /// with a sign per field (bc::symmetric, boundary_condition.hpp:108-136) or
-/// fixed (the ghost cells keep their values, EmptyBoundaryCondition on that
-/// axis, mhd_model_base.hpp:125).
+/// fixed (every fill copies the axis' ghost cells from given fields: a user
+/// boundary functor writing set values, mhd_model_base.hpp:131-155).
 struct Boundary {
   enum Kind { MIRROR = 0, PERIODIC = 1, FIXED = 2 };
   misohip_bc d{};
@@ -280,6 +280,11 @@ struct Boundary {
   /// sign of field f when mirrored across the faces of axis a (+1 / -1)
   Boundary &sign(int axis, int f, int s) {
     d.sym_sign[axis][f] = s >= 0 ? 1 : -1;
+    return *this;
+  }
+  /// the fields whose ghost cells a FIXED axis takes (kept alive by the caller)
+  template <typename F> Boundary &fixed_values(const F &fields) {
+    for (int f = 0; f < 9; ++f) d.fixed[f] = fields.field(f);
     return *this;
   }
 };

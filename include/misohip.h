@@ -78,12 +78,16 @@ typedef struct misohip_fields {
 /* Boundary description.  periodic[a] = 1: the axis is filled by the
  * one-rank halo self-exchange; 0: the mirror boundary of
  * boundary_condition.hpp:108-136 on both faces with sign sym_sign[axis][f];
- * 2: fixed, the ghost cells of the axis keep their values (a user boundary
- * functor that does nothing on that axis, as EmptyBoundaryCondition,
- * mhd_model_base.hpp:125; edges shared with a periodic axis still wrap). */
+ * 2: fixed, every fill sets the ghost cells of the axis from the `fixed`
+ * fields (nine device fields of the grid's shape and layout; only their
+ * ghost cells are read) -- the user boundary functor that writes given
+ * values into the ghost zone (mhd_model_base.hpp:131-155 contract; the
+ * reference applies the user functor before the halo exchange, so edges
+ * shared with a periodic axis take the periodic image). */
 typedef struct misohip_bc {
   int periodic[3];
   int sym_sign[3][9];
+  const void *fixed[9];
 } misohip_bc;
 
 /* Config values the step consumes (config_defaults.hpp, eos.hpp) and the

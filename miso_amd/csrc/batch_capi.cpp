@@ -135,9 +135,13 @@ int create_t(misohip_ctx *ctx, int n, const misohip_grid *grids, const misohip_f
   b->n = n;
   b->p = *p;
   const int chunk = ctx->tile.sweep_chunk < 4 ? 4 : ctx->tile.sweep_chunk;
-  BcDev bd;
+  BcDev bd{};
   for (int a = 0; a < 3; ++a) {
-    bd.periodic[a] = bc->periodic[a] == 2 ? 2 : (bc->periodic[a] ? 1 : 0);
+    if (bc->periodic[a] == 2) {  // fixed-value boundaries are not batched
+      delete b;
+      return MISOHIP_E_ARG;
+    }
+    bd.periodic[a] = bc->periodic[a] ? 1 : 0;
     for (int f = 0; f < 9; ++f) bd.sign[a][f] = bc->sym_sign[a][f] >= 0 ? 1.0f : -1.0f;
   }
   b->bc = bd;

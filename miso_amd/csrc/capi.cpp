@@ -76,12 +76,23 @@ template <typename R> Fld<const R> cfld(const misohip_fields *f) {
   return r;
 }
 
+// a fixed axis needs its nine value fields
+bool bc_invalid(const misohip_bc *bc) {
+  if (!bc) return true;
+  for (int a = 0; a < 3; ++a)
+    if (bc->periodic[a] == 2)
+      for (int f = 0; f < 9; ++f)
+        if (!bc->fixed[f]) return true;
+  return false;
+}
+
 BcDev make_bc(const misohip_bc *bc) {
   BcDev b;
   for (int a = 0; a < 3; ++a) {
     b.periodic[a] = bc->periodic[a] == 2 ? 2 : (bc->periodic[a] ? 1 : 0);
     for (int f = 0; f < 9; ++f) b.sign[a][f] = bc->sym_sign[a][f] >= 0 ? 1.0f : -1.0f;
   }
+  for (int f = 0; f < 9; ++f) b.fixed[f] = bc->fixed[f];
   return b;
 }
 
@@ -352,7 +363,7 @@ int misohip_copy_to_host(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype 
 int misohip_ghost_fill(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                        const misohip_fields *qq, const misohip_bc *bc) {
   MISOHIP_COMMON_CHECKS(ctx, g, dt)
-  if (check_fields(qq) || !bc) return MISOHIP_E_ARG;
+  if (check_fields(qq) || bc_invalid(bc)) return MISOHIP_E_ARG;
   const BcDev b = make_bc(bc);
   if (dt == MISOHIP_F64) return ghost_t<double>(ctx, qq, make_shape(g), b);
   return ghost_t<float>(ctx, qq, make_shape(g), b);
@@ -405,7 +416,8 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                    const misohip_params *p, const misohip_bc *bc, const misohip_fields *qq,
                    const misohip_work *w, double dt_step) {
   MISOHIP_COMMON_CHECKS(ctx, g, dt)
-  if (!p || !bc || !w || check_fields(qq) || check_fields(&w->argm) || check_fields(&w->rslt) ||
+  if (!p || bc_invalid(bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
+      check_fields(&w->rslt) ||
       !w->cc)
     return MISOHIP_E_ARG;
   if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;

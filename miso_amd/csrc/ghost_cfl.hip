@@ -21,7 +21,8 @@
 
 namespace misohip {
 
-// kind: 0 mirror, 1 periodic, 2 fixed (the ghost cell is its own source)
+// kind: 0 mirror, 1 periodic, 2 fixed (the ghost cell keeps its coordinate
+// and takes its value from the fixed fields)
 __device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool &mirrored) {
   mirrored = false;
   if (m == 0 || kind == 2) return x;
@@ -74,9 +75,13 @@ __device__ __forceinline__ void ghost_cell(const Fld<R> &qq, const Shape &sh, co
   const int sj = map_coord(j, sh.Jt, sh.mj, bc.periodic[1], my);
   const int sk = map_coord(k, sh.Kt, sh.mk, bc.periodic[2], mz);
   const int dst = sh.idx(i, j, k), src = sh.idx(si, sj, sk);
+  // a ghost coordinate of a fixed axis reads the fixed fields
+  const bool from_fixed = (bc.periodic[0] == 2 && (i < sh.mi || i >= sh.It - sh.mi)) ||
+                          (bc.periodic[1] == 2 && (j < sh.mj || j >= sh.Jt - sh.mj)) ||
+                          (bc.periodic[2] == 2 && (k < sh.mk || k >= sh.Kt - sh.mk));
 #pragma unroll
   for (int f = 0; f < 9; ++f) {
-    R v = qq.q[f][src];
+    R v = from_fixed ? static_cast<const R *>(bc.fixed[f])[src] : qq.q[f][src];
     if (mx) v = R(bc.sign[0][f]) * v;
     if (my) v = R(bc.sign[1][f]) * v;
     if (mz) v = R(bc.sign[2][f]) * v;

@@ -21,10 +21,12 @@ struct AvScalars {
   int chunk;  // cells walked per thread by the y/z sweeps
 };
 
-// Boundary description on the device: periodic flag and mirror signs.
+// Boundary description on the device: axis kind (0 mirror, 1 periodic,
+// 2 fixed from the `fixed` fields), mirror signs, fixed-value fields.
 struct BcDev {
   int periodic[3];
   float sign[3][9];
+  const void *fixed[9];
 };
 
 // one ray of the sweep, everything the plane kernel needs

@@ -32,7 +32,8 @@ class MisohipFields(C.Structure):
 
 
 class MisohipBc(C.Structure):
-    _fields_ = [("periodic", C.c_int * 3), ("sym_sign", (C.c_int * 9) * 3)]
+    _fields_ = [("periodic", C.c_int * 3), ("sym_sign", (C.c_int * 9) * 3),
+                ("fixed", C.c_void_p * 9)]
 
 
 class MisohipParams(C.Structure):

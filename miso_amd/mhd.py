@@ -217,15 +217,19 @@ class ExecContext:
 class BoundaryCondition:
     """Per-axis boundary: periodic self-exchange (True / 1), mirror with
     per-field signs (False / 0; boundary_condition.hpp:108-136, signs +1
-    unless given) or fixed ghost values ("fixed" / 2; a user functor that
-    leaves the axis alone, mhd_model_base.hpp:125)."""
+    unless given) or fixed ghost values ("fixed" / 2: every fill copies the
+    axis' ghost cells from the `fixed` Fields -- a user boundary functor
+    writing given values, the mhd_model_base.hpp:131-155 contract)."""
 
     FIXED = 2
 
-    def __init__(self, periodic=(True, True, True), sym_sign=None):
+    def __init__(self, periodic=(True, True, True), sym_sign=None, fixed=None):
         self.periodic = tuple(2 if (p == "fixed" or (not isinstance(p, bool) and p == 2))
                               else int(bool(p)) for p in periodic)
         self.sym_sign = sym_sign or ((1,) * 9,) * 3
+        self.fixed = fixed  # Fields holding the ghost values of fixed axes
+        if 2 in self.periodic and fixed is None:
+            raise ValueError("a fixed boundary needs the Fields that hold its ghost values")
 
     def c_struct(self) -> hip.MisohipBc:
         b = hip.MisohipBc()
@@ -233,6 +237,9 @@ class BoundaryCondition:
             b.periodic[a] = self.periodic[a]
             for f in range(9):
                 b.sym_sign[a][f] = int(self.sym_sign[a][f])
+        if self.fixed is not None:
+            for f in range(9):
+                b.fixed[f] = self.fixed.data[f].data_ptr()
         return b
 
 

@@ -129,7 +129,8 @@ class RadiativeMHD:
         self.s_rad = s.s_rad(zc, self.kappa0)
         self.config = Config({"mhd": {"cfl_number": cfl_number}})
         self.model = MHD(self.config, grid, ctx, s.gm, src_kind=1, grav_axis=2, grav=s.grav)
-        self.bc = BoundaryCondition(periodic=(True, True, "fixed"))
+        self.reservoir = None
+        self.bc = None
         self.rt = RT(grid, 24)
         self.rt_kw = dict(RT_FACES, tolerance=rt_tolerance, max_iters=rt_max_iters)
         self.qrad = self.rt.qrad_field       # the Q_rad the next heating uses
@@ -143,6 +144,12 @@ class RadiativeMHD:
     def initialize(self, seed: int) -> None:
         q = stratified_box(self.grid, self.strat, seed, device=self.grid.device)
         self.model.qq.load_ijk(q)
+        # the z ghost cells keep the initial hydrostatic profile at rest
+        if self.reservoir is None:
+            from .mhd import Fields
+            self.reservoir = Fields(self.grid)
+        self.reservoir.data.copy_(self.model.qq.data)
+        self.bc = BoundaryCondition(periodic=(True, True, "fixed"), fixed=self.reservoir)
         self.model.integrator.apply_boundary_condition(self.bc, self.model.qq)
 
     # -- radiative transfer -------------------------------------------------

@@ -37,9 +37,9 @@ typedef struct og_grid {
  * boundary_condition.hpp).  `periodic[a]` = 1 selects the halo
  * self-exchange on axis a; 0 the mirror boundary of
  * boundary_condition.hpp:108-136 on both sides with per-field sign
- * `sym_sign[a][f]` (+1 / -1); 2 leaves the axis' ghost cells alone (a user
- * boundary functor that does nothing there, like EmptyBoundaryCondition,
- * mhd_model_base.hpp:125).  `src_kind` 1 = constant gravity along
+ * `sym_sign[a][f]` (+1 / -1); 2 sets the axis' ghost cells from `fixed`
+ * (a user boundary functor writing given ghost values; it runs first, then
+ * the mirrors, then the halo exchange).  `src_kind` 1 = constant gravity along
  * `grav_axis` (the force -g*ro of demo/hd2d_rayleigh_taylor/src/main.cpp:109-112).
  */
 typedef struct og_params {
@@ -49,6 +49,8 @@ typedef struct og_params {
   int src_kind;
   int grav_axis;
   double grav;
+  const void *fixed; /* periodic[a] == 2: 9 fields (the q layout) whose ghost
+                        cells the fill copies in */
 } og_params;
 
 /* Metric arrays are packed as x[It] y[Jt] z[Kt] dx[It] dy[Jt] dz[Kt]

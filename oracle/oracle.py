@@ -34,7 +34,7 @@ class OgParams(C.Structure):
         ("fh", C.c_double), ("cs_fac", C.c_double), ("ca_fac", C.c_double),
         ("vv_fac", C.c_double), ("periodic", C.c_int * 3),
         ("sym_sign", (C.c_int * 9) * 3), ("src_kind", C.c_int),
-        ("grav_axis", C.c_int), ("grav", C.c_double)]
+        ("grav_axis", C.c_int), ("grav", C.c_double), ("fixed", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -178,6 +178,7 @@ class Params:
     src_kind: int = 0
     grav_axis: int = 2
     grav: float = 0.0
+    fixed: object = None  # (9, It, Jt, Kt) array, MISO order, for axes with periodic == 2
 
     def c(self) -> OgParams:
         p = OgParams()
@@ -189,6 +190,10 @@ class Params:
                 p.sym_sign[a][f] = int(self.sym_sign[a][f])
         p.src_kind = int(self.src_kind)
         p.grav_axis = int(self.grav_axis)
+        if any(int(x) == 2 for x in self.periodic):
+            assert self.fixed is not None and self.fixed.flags["C_CONTIGUOUS"], \
+                "a fixed axis needs the fixed-value fields"
+            p.fixed = self.fixed.ctypes.data
         return p
 
 

@@ -207,6 +207,14 @@ int cpp_run_steps(int fp64, int ni, int nj, int nk, const double *bounds, const 
     auto ic = [&](hip::HostFields<R> &q, const hip::HostGrid<R> &) {
       for (int f = 0; f < 9; ++f) std::memcpy(q.q[f].data(), src_q + f * N, sizeof(R) * N);
     };
+    // a fixed axis takes its ghost values from the initial state
+    hip::Fields<R> fixed(mhd.grid);
+    if (kinds[0] == 2 || kinds[1] == 2 || kinds[2] == 2) {
+      hip::HostFields<R> h(hg.i_total(), hg.j_total(), hg.k_total());
+      ic(h, hg);
+      fixed.copy_from(ctx, h);
+      bc.fixed_values(fixed);
+    }
     mhd.apply_initial_condition(ic, bc);
     for (int s = 0; s < nsteps; ++s) {
       R dt;

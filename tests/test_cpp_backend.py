@@ -45,7 +45,7 @@ def test_cpp_hd1d_shock_tube():
 CASES = {
     "orszag_tang_32_f64": dict(sizes=(32, 32, 1), steps=10, dtype=np.float64),
     "turbulence_12_f32": dict(sizes=(12, 12, 12), steps=4, dtype=np.float32),
-    "stratified_fixed_z_8x8x16_f64": dict(sizes=(8, 8, 16), steps=3, dtype=np.float64,
+    "stratified_fixed_z_8x8x64_f64": dict(sizes=(8, 8, 64), steps=3, dtype=np.float64,
                                           strat=True),
 }
 
@@ -60,7 +60,7 @@ def test_cpp_layer_matches_oracle(case):
     c = CASES[case]
     sizes, n, dt_ = c["sizes"], c["steps"], c["dtype"]
     if c.get("strat"):
-        bounds = ((0.0, 0.5), (0.0, 0.5), (0.0, 1.0))
+        bounds = ((0.0, 0.125), (0.0, 0.125), (0.0, 1.0))
         kinds, src = (1, 1, 2), dict(src_kind=1, grav_axis=2, grav=Stratification().grav)
         g = Grid(*sizes, 2, bounds=bounds, device="cpu")
         q0 = problems.to_numpy(stratified_box(g, Stratification(), seed=2))
@@ -73,7 +73,8 @@ def test_cpp_layer_matches_oracle(case):
     q0 = np.ascontiguousarray(q0.astype(dt_))
     og = O.make_grid(sizes, 2, bounds=bounds, dtype=dt_)
     ref = q0.copy()
-    p = O.Params(cfl_number=0.4, periodic=kinds, **src)
+    p = O.Params(cfl_number=0.4, periodic=kinds, **src,
+                 **({"fixed": q0.copy()} if 2 in kinds else {}))
     O.ghost_fill(og, p, ref)
     ref_dts = O.run(og, p, ref, n)
     out = np.empty_like(q0)

@@ -215,23 +215,38 @@ def test_ghost_fill_reference_kat():
         assert np.array_equal(q[:, i, j, k], q[:, src[0], src[1], src[2]])
 
 
-def test_ghost_fill_fixed_axis_keeps_its_ghosts():
-    """periodic[a] = 2: the axis' ghost planes keep their values; the x and
-    y halo exchange still wraps the cells they share with those planes."""
+def test_ghost_fill_fixed_axis_takes_the_fixed_values():
+    """periodic[a] = 2: every fill sets the axis' ghost planes from the
+    fixed fields; the x and y halo exchange then wraps the cells they share
+    with those planes (the user functor runs before the exchange); a
+    mirrored axis reflects the fixed values it meets."""
     g = O.make_grid((6, 5, 7), 2)
     It, Jt, Kt = g.shape
-    q = np.random.default_rng(3).standard_normal((9,) + g.shape)
+    rng = np.random.default_rng(3)
+    q = rng.standard_normal((9,) + g.shape)
+    fx = rng.standard_normal((9,) + g.shape)
     q0 = q.copy()
-    O.ghost_fill(g, O.Params(periodic=(1, 1, 2)), q)
+    O.ghost_fill(g, O.Params(periodic=(1, 1, 2), fixed=fx), q)
     ii, jj = slice(2, -2), slice(2, -2)
     for k in (0, 1, Kt - 2, Kt - 1):
-        assert np.array_equal(q[:, ii, jj, k], q0[:, ii, jj, k])
-        # x-ghost / z-ghost edge: periodic image along x of the kept plane
-        assert np.array_equal(q[:, 0, 3, k], q0[:, It - 4, 3, k])
-        assert np.array_equal(q[:, It - 1, 3, k], q0[:, 3, 3, k])
+        assert np.array_equal(q[:, ii, jj, k], fx[:, ii, jj, k])
+        # x-ghost / z-ghost edge: periodic image along x of the fixed plane
+        assert np.array_equal(q[:, 0, 3, k], fx[:, It - 4, 3, k])
+        assert np.array_equal(q[:, It - 1, 3, k], fx[:, 3, 3, k])
     # interior untouched, x faces wrap as with a periodic z
     assert np.array_equal(q[:, ii, jj, 2:-2], q0[:, ii, jj, 2:-2])
     assert np.array_equal(q[:, 1, 3, 4], q0[:, It - 3, 3, 4])
+    # a second fill after the ghosts were disturbed restores them
+    q[:, ii, jj, 0] = 7.0
+    O.ghost_fill(g, O.Params(periodic=(1, 1, 2), fixed=fx), q)
+    assert np.array_equal(q[:, ii, jj, 0], fx[:, ii, jj, 0])
+    # y mirror + z fixed: the (y ghost, z ghost) edge mirrors the fixed plane
+    s = [[1] * 9 for _ in range(3)]
+    s[1][2] = -1
+    q = q0.copy()
+    O.ghost_fill(g, O.Params(periodic=(1, 0, 2), sym_sign=s, fixed=fx), q)
+    assert np.array_equal(q[2, 3, 0, 0], -fx[2, 3, 3, 0])
+    assert np.array_equal(q[0, 3, 1, Kt - 1], fx[0, 3, 2, Kt - 1])
 
 
 def sod_exact(x, t, x0, gm, rl, pl, ul, rr, pr, ur):
