@@ -272,7 +272,7 @@ def rmhd_leg(args):
     t0 = time.perf_counter()
     data_box = make_box(n, 101)
     F, T = collect_columns(data_box, seeds=(101, 102), steps=nsteps + every,
-                           sample_every=4, max_columns=args.rmhd_train_columns)
+                           sample_every=4, max_columns=args.rmhd_train_columns, log=log)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     del data_box
@@ -280,7 +280,7 @@ def rmhd_leg(args):
     log(f"training data: {F.shape[0]} columns in {gen_s:.1f} s")
     sur = ColumnSurrogate(ctx, n, hidden=256, hidden_layers=3, max_batch=n * n, seed=0)
     t0 = time.perf_counter()
-    mse, val = sur.fit(F, T, epochs=args.rmhd_epochs, batch=1024, lr=1e-3)
+    mse, val = sur.fit(F, T, epochs=args.rmhd_epochs, batch=1024, lr=1e-3, log_every=5, log=log)
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     ncols = int(F.shape[0])
@@ -319,8 +319,10 @@ def rmhd_leg(args):
         sur.predict(box.grid, box.model.qq, box.strat.gm, tmp)
         errs["network_only"].append(rel_l2(tmp, truth, box.grid))
 
-    for _ in range(nsteps + every):
+    for s in range(nsteps + every):
         box.step_learned(probe)
+        if s % 8 == 0:
+            log(f"accuracy pass step {s}")
     for k, v in errs.items():
         learned[f"qrad_rel_l2_{k}"] = {"mean": float(np.mean(v)), "max": float(np.max(v)),
                                        "checks": len(v)}

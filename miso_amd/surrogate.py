@@ -116,7 +116,7 @@ class ColumnSurrogate:
 
     # -- training -------------------------------------------------------------
     def fit(self, feats, targets, epochs=40, batch=1024, lr=1e-3, lr_min=1e-5, val_frac=0.1,
-            seed=0, log_every=0):
+            seed=0, log_every=0, log=None):
         """Fit on raw features (N, 3 nz) fp32 and raw Q_rad columns (N, nz)
         (device tensors): sets the normalisation from the training split,
         then runs Adam with a cosine learning-rate schedule.  Returns
@@ -161,8 +161,8 @@ class ColumnSurrogate:
                     tot = loss.value
             last = tot
             self.history.append(tot)
-            if log_every and (ep % log_every == 0 or ep == epochs - 1):
-                print(f"  epoch {ep:3d} train mse {tot:.3e}", flush=True)
+            if log_every and log and (ep % log_every == 0 or ep == epochs - 1):
+                log(f"  epoch {ep:3d} train mse {tot:.3e}")
         val = None
         if nval:
             val = self.validate(Xn[va], targets[va], mu_out, sd_out)

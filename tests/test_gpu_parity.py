@@ -101,8 +101,14 @@ def test_ghost_fill(ctx, geom, kind):
     mask = _face_mask(g)
     assert_bitwise(out[:, mask], ref[:, mask], f"ghost fill {geom}/{kind}")
     if kind == "fixed_z" and g.k_size > 1:
-        # the fixed axis' ghost planes hold the fixed values; edges wrap along x/y
-        assert_bitwise(out, ref, f"ghost fill {geom}/{kind} with edges")
+        # the fixed axis' ghost planes hold the fixed values, and their edges
+        # with the x and y ghosts wrap (only x-y corners are stale in the
+        # reference's pack-all-then-unpack exchange)
+        It, Jt, Kt = g.shape
+        gi = np.zeros(It, bool); gi[:g.i_margin] = gi[It - g.i_margin:] = True
+        gj = np.zeros(Jt, bool); gj[:g.j_margin] = gj[Jt - g.j_margin:] = True
+        keep = ~(gi[:, None, None] & gj[None, :, None]) & np.ones(g.shape, bool)
+        assert_bitwise(out[:, keep], ref[:, keep], f"ghost fill {geom}/{kind} with z edges")
 
 
 def test_ghost_fill_reference_kat(ctx):
