@@ -54,7 +54,11 @@ def parse():
     ap.add_argument("--rmhd-cycles", type=int, default=3,
                     help="timed learned-RT cycles of rt_every steps")
     ap.add_argument("--rmhd-rt-every", type=int, default=16)
-    ap.add_argument("--rmhd-flag-tol", type=float, default=0.03)
+    ap.add_argument("--rmhd-flag-tol", type=float, default=None,
+                    help="columns whose network misses the solve by more than this take the "
+                         "solved values (default: no flags)")
+    ap.add_argument("--rmhd-no-correct", action="store_true",
+                    help="do not add the correction the last plain solve leaves")
     ap.add_argument("--rmhd-train-columns", type=int, default=100000)
     ap.add_argument("--rmhd-epochs", type=int, default=40)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
@@ -289,7 +293,8 @@ def rmhd_leg(args):
     log(f"trained: mse {mse:.3e}, validation rel L2 {val:.3e}, {train_s:.1f} s")
     # timed learned run: one untimed cycle, then args.rmhd_cycles cycles
     box = make_box(n, 7)
-    box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol)
+    box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
+                         correct=not args.rmhd_no_correct)
     for _ in range(every):
         box.step_learned()
     torch.cuda.synchronize()
@@ -305,8 +310,10 @@ def rmhd_leg(args):
     torch.cuda.empty_cache()
     # accuracy: the same run again, Q_rad checked against converged solves
     box = make_box(n, 7)
-    box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol)
-    errs = {"learned": [], "network_only": [], "stale_rt": []}
+    box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
+                         correct=not args.rmhd_no_correct)
+    errs = {"learned": [], "network_only": [], "network_corrected": [], "stale_rt": []}
+    tmp = torch.empty_like(box.q_rt)
 
     def probe(q, solved):
         s = box.steps_done
@@ -315,17 +322,20 @@ def rmhd_leg(args):
         truth = box.reference_qrad()
         errs["learned"].append(rel_l2(q, truth, box.grid))
         errs["stale_rt"].append(rel_l2(box.q_rt, truth, box.grid))
-        tmp = torch.empty_like(q)
-        sur.predict(box.grid, box.model.qq, box.strat.gm, tmp)
+        box.learned_qrad(out=tmp, correct=False, use_flags=False)
         errs["network_only"].append(rel_l2(tmp, truth, box.grid))
+        if box.corr is not None:
+            box.learned_qrad(out=tmp, correct=True, use_flags=False)
+            errs["network_corrected"].append(rel_l2(tmp, truth, box.grid))
 
     for s in range(nsteps + every):
         box.step_learned(probe)
         if s % 8 == 0:
             log(f"accuracy pass step {s}")
     for k, v in errs.items():
-        learned[f"qrad_rel_l2_{k}"] = {"mean": float(np.mean(v)), "max": float(np.max(v)),
-                                       "checks": len(v)}
+        if v:
+            learned[f"qrad_rel_l2_{k}"] = {"mean": float(np.mean(v)), "max": float(np.max(v)),
+                                           "checks": len(v)}
     learned["training"] = {"columns": ncols, "epochs": args.rmhd_epochs, "train_mse": mse,
                            "validation_rel_l2": val, "data_seconds": gen_s,
                            "train_seconds": train_s}

@@ -243,13 +243,17 @@ int misohip_mlp_adam(misohip_ctx *ctx, misohip_mlp *m, float lr, float beta1, fl
  * column; FP64 fields.  Xf (optional) receives the fp32 values. */
 int misohip_mlp_features(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g,
                          const misohip_fields *qq, double gm, void *X, float *Xf);
-/* Q_rad on every interior cell from the network; flagged columns take q_rt */
+/* Q_rad on every interior cell from the network, plus corr (optional FP64
+ * field) where given; flagged columns (flags optional) take q_rt */
 int misohip_mlp_qrad(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, const void *X,
-                     float *Y, const int *flags, const void *q_rt, void *qrad);
-/* residual indicator: flag columns whose network Q_rad differs from q_rt by
- * more than tol in relative L2 over the column */
+                     float *Y, const int *flags, const void *q_rt, const void *corr,
+                     void *qrad);
+/* residual indicator of the last network output Y against a plain solve
+ * q_rt: flag columns whose relative L2 error over the column exceeds tol;
+ * corr (optional) receives q_rt - network on every interior cell, the
+ * correction misohip_mlp_qrad can add until the next solve */
 int misohip_mlp_residual(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, const float *Y,
-                         const void *q_rt, int *flags, double tol);
+                         const void *q_rt, int *flags, double tol, void *corr);
 
 /* Launch shapes of the stencil kernels (no counterpart in the reference,
  * whose CUDA kernels use fixed 8x8x8 blocks, cuda_util.cuh:35-60).

@@ -257,10 +257,11 @@ __global__ void column_features_kernel(Fld<const double> q, Shape sh, double gm,
 }
 
 // Q_rad field from the network output: qrad = y * sd_q[kz] + mu_q[kz]
+// (+ corr, the column correction the last plain solve left)
 __global__ void column_output_kernel(const float *__restrict__ Y, Shape sh,
                                      const float *__restrict__ mu, const float *__restrict__ sd,
                                      const int *__restrict__ flag, const double *__restrict__ q_rt,
-                                     double *__restrict__ qrad) {
+                                     const double *__restrict__ corr, double *__restrict__ qrad) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nz) return;
@@ -273,15 +274,16 @@ __global__ void column_output_kernel(const float *__restrict__ Y, Shape sh,
     qrad[c] = q_rt[c];  // fallback column: last plain-RT value
     return;
   }
-  qrad[c] = (double)(Y[col * nz + kz] * sd[kz] + mu[kz]);
+  const double net = (double)(Y[col * nz + kz] * sd[kz] + mu[kz]);
+  qrad[c] = corr ? net + corr[c] : net;
 }
 
 // residual indicator per column: relative L2 of (mlp - rt) over the column
-// above `tol` -> flag = 1
+// above `tol` -> flag = 1; corr (optional) receives rt - mlp on every cell
 __global__ void column_residual_kernel(const float *__restrict__ Y, Shape sh,
                                        const float *__restrict__ mu, const float *__restrict__ sd,
                                        const double *__restrict__ q_rt, int *__restrict__ flag,
-                                       float tol) {
+                                       float tol, double *__restrict__ corr) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
   const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= (long)ni * nj) return;
@@ -291,6 +293,7 @@ __global__ void column_residual_kernel(const float *__restrict__ Y, Shape sh,
     const int c = sh.idx(sh.mi + i, sh.mj + j, sh.mk + kz);
     const double qr = q_rt[c];
     const double qm = (double)(Y[col * nz + kz] * sd[kz] + mu[kz]);
+    if (corr) corr[c] = qr - qm;
     num += (qm - qr) * (qm - qr);
     den += qr * qr;
   }
@@ -378,19 +381,19 @@ hipError_t launch_column_features(hipStream_t st, const Fld<const double> &q, co
 
 hipError_t launch_column_output(hipStream_t st, const float *Y, const Shape &sh, const float *mu,
                                 const float *sd, const int *flag, const double *q_rt,
-                                double *qrad) {
+                                const double *corr, double *qrad) {
   const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj) * (sh.Kt - 2 * sh.mk);
   hipLaunchKernelGGL(column_output_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Y,
-                     sh, mu, sd, flag, q_rt, qrad);
+                     sh, mu, sd, flag, q_rt, corr, qrad);
   return hipGetLastError();
 }
 
 hipError_t launch_column_residual(hipStream_t st, const float *Y, const Shape &sh,
                                   const float *mu, const float *sd, const double *q_rt, int *flag,
-                                  float tol) {
+                                  float tol, double *corr) {
   const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj);
   hipLaunchKernelGGL(column_residual_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     Y, sh, mu, sd, q_rt, flag, tol);
+                     Y, sh, mu, sd, q_rt, flag, tol, corr);
   return hipGetLastError();
 }
 

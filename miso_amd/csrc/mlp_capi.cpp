@@ -25,9 +25,10 @@ hipError_t launch_to_bf16(hipStream_t, const float *, uint16_t *, long);
 hipError_t launch_column_features(hipStream_t, const Fld<const double> &, const Shape &, double,
                                   const float *, const float *, uint16_t *, float *);
 hipError_t launch_column_output(hipStream_t, const float *, const Shape &, const float *,
-                                const float *, const int *, const double *, double *);
+                                const float *, const int *, const double *, const double *,
+                                double *);
 hipError_t launch_column_residual(hipStream_t, const float *, const Shape &, const float *,
-                                  const float *, const double *, int *, float);
+                                  const float *, const double *, int *, float, double *);
 }  // namespace misohip
 
 struct misohip_mlp {
@@ -308,10 +309,12 @@ int misohip_mlp_features(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g
   return MISOHIP_OK;
 }
 
-// Q_rad of every interior cell from the network (columns along z); columns
-// with flag != 0 take q_rt instead.  Y is scratch of (columns x n_out).
+// Q_rad of every interior cell from the network (columns along z), plus the
+// correction field corr when given; columns with flag != 0 take q_rt
+// instead.  Y is scratch of (columns x n_out).
 int misohip_mlp_qrad(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, const void *X,
-                     float *Y, const int *flags, const void *q_rt, void *qrad) {
+                     float *Y, const int *flags, const void *q_rt, const void *corr,
+                     void *qrad) {
   if (!ctx || !m || !g || !X || !Y || !qrad) return MISOHIP_E_ARG;
   const int ncol = (g->i_total - 2 * g->i_margin) * (g->j_total - 2 * g->j_margin);
   if (g->k_total - 2 * g->k_margin != m->n_out) return MISOHIP_E_ARG;
@@ -325,19 +328,22 @@ int misohip_mlp_qrad(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, co
       return rc;
   }
   MISOHIP_TRY(launch_column_output(ctx->stream, Y, shape_of(g), m->mu_out, m->sd_out, flags,
-                                   static_cast<const double *>(q_rt), static_cast<double *>(qrad)));
+                                   static_cast<const double *>(q_rt),
+                                   static_cast<const double *>(corr), static_cast<double *>(qrad)));
   return MISOHIP_OK;
 }
 
 // residual indicator: flag columns whose network Q_rad (from Y) differs from
-// the plain-RT q_rt by more than tol in relative L2 over the column
+// the plain-RT q_rt by more than tol in relative L2 over the column; corr
+// (optional) receives q_rt - network on every interior cell
 int misohip_mlp_residual(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, const float *Y,
-                         const void *q_rt, int *flags, double tol) {
+                         const void *q_rt, int *flags, double tol, void *corr) {
   if (!ctx || !m || !g || !Y || !q_rt || !flags) return MISOHIP_E_ARG;
   MISOHIP_TRY(hipSetDevice(ctx->device));
   Timed t_(ctx, MISOHIP_KIND_MLP);
   MISOHIP_TRY(launch_column_residual(ctx->stream, Y, shape_of(g), m->mu_out, m->sd_out,
-                                     static_cast<const double *>(q_rt), flags, (float)tol));
+                                     static_cast<const double *>(q_rt), flags, (float)tol,
+                                     static_cast<double *>(corr)));
   return MISOHIP_OK;
 }
 

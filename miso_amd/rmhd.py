@@ -172,22 +172,35 @@ class RadiativeMHD:
         r.solve(self.ctx, **self.rt_kw)
         return r.qrad(self.ctx, **self.rt_kw)
 
-    def attach_surrogate(self, surrogate, rt_every: int = 16, flag_tol: float = 5e-2) -> None:
+    def attach_surrogate(self, surrogate, rt_every: int = 16, flag_tol: float | None = None,
+                         correct: bool = True) -> None:
+        """Learned RT: the network every step, a plain solve every rt_every
+        steps.  correct: add the column correction the last solve left (the
+        solve minus the network on the same state), so the network carries
+        the change since the solve; flag_tol: columns whose network misses
+        the solve by more than this (relative L2) take the solved values
+        until the next solve (None: no flags)."""
         import torch
         self.surrogate = surrogate
         self.rt_every = int(rt_every)
-        self.flag_tol = float(flag_tol)
+        self.flag_tol = flag_tol
+        self.correct = bool(correct)
         ncol = self.grid.i_size * self.grid.j_size
         self.flags = torch.zeros(ncol, dtype=torch.int32, device=self.grid.device)
         self.q_rt = torch.zeros_like(self.rt.qrad_field)
         self.q_mlp = torch.zeros_like(self.rt.qrad_field)
+        self.corr = torch.zeros_like(self.rt.qrad_field) if self.correct else None
 
-    def learned_qrad(self, use_flags: bool = True):
-        """Network Q_rad of the current state into self.q_mlp (flagged
-        columns take the last solved value)."""
-        self.surrogate.predict(self.grid, self.model.qq, self.strat.gm, self.q_mlp,
-                               flags=self.flags if use_flags else None, q_rt=self.q_rt)
-        return self.q_mlp
+    def learned_qrad(self, out=None, correct=None, use_flags=None):
+        """Network Q_rad of the current state (into self.q_mlp by default),
+        with the run's correction and flags unless told otherwise."""
+        correct = self.correct if correct is None else correct
+        use_flags = (self.flag_tol is not None) if use_flags is None else use_flags
+        out = self.q_mlp if out is None else out
+        self.surrogate.predict(self.grid, self.model.qq, self.strat.gm, out,
+                               flags=self.flags if use_flags else None, q_rt=self.q_rt,
+                               corr=self.corr if correct else None)
+        return out
 
     # -- stepping -----------------------------------------------------------
     def _advance(self, qrad) -> float:
@@ -209,10 +222,11 @@ class RadiativeMHD:
         if self.steps_done % self.rt_every == 0:
             q = self.solve_rt()
             self.q_rt.copy_(q)
-            # network on the same state -> residual flags for the next steps
+            # network on the same state -> correction and residual flags
             self.surrogate.predict(self.grid, self.model.qq, self.strat.gm, self.q_mlp,
                                    flags=None, q_rt=None)
-            self.surrogate.residual(self.grid, self.q_rt, self.flags, self.flag_tol)
+            tol = self.flag_tol if self.flag_tol is not None else float("inf")
+            self.surrogate.residual(self.grid, self.q_rt, self.flags, tol, self.corr)
             if probe:
                 probe(self.q_rt, True)
             return self._advance(self.q_rt)

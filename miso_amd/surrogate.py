@@ -93,9 +93,10 @@ class ColumnSurrogate:
                  C.c_void_p(Xf.data_ptr()) if keep_float else None)
         return X[:ncol], Xf[:ncol]
 
-    def predict(self, grid, qq, gm, qrad_out, flags=None, q_rt=None):
+    def predict(self, grid, qq, gm, qrad_out, flags=None, q_rt=None, corr=None):
         """Q_rad of every interior cell into qrad_out (a (Kt, Jt, It) float64
-        field); columns with flags != 0 take q_rt."""
+        field), plus the correction field corr when given; columns with
+        flags != 0 take q_rt."""
         ncol = grid.i_size * grid.j_size
         X, _ = self.features(grid, qq, gm)
         _, _, Y = self._buffers(ncol)
@@ -103,16 +104,19 @@ class ColumnSurrogate:
                  C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()),
                  C.c_void_p(flags.data_ptr()) if flags is not None else None,
                  C.c_void_p(q_rt.data_ptr()) if q_rt is not None else None,
+                 C.c_void_p(corr.data_ptr()) if corr is not None else None,
                  C.c_void_p(qrad_out.data_ptr()))
         return qrad_out
 
-    def residual(self, grid, q_rt, flags, tol):
+    def residual(self, grid, q_rt, flags, tol, corr=None):
         """Flag the columns whose last prediction misses q_rt by more than tol
-        (relative L2 over the column)."""
+        (relative L2 over the column); corr (optional field) receives
+        q_rt - prediction."""
         _, _, Y = self._buffers(grid.i_size * grid.j_size)
         hip.call("misohip_mlp_residual", self.ctx.handle, self.h, C.byref(grid.c_struct()),
                  C.c_void_p(Y.data_ptr()), C.c_void_p(q_rt.data_ptr()),
-                 C.c_void_p(flags.data_ptr()), float(tol))
+                 C.c_void_p(flags.data_ptr()), float(tol),
+                 C.c_void_p(corr.data_ptr()) if corr is not None else None)
 
     # -- training -------------------------------------------------------------
     def fit(self, feats, targets, epochs=40, batch=1024, lr=1e-3, lr_min=1e-5, val_frac=0.1,
