@@ -61,6 +61,9 @@ def parse():
                     help="do not add the correction the last plain solve leaves")
     ap.add_argument("--rmhd-train-columns", type=int, default=100000)
     ap.add_argument("--rmhd-epochs", type=int, default=40)
+    ap.add_argument("--rmhd-features", type=int, default=4, choices=(3, 4),
+                    help="column inputs: 3 = (ln ro, ln T, B^2), 4 = plus the horizontal "
+                         "Laplacian of ln T")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--ens-runs", type=int, default=64)
@@ -276,13 +279,15 @@ def rmhd_leg(args):
     t0 = time.perf_counter()
     data_box = make_box(n, 101)
     F, T = collect_columns(data_box, seeds=(101, 102), steps=nsteps + every,
-                           sample_every=4, max_columns=args.rmhd_train_columns, log=log)
+                           sample_every=4, max_columns=args.rmhd_train_columns, log=log,
+                           features=args.rmhd_features)
     torch.cuda.synchronize()
     gen_s = time.perf_counter() - t0
     del data_box
     torch.cuda.empty_cache()
     log(f"training data: {F.shape[0]} columns in {gen_s:.1f} s")
-    sur = ColumnSurrogate(ctx, n, hidden=256, hidden_layers=3, max_batch=n * n, seed=0)
+    sur = ColumnSurrogate(ctx, n, hidden=256, hidden_layers=3, max_batch=n * n, seed=0,
+                          features=args.rmhd_features)
     t0 = time.perf_counter()
     mse, val = sur.fit(F, T, epochs=args.rmhd_epochs, batch=1024, lr=1e-3, log_every=5, log=log)
     torch.cuda.synchronize()
@@ -337,6 +342,7 @@ def rmhd_leg(args):
             learned[f"qrad_rel_l2_{k}"] = {"mean": float(np.mean(v)), "max": float(np.max(v)),
                                            "checks": len(v)}
     learned["training"] = {"columns": ncols, "epochs": args.rmhd_epochs, "train_mse": mse,
+                           "features": args.rmhd_features,
                            "validation_rel_l2": val, "data_seconds": gen_s,
                            "train_seconds": train_s}
     log(f"learned RT {n}^3: {learned}")

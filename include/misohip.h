@@ -239,8 +239,10 @@ int misohip_mlp_grad(misohip_ctx *ctx, misohip_mlp *m, const void *X, const floa
 int misohip_mlp_adam(misohip_ctx *ctx, misohip_mlp *m, float lr, float beta1, float beta2,
                      float eps, float weight_decay);
 /* column features of a radiative box (columns along z, i fastest):
- * normalised (ln ro, ln T, |B|^2) of the n_in/3 interior cells of each
- * column; FP64 fields.  Xf (optional) receives the fp32 values. */
+ * normalised (ln ro, ln T, |B|^2) of the nz interior cells of each column
+ * (n_in = 3 nz), and with n_in = 4 nz also the horizontal Laplacian of ln T
+ * at a stride of 4 cells (x and y periodic); FP64 fields.  Xf (optional)
+ * receives the fp32 values. */
 int misohip_mlp_features(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g,
                          const misohip_fields *qq, double gm, void *X, float *Xf);
 /* Q_rad on every interior cell from the network, plus corr (optional FP64

@@ -232,9 +232,12 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict
 // Column features of the radiative box: for column (i, j) and cell kz of the
 // interior, x[col][f*nz + kz] = (feat_f - mu[f*nz+kz]) / sd[f*nz+kz] with
 // feat = (ln ro, ln T, |B|^2), T = (gm-1) ei.  Columns run i fastest.
+// nfeat == 4 adds the horizontal Laplacian of ln T at stride h (x and y
+// periodic: neighbours wrap inside the interior).
 __global__ void column_features_kernel(Fld<const double> q, Shape sh, double gm,
                                        const float *__restrict__ mu, const float *__restrict__ sd,
-                                       uint16_t *__restrict__ X, float *__restrict__ Xf) {
+                                       uint16_t *__restrict__ X, float *__restrict__ Xf, int nfeat,
+                                       int h) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nz) return;
@@ -245,14 +248,23 @@ __global__ void column_features_kernel(Fld<const double> q, Shape sh, double gm,
   const long col = (long)j * ni + i;
   const double ro = q.q[0][c], ei = q.q[7][c];
   const double bx = q.q[4][c], by = q.q[5][c], bz = q.q[6][c];
-  const float f[3] = {(float)log(ro), (float)log((gm - 1.0) * ei),
-                      (float)(bx * bx + by * by + bz * bz)};
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
+  const double lnT = log((gm - 1.0) * ei);
+  float f[4] = {(float)log(ro), (float)lnT, (float)(bx * bx + by * by + bz * bz), 0.f};
+  if (nfeat == 4) {
+    const int ip = (i + h) % ni, im = (i - h % ni + ni) % ni;
+    const int jp = (j + h) % nj, jm = (j - h % nj + nj) % nj;
+    const int k = sh.mk + kz;
+    const double s = log((gm - 1.0) * q.q[7][sh.idx(sh.mi + ip, sh.mj + j, k)]) +
+                     log((gm - 1.0) * q.q[7][sh.idx(sh.mi + im, sh.mj + j, k)]) +
+                     log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jp, k)]) +
+                     log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jm, k)]);
+    f[3] = (float)(s - 4.0 * lnT);
+  }
+  for (int a = 0; a < nfeat; ++a) {
     const int fi = a * nz + kz;
     const float x = (f[a] - mu[fi]) / sd[fi];
-    X[col * 3 * nz + fi] = f2bf(x);
-    if (Xf) Xf[col * 3 * nz + fi] = x;
+    X[col * nfeat * nz + fi] = f2bf(x);
+    if (Xf) Xf[col * nfeat * nz + fi] = x;
   }
 }
 
@@ -372,10 +384,10 @@ hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, long n) {
 
 hipError_t launch_column_features(hipStream_t st, const Fld<const double> &q, const Shape &sh,
                                   double gm, const float *mu, const float *sd, uint16_t *X,
-                                  float *Xf) {
+                                  float *Xf, int nfeat, int h) {
   const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj) * (sh.Kt - 2 * sh.mk);
   hipLaunchKernelGGL(column_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     q, sh, gm, mu, sd, X, Xf);
+                     q, sh, gm, mu, sd, X, Xf, nfeat, h);
   return hipGetLastError();
 }
 

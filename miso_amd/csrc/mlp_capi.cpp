@@ -23,7 +23,7 @@ hipError_t launch_adam(hipStream_t, float *, uint16_t *, const float *, float *,
                        float, float, float, float, float, float, float);
 hipError_t launch_to_bf16(hipStream_t, const float *, uint16_t *, long);
 hipError_t launch_column_features(hipStream_t, const Fld<const double> &, const Shape &, double,
-                                  const float *, const float *, uint16_t *, float *);
+                                  const float *, const float *, uint16_t *, float *, int, int);
 hipError_t launch_column_output(hipStream_t, const float *, const Shape &, const float *,
                                 const float *, const int *, const double *, const double *,
                                 double *);
@@ -299,13 +299,16 @@ int misohip_mlp_adam(misohip_ctx *ctx, misohip_mlp *m, float lr, float beta1, fl
 int misohip_mlp_features(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g,
                          const misohip_fields *qq, double gm, void *X, float *Xf) {
   if (!ctx || !m || !g || !qq || !X) return MISOHIP_E_ARG;
-  if (3 * (g->k_total - 2 * g->k_margin) != m->n_in) return MISOHIP_E_ARG;
+  const int nz = g->k_total - 2 * g->k_margin;
+  const int nfeat = nz > 0 && m->n_in % nz == 0 ? m->n_in / nz : 0;
+  if (nfeat != 3 && nfeat != 4) return MISOHIP_E_ARG;
+  if (nfeat == 4 && (g->i_size < 2 || g->j_size < 2)) return MISOHIP_E_ARG;
   MISOHIP_TRY(hipSetDevice(ctx->device));
   Fld<const double> q;
   for (int f = 0; f < 9; ++f) q.q[f] = static_cast<const double *>(qq->q[f]);
   Timed t_(ctx, MISOHIP_KIND_MLP);
   MISOHIP_TRY(launch_column_features(ctx->stream, q, shape_of(g), gm, m->mu_in, m->sd_in,
-                                     static_cast<uint16_t *>(X), Xf));
+                                     static_cast<uint16_t *>(X), Xf, nfeat, 4));
   return MISOHIP_OK;
 }
 

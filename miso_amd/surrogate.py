@@ -3,8 +3,10 @@ the reference).
 
 One column is the vertical line of nz interior cells at (i, j).  Its inputs
 are (ln ro, ln T, |B|^2) over the column (3 nz values, normalised per
-feature and height), its output is Q_rad over the column (nz values,
-normalised per height).  The network (3 nz -> hidden x layers -> nz, SiLU)
+feature and height) and, with features=4, the horizontal Laplacian of ln T
+(what a column sees of its neighbours' temperature, which inclined rays
+carry in); its output is Q_rad over the column (nz values, normalised per
+height).  The network (features x nz -> hidden x layers -> nz, SiLU)
 lives in libmisohip.so: BF16 weights on the MFMA units with FP32
 accumulation, FP32 master weights, Adam, all forward and backward kernels
 hand-written (misohip_mlp_*).  torch only holds device buffers and shuffles
@@ -33,10 +35,12 @@ def interior_columns(field, grid):
 
 class ColumnSurrogate:
     def __init__(self, ctx, nz: int, hidden: int = 256, hidden_layers: int = 3,
-                 max_batch: int = 4096, seed: int = 0, device="cuda"):
+                 max_batch: int = 4096, seed: int = 0, device="cuda", features: int = 3):
         import torch
+        assert features in (3, 4)
         self.ctx, self.nz, self.device = ctx, nz, device
-        self.n_in, self.n_out = 3 * nz, nz
+        self.features = features
+        self.n_in, self.n_out = features * nz, nz
         self.sizes = [self.n_in] + [hidden] * hidden_layers + [self.n_out]
         self.max_batch = int(max_batch)
         h = C.c_void_p()
@@ -190,12 +194,13 @@ class ColumnSurrogate:
         return float((Y - t).norm() / t.norm())
 
 
-def collect_columns(box, seeds, steps, sample_every, max_columns=None, rng_seed=0, log=None):
+def collect_columns(box, seeds, steps, sample_every, max_columns=None, rng_seed=0, log=None,
+                    features=3):
     """Run the radiative box with a plain RT solve on every step for each
     seed and keep (raw features, Q_rad) columns at steps 0, sample_every, ...
     below `steps`.  With max_columns, an even random subsample of that many
-    columns over all snapshots is kept.  Returns device tensors (N, 3 nz)
-    fp32 and (N, nz) float64."""
+    columns over all snapshots is kept.  Returns device tensors
+    (N, features nz) fp32 and (N, nz) float64."""
     import torch
     grid = box.grid
     ncol = grid.i_size * grid.j_size
@@ -204,7 +209,7 @@ def collect_columns(box, seeds, steps, sample_every, max_columns=None, rng_seed=
     per = ncol if max_columns is None else min(ncol, -(-max_columns // nsnap))
     gen = torch.Generator(device="cpu").manual_seed(rng_seed)
     feats, targs = [], []
-    raw = _RawFeatures(box.ctx, grid)
+    raw = _RawFeatures(box.ctx, grid, features)
     for seed in seeds:
         box.initialize(seed)
         box.steps_done = 0
@@ -230,21 +235,22 @@ class _RawFeatures:
     """Un-normalised column features through misohip_mlp_features (a small
     network handle with identity normalisation)."""
 
-    def __init__(self, ctx, grid):
+    def __init__(self, ctx, grid, features=3):
         import torch
         self.ctx, self.grid = ctx, grid
         nz = grid.k_size
+        nin = features * nz
         h = C.c_void_p()
-        hip.call("misohip_mlp_create", ctx.handle, 3 * nz, 32, 1, nz, 32, C.byref(h))
+        hip.call("misohip_mlp_create", ctx.handle, nin, 32, 1, nz, 32, C.byref(h))
         self.h = h
-        z = np.zeros(3 * nz, np.float32)
-        o = np.ones(3 * nz, np.float32)
+        z = np.zeros(nin, np.float32)
+        o = np.ones(nin, np.float32)
         p = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
         self._keep = (z, o)
         hip.call("misohip_mlp_set_norm", ctx.handle, h, p(z), p(o), p(z[:nz]), p(o[:nz]))
         ncol = grid.i_size * grid.j_size
-        self.X = torch.empty((ncol, 3 * nz), dtype=torch.bfloat16, device=grid.device)
-        self.Xf = torch.empty((ncol, 3 * nz), dtype=torch.float32, device=grid.device)
+        self.X = torch.empty((ncol, nin), dtype=torch.bfloat16, device=grid.device)
+        self.Xf = torch.empty((ncol, nin), dtype=torch.float32, device=grid.device)
 
     def __call__(self, qq, gm):
         hip.call("misohip_mlp_features", self.ctx.handle, self.h, C.byref(self.grid.c_struct()),

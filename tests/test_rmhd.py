@@ -131,7 +131,8 @@ def test_column_features_outputs_and_flags(ctx):
 
 
 @pytest.mark.gpu
-def test_learned_rt_beats_stale_rt_between_solves(ctx):
+@pytest.mark.parametrize("features", [3, 4])
+def test_learned_rt_beats_stale_rt_between_solves(ctx, features):
     """Train the column network on GPU-RT columns of boxes with other seeds,
     then run the learned stepping (plain RT every 8 steps) and compare, at
     every step between solves, the network's Q_rad and the last solved
@@ -142,9 +143,11 @@ def test_learned_rt_beats_stale_rt_between_solves(ctx):
     from miso_amd.rmhd import rel_l2
     nxy, nz = 32, 64
     data_box = _box(ctx, nxy, nz, seed=1)
-    F, Tq = collect_columns(data_box, seeds=(1, 2, 3), steps=24, sample_every=3)
-    assert F.shape == (3 * 8 * nxy * nxy, 3 * nz)
-    sur = ColumnSurrogate(ctx, nz, hidden=256, hidden_layers=3, max_batch=nxy * nxy, seed=0)
+    F, Tq = collect_columns(data_box, seeds=(1, 2, 3), steps=24, sample_every=3,
+                            features=features)
+    assert F.shape == (3 * 8 * nxy * nxy, features * nz)
+    sur = ColumnSurrogate(ctx, nz, hidden=256, hidden_layers=3, max_batch=nxy * nxy, seed=0,
+                          features=features)
     train_mse, val = sur.fit(F, Tq, epochs=30, batch=256, lr=2e-3)
     print(f"train mse {train_mse:.3e}  validation rel L2 {val:.3e}", flush=True)
     assert val < 5e-2
