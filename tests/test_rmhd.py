@@ -169,3 +169,21 @@ def test_learned_rt_beats_stale_rt_between_solves(ctx, features):
           f"stale RT {es:.3e}", flush=True)
     assert ec < es
     sur.close()
+
+
+@pytest.mark.gpu
+def test_fourth_feature_is_the_horizontal_laplacian_of_ln_t(ctx):
+    import torch
+    from miso_amd.surrogate import ColumnSurrogate, interior_columns
+    box = _box(ctx, 16, 64)
+    g, nz = box.grid, 64
+    sur = ColumnSurrogate(ctx, nz, hidden=64, hidden_layers=1, max_batch=256, features=4)
+    _, Xf = sur.features(g, box.model.qq, box.strat.gm, keep_float=True)
+    torch.cuda.synchronize()
+    k0, j0, i0 = g.k_margin, g.j_margin, g.i_margin
+    lnT = torch.log((box.strat.gm - 1.0) *
+                    box.model.qq.data[7][k0:k0 + nz, j0:j0 + 16, i0:i0 + 16].double())
+    lap = (lnT.roll(-4, 2) + lnT.roll(4, 2) + lnT.roll(-4, 1) + lnT.roll(4, 1) - 4 * lnT)
+    ref = lap.permute(1, 2, 0).reshape(16 * 16, nz).float()
+    assert torch.allclose(Xf[:, 3 * nz:], ref, rtol=1e-4, atol=1e-6)
+    sur.close()
