@@ -60,7 +60,7 @@ def parse():
     ap.add_argument("--rmhd-no-correct", action="store_true",
                     help="do not add the correction the last plain solve leaves")
     ap.add_argument("--rmhd-train-columns", type=int, default=100000)
-    ap.add_argument("--rmhd-epochs", type=int, default=40)
+    ap.add_argument("--rmhd-epochs", type=int, default=60)
     ap.add_argument("--rmhd-features", type=int, default=4, choices=(3, 4),
                     help="column inputs: 3 = (ln ro, ln T, B^2), 4 = plus the horizontal "
                          "Laplacian of ln T")
@@ -318,6 +318,7 @@ def rmhd_leg(args):
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
                          correct=not args.rmhd_no_correct)
     errs = {"learned": [], "network_only": [], "network_corrected": [], "stale_rt": []}
+    by_age = {}
     tmp = torch.empty_like(box.q_rt)
 
     def probe(q, solved):
@@ -326,6 +327,7 @@ def rmhd_leg(args):
             return
         truth = box.reference_qrad()
         errs["learned"].append(rel_l2(q, truth, box.grid))
+        by_age.setdefault(s % every, []).append(errs["learned"][-1])
         errs["stale_rt"].append(rel_l2(box.q_rt, truth, box.grid))
         box.learned_qrad(out=tmp, correct=False, use_flags=False)
         errs["network_only"].append(rel_l2(tmp, truth, box.grid))
@@ -341,6 +343,8 @@ def rmhd_leg(args):
         if v:
             learned[f"qrad_rel_l2_{k}"] = {"mean": float(np.mean(v)), "max": float(np.max(v)),
                                            "checks": len(v)}
+    learned["qrad_rel_l2_learned_by_steps_since_solve"] = {
+        str(a): float(np.mean(v)) for a, v in sorted(by_age.items())}
     learned["training"] = {"columns": ncols, "epochs": args.rmhd_epochs, "train_mse": mse,
                            "features": args.rmhd_features,
                            "validation_rel_l2": val, "data_seconds": gen_s,

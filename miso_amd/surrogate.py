@@ -39,7 +39,7 @@ class ColumnSurrogate:
         import torch
         assert features in (3, 4)
         self.ctx, self.nz, self.device = ctx, nz, device
-        self.features = features
+        self.n_features = features
         self.n_in, self.n_out = features * nz, nz
         self.sizes = [self.n_in] + [hidden] * hidden_layers + [self.n_out]
         self.max_batch = int(max_batch)
