@@ -175,14 +175,24 @@ __global__ void silu_backward_kernel(const float *__restrict__ dact, const float
   dpre_bf[t] = f2bf(dact[t] * sg * (1.f + x * (1.f - sg)));
 }
 
-// column sums: db[n] = sum_m d[m][n] (deterministic: one thread per n, strided m)
-__global__ void colsum_bf16_kernel(const uint16_t *__restrict__ d, float *__restrict__ db, int M,
-                                   int N) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// column sums db[n] = sum_m d[m][n]: 64 columns per block along the lanes,
+// 16 row groups (rows g, g+16, ...) summed in a fixed order (deterministic)
+__global__ __launch_bounds__(1024) void colsum_bf16_kernel(const uint16_t *__restrict__ d,
+                                                           float *__restrict__ db, int M, int N) {
+  __shared__ float part[16][64];
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int m = 0; m < M; ++m) s += bf2f(d[(long)m * N + n]);
-  db[n] = s;
+  if (n < N)
+    for (int m = g; m < M; m += 16) s += bf2f(d[(long)m * N + n]);
+  part[g][lane] = s;
+  __syncthreads();
+  if (g == 0 && n < N) {
+    float t = part[0][lane];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) t += part[r][lane];
+    db[n] = t;
+  }
 }
 
 // MSE loss on a batch: dY = 2 (Y - T) / (M N) (bf16), per-block partial losses
@@ -232,62 +242,93 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict
 // Column features of the radiative box: for column (i, j) and cell kz of the
 // interior, x[col][f*nz + kz] = (feat_f - mu[f*nz+kz]) / sd[f*nz+kz] with
 // feat = (ln ro, ln T, |B|^2), T = (gm-1) ei.  Columns run i fastest.
-// nfeat == 4 adds the horizontal Laplacian of ln T at stride h (x and y
+// Column features, tiled through LDS: a block reads 64 consecutive x cells
+// (one y row) x 32 heights with lanes along x (coalesced field reads),
+// then writes each column's 32 consecutive heights (coalesced row writes of
+// X).  nfeat == 4 adds the horizontal Laplacian of ln T at stride h (x and y
 // periodic: neighbours wrap inside the interior).
-__global__ void column_features_kernel(Fld<const double> q, Shape sh, double gm,
-                                       const float *__restrict__ mu, const float *__restrict__ sd,
-                                       uint16_t *__restrict__ X, float *__restrict__ Xf, int nfeat,
-                                       int h) {
+constexpr int FT_X = 64, FT_Z = 32;
+
+__global__ __launch_bounds__(256) void column_features_kernel(
+    Fld<const double> q, Shape sh, double gm, const float *__restrict__ mu,
+    const float *__restrict__ sd, uint16_t *__restrict__ X, float *__restrict__ Xf, int nfeat,
+    int h) {
+  __shared__ float tile[4][FT_Z][FT_X + 1];
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)ni * nj * nz) return;
-  const int i = (int)(t % ni);
-  const long r = t / ni;
-  const int j = (int)(r % nj), kz = (int)(r / nj);
-  const int c = sh.idx(sh.mi + i, sh.mj + j, sh.mk + kz);
-  const long col = (long)j * ni + i;
-  const double ro = q.q[0][c], ei = q.q[7][c];
-  const double bx = q.q[4][c], by = q.q[5][c], bz = q.q[6][c];
-  const double lnT = log((gm - 1.0) * ei);
-  float f[4] = {(float)log(ro), (float)lnT, (float)(bx * bx + by * by + bz * bz), 0.f};
-  if (nfeat == 4) {
-    const int ip = (i + h) % ni, im = (i - h % ni + ni) % ni;
-    const int jp = (j + h) % nj, jm = (j - h % nj + nj) % nj;
+  const int i0 = blockIdx.x * FT_X, j = blockIdx.y, kz0 = blockIdx.z * FT_Z;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = i0 + lane;
+  for (int r = wv; r < FT_Z; r += 4) {
+    const int kz = kz0 + r;
+    if (i >= ni || kz >= nz) continue;
     const int k = sh.mk + kz;
-    const double s = log((gm - 1.0) * q.q[7][sh.idx(sh.mi + ip, sh.mj + j, k)]) +
-                     log((gm - 1.0) * q.q[7][sh.idx(sh.mi + im, sh.mj + j, k)]) +
-                     log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jp, k)]) +
-                     log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jm, k)]);
-    f[3] = (float)(s - 4.0 * lnT);
+    const int c = sh.idx(sh.mi + i, sh.mj + j, k);
+    const double ro = q.q[0][c], ei = q.q[7][c];
+    const double bx = q.q[4][c], by = q.q[5][c], bz = q.q[6][c];
+    const double lnT = log((gm - 1.0) * ei);
+    tile[0][r][lane] = (float)log(ro);
+    tile[1][r][lane] = (float)lnT;
+    tile[2][r][lane] = (float)(bx * bx + by * by + bz * bz);
+    if (nfeat == 4) {
+      const int ip = (i + h) % ni, im = (i - h % ni + ni) % ni;
+      const int jp = (j + h) % nj, jm = (j - h % nj + nj) % nj;
+      const double s = log((gm - 1.0) * q.q[7][sh.idx(sh.mi + ip, sh.mj + j, k)]) +
+                       log((gm - 1.0) * q.q[7][sh.idx(sh.mi + im, sh.mj + j, k)]) +
+                       log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jp, k)]) +
+                       log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jm, k)]);
+      tile[3][r][lane] = (float)(s - 4.0 * lnT);
+    }
   }
-  for (int a = 0; a < nfeat; ++a) {
-    const int fi = a * nz + kz;
-    const float x = (f[a] - mu[fi]) / sd[fi];
+  __syncthreads();
+  // write: 32 lanes along the heights of one (column, feature) pair
+  const int zl = threadIdx.x & (FT_Z - 1), pr = threadIdx.x / FT_Z;  // 8 pairs per pass
+  const int kz = kz0 + zl;
+  if (kz >= nz) return;
+  for (int p = pr; p < nfeat * FT_X; p += 256 / FT_Z) {
+    const int f = p / FT_X, il = p % FT_X;
+    if (i0 + il >= ni) continue;
+    const long col = (long)j * ni + i0 + il;
+    const int fi = f * nz + kz;
+    const float x = (tile[f][zl][il] - mu[fi]) / sd[fi];
     X[col * nfeat * nz + fi] = f2bf(x);
     if (Xf) Xf[col * nfeat * nz + fi] = x;
   }
 }
 
 // Q_rad field from the network output: qrad = y * sd_q[kz] + mu_q[kz]
-// (+ corr, the column correction the last plain solve left)
-__global__ void column_output_kernel(const float *__restrict__ Y, Shape sh,
-                                     const float *__restrict__ mu, const float *__restrict__ sd,
-                                     const int *__restrict__ flag, const double *__restrict__ q_rt,
-                                     const double *__restrict__ corr, double *__restrict__ qrad) {
-  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)ni * nj * nz) return;
-  const int i = (int)(t % ni);
-  const long r = t / ni;
-  const int j = (int)(r % nj), kz = (int)(r / nj);
-  const long col = (long)j * ni + i;
-  const int c = sh.idx(sh.mi + i, sh.mj + j, sh.mk + kz);
-  if (flag && flag[col]) {
-    qrad[c] = q_rt[c];  // fallback column: last plain-RT value
-    return;
+// (+ corr, the column correction the last plain solve left).  Tiled like
+// the features: Y rows (heights of a column) are read along the heights,
+// the field is written along x.
+__global__ __launch_bounds__(256) void column_output_kernel(
+    const float *__restrict__ Y, Shape sh, const float *__restrict__ mu,
+    const float *__restrict__ sd, const int *__restrict__ flag, const double *__restrict__ q_rt,
+    const double *__restrict__ corr, double *__restrict__ qrad) {
+  __shared__ float tile[FT_Z][FT_X + 1];
+  const int ni = sh.It - 2 * sh.mi, nz = sh.Kt - 2 * sh.mk;
+  const int i0 = blockIdx.x * FT_X, j = blockIdx.y, kz0 = blockIdx.z * FT_Z;
+  {
+    const int zl = threadIdx.x & (FT_Z - 1), pr = threadIdx.x / FT_Z;
+    const int kz = kz0 + zl;
+    for (int il = pr; il < FT_X; il += 256 / FT_Z)
+      if (i0 + il < ni && kz < nz) tile[zl][il] = Y[((long)j * ni + i0 + il) * nz + kz];
   }
-  const double net = (double)(Y[col * nz + kz] * sd[kz] + mu[kz]);
-  qrad[c] = corr ? net + corr[c] : net;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = i0 + lane;
+  if (i >= ni) return;
+  const long col = (long)j * ni + i;
+  const bool fallback = flag && flag[col];
+  for (int r = wv; r < FT_Z; r += 4) {
+    const int kz = kz0 + r;
+    if (kz >= nz) break;
+    const int c = sh.idx(sh.mi + i, sh.mj + j, sh.mk + kz);
+    if (fallback) {
+      qrad[c] = q_rt[c];  // fallback column: last plain-RT value
+      continue;
+    }
+    const double net = (double)(tile[r][lane] * sd[kz] + mu[kz]);
+    qrad[c] = corr ? net + corr[c] : net;
+  }
 }
 
 // residual indicator per column: relative L2 of (mlp - rt) over the column
@@ -356,7 +397,7 @@ hipError_t launch_silu_backward(hipStream_t st, const float *dact, const float *
 }
 
 hipError_t launch_colsum_bf16(hipStream_t st, const uint16_t *d, float *db, int M, int N) {
-  hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 255) / 256), dim3(256), 0, st, d, db, M, N);
+  hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 63) / 64), dim3(1024), 0, st, d, db, M, N);
   return hipGetLastError();
 }
 
@@ -385,18 +426,21 @@ hipError_t launch_to_bf16(hipStream_t st, const float *x, uint16_t *y, long n) {
 hipError_t launch_column_features(hipStream_t st, const Fld<const double> &q, const Shape &sh,
                                   double gm, const float *mu, const float *sd, uint16_t *X,
                                   float *Xf, int nfeat, int h) {
-  const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj) * (sh.Kt - 2 * sh.mk);
-  hipLaunchKernelGGL(column_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                     q, sh, gm, mu, sd, X, Xf, nfeat, h);
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
+  if (nfeat < 3 || nfeat > 4) return hipErrorInvalidValue;
+  dim3 grid((ni + FT_X - 1) / FT_X, nj, (nz + FT_Z - 1) / FT_Z);
+  hipLaunchKernelGGL(column_features_kernel, grid, dim3(256), 0, st, q, sh, gm, mu, sd, X, Xf,
+                     nfeat, h);
   return hipGetLastError();
 }
 
 hipError_t launch_column_output(hipStream_t st, const float *Y, const Shape &sh, const float *mu,
                                 const float *sd, const int *flag, const double *q_rt,
                                 const double *corr, double *qrad) {
-  const long n = (long)(sh.It - 2 * sh.mi) * (sh.Jt - 2 * sh.mj) * (sh.Kt - 2 * sh.mk);
-  hipLaunchKernelGGL(column_output_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, Y,
-                     sh, mu, sd, flag, q_rt, corr, qrad);
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nz = sh.Kt - 2 * sh.mk;
+  dim3 grid((ni + FT_X - 1) / FT_X, nj, (nz + FT_Z - 1) / FT_Z);
+  hipLaunchKernelGGL(column_output_kernel, grid, dim3(256), 0, st, Y, sh, mu, sd, flag, q_rt, corr,
+                     qrad);
   return hipGetLastError();
 }
 
