@@ -134,6 +134,53 @@ __global__ __launch_bounds__(256) void av_sweep_kernel(Fld<const R> qq, Fld<R> r
                            R(sc.dt), nbx, nby, xcd_virtual_block(blockIdx.x, nb));
 }
 
+// The sweep along a direction of extent one (the reference runs it, with
+// stride 0: every stencil point is the cell itself) evaluated on one cell's
+// values in registers, with exactly the generic kernel's arithmetic; dd is
+// that direction's inverse spacing (0).  in/out: ro vx vy vz bx by bz ei ph.
+template <typename R>
+__device__ __forceinline__ void degenerate_point(const R in[9], R cc, R ep, R fh, R dd, R dt,
+                                                 R out[9]) {
+  const R cdw = R(0.5) * (cc + cc), cup = R(0.5) * (cc + cc);
+  R v[5], fdw, fup;
+  const R ro = in[0];
+#pragma unroll
+  for (int p = 0; p < 5; ++p) v[p] = ro;
+  flux_pair(v, cdw, cup, ep, fh, fdw, fup);
+  const R ro_n = ro - (fup - fdw) * dd * dt;
+  R vn[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+#pragma unroll
+    for (int p = 0; p < 5; ++p) v[p] = ro * in[1 + m];
+    flux_pair(v, cdw, cup, ep, fh, fdw, fup);
+    vn[m] = (v[2] - (fup - fdw) * dd * dt) / ro_n;
+  }
+  R bn[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int f = m < 3 ? 4 + m : 8;
+#pragma unroll
+    for (int p = 0; p < 5; ++p) v[p] = in[f];
+    flux_pair(v, cdw, cup, ep, fh, fdw, fup);
+    bn[m] = v[2] - (fup - fdw) * dd * dt;
+  }
+#pragma unroll
+  for (int p = 0; p < 5; ++p) v[p] = ro * in[7];
+  flux_pair(v, cdw, cup, ep, fh, fdw, fup);
+  const R vx = in[1], vy = in[2], vz = in[3], bx = in[4], by = in[5], bz = in[6], ei = in[7];
+  const double pii8 = double(Const<R>::pii8);
+  const double Et = double(ro) * ei + 0.5 * double(ro) * (double(vx) * vx + double(vy) * vy + double(vz) * vz) +
+                    pii8 * (double(bx) * bx + double(by) * by + double(bz) * bz);
+  const double dro = ro_n, dvx = vn[0], dvy = vn[1], dvz = vn[2];
+  const double dbx = bn[0], dby = bn[1], dbz = bn[2];
+  const double ein = (Et - double((fup - fdw) * dd * dt) - 0.5 * dro * (dvx * dvx + dvy * dvy + dvz * dvz) -
+                      pii8 * (dbx * dbx + dby * dby + dbz * dbz)) /
+                     dro;
+  out[0] = ro_n; out[1] = vn[0]; out[2] = vn[1]; out[3] = vn[2];
+  out[4] = bn[0]; out[5] = bn[1]; out[6] = bn[2]; out[7] = R(ein); out[8] = bn[3];
+}
+
 // ---------------------------------------------------------------------------
 // y and z sweeps that walk along the sweep axis.  Lanes run along x
 // (coalesced); each thread walks `chunk` cells along DIR keeping a window of
@@ -167,9 +214,13 @@ __device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R
 
 // conservative update of one cell from its two interface fluxes
 // (mhd_artificial_viscosity_host.hpp:131-344)
-template <typename R>
+// DEGEN: apply the sweep of a following direction of extent one (2D runs:
+// z) to the result before storing; dz0 is its inverse spacing, cc the cell's
+// frozen speed.
+template <typename R, bool DEGEN = false>
 __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9], const R fup[9],
-                                              R dd, R dt, const Fld<R> &out, int c) {
+                                              R dd, R dt, const Fld<R> &out, int c, R cc = R(0),
+                                              R ep = R(0), R fh = R(0), R dz0 = R(0)) {
   const R ro_n = a.v[0] - (fup[0] - fdw[0]) * dd * dt;
   const R vxn = (a.v[1] - (fup[1] - fdw[1]) * dd * dt) / ro_n;
   const R vyn = (a.v[2] - (fup[2] - fdw[2]) * dd * dt) / ro_n;
@@ -189,6 +240,14 @@ __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9
                       0.5 * dro * (dvx * dvx + dvy * dvy + dvz * dvz) -
                       pii8 * (dbx * dbx + dby * dby + dbz * dbz)) /
                      dro;
+  if (DEGEN) {
+    const R in[9] = {ro_n, vxn, vyn, vzn, bxn, byn, bzn, R(ein), phn};
+    R o[9];
+    degenerate_point(in, cc, ep, fh, dz0, dt, o);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) out.q[f][c] = o[f];
+    return;
+  }
   out.q[0][c] = ro_n;
   out.q[1][c] = vxn;
   out.q[2][c] = vyn;
@@ -200,12 +259,13 @@ __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9
   out.q[8][c] = phn;
 }
 
-template <typename R, int DIR>
+template <typename R, int DIR, bool DEGEN = false>
 __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<R> &rslt,
                                               const R *__restrict__ cc,
                                               const R *__restrict__ dd_arr, const Shape &sh,
                                               int chunk, int nchunks, int nbx,
-                                              const AvScalars &sc, const R dt, int V) {
+                                              const AvScalars &sc, const R dt, int V,
+                                              R dz0 = R(0)) {
   const int ch = V % nchunks, r = V / nchunks;
   const int bx = r % nbx, bo = r / nbx;
   const int wx = walk_waves_x(DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj);
@@ -265,7 +325,7 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
       d0[f] = d1;
     }
     const R dd = dd_arr[m_dir + p0 + t];
-    av_cell_update(a0, fdw, fup, dd, dt, rslt, c);
+    av_cell_update<R, DEGEN>(a0, fdw, fup, dd, dt, rslt, c, cc_0, ep, fh, dz0);
 #pragma unroll
     for (int f = 0; f < 9; ++f) fdw[f] = fup[f];
     a0 = a1;
@@ -276,15 +336,16 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
   }
 }
 
-template <typename R, int DIR>
+template <typename R, int DIR, bool DEGEN>
 __global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rslt,
                                                      const R *__restrict__ cc,
                                                      const R *__restrict__ dd_arr, Shape sh,
                                                      int chunk, int nchunks, int nbx, int nbo,
-                                                     int nb, AvScalars sc) {
+                                                     int nb, AvScalars sc,
+                                                     const R *__restrict__ dz_arr) {
   (void)nbo;
-  av_walk_block<R, DIR>(qq, rslt, cc, dd_arr, sh, chunk, nchunks, nbx, sc, R(sc.dt),
-                        xcd_virtual_block(blockIdx.x, nb));
+  av_walk_block<R, DIR, DEGEN>(qq, rslt, cc, dd_arr, sh, chunk, nchunks, nbx, sc, R(sc.dt),
+                               xcd_virtual_block(blockIdx.x, nb), DEGEN ? dz_arr[0] : R(0));
 }
 
 // ---------------------------------------------------------------------------
@@ -378,10 +439,10 @@ static hipError_t launch_av_t(hipStream_t st, const Fld<const R> &q, const Fld<R
   return hipGetLastError();
 }
 
-template <typename R, int DIR>
+template <typename R, int DIR, bool DEGEN = false>
 static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
                                 const R *cc_in, const R *dd, const Shape &sh,
-                                const AvScalars &sc) {
+                                const AvScalars &sc, const R *dz_arr = nullptr) {
   const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
   const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
   const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
@@ -391,8 +452,8 @@ static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
   const long nbl = (long)nchunks * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((av_walk_kernel<R, DIR>), dim3((unsigned)nbl), dim3(256), 0, st, q, r, cc_in,
-                     dd, sh, chunk, nchunks, nbx, nbo, (int)nbl, sc);
+  hipLaunchKernelGGL((av_walk_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
+                     cc_in, dd, sh, chunk, nchunks, nbx, nbo, (int)nbl, sc, dz_arr);
   return hipGetLastError();
 }
 
@@ -427,6 +488,17 @@ hipError_t launch_av_sweep(hipStream_t st, int dir, const Fld<const R> &q, const
   }
   if (sh.ks) return launch_walk_t<R, 2>(st, q, r, cc_in, dd, sh, sc);
   return launch_av_t<R, 2>(st, q, r, cc_store, cc_in, dd, sh, sc);
+}
+
+// 2D runs (x and y active, z of extent one): the y sweep with the
+// degenerate z sweep applied to each result in registers; dzi is the z
+// metric (its single inverse spacing is read on the device).
+template <typename R>
+hipError_t launch_av_sweep_y_degen_z(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
+                                     const R *cc_in, const R *dyi, const R *dzi, const Shape &sh,
+                                     const AvScalars &sc) {
+  if (!sh.is || !sh.js || sh.ks) return hipErrorInvalidValue;
+  return launch_walk_t<R, 1, true>(st, q, r, cc_in, dyi, sh, sc, dzi);
 }
 
 template <typename R>
@@ -465,12 +537,13 @@ __global__ __launch_bounds__(256) void batch_av_kernel(const BatchInst<R> *__res
     const int j_lo = sh.mj - sh.js, j_n = nj + 2 * sh.js;
     const int k_lo = sh.mk - sh.ks, k_n = nk + 2 * sh.ks;
     av_xshfl_block<R>(q, out, I.cc, I.dxi, sh, j_lo, k_lo, j_n, k_n, (ni + 59) / 60, sc, dt, b);
-  } else if (MODE == 1) {
+  } else if (MODE == 1 || MODE == 3) {
     const int n_dir = DIR == 1 ? nj : nk;
     const int chunk = I.chunk;
     const int wx = walk_waves_x(DIR == 1 ? nk : nj);
-    av_walk_block<R, DIR>(q, out, I.cc, DIR == 1 ? I.dyi : I.dzi, sh, chunk,
-                          (n_dir + chunk - 1) / chunk, ((ni + 63) / 64 + wx - 1) / wx, sc, dt, b);
+    av_walk_block<R, DIR, MODE == 3>(q, out, I.cc, DIR == 1 ? I.dyi : I.dzi, sh, chunk,
+                                     (n_dir + chunk - 1) / chunk, ((ni + 63) / 64 + wx - 1) / wx,
+                                     sc, dt, b, MODE == 3 ? I.dzi[0] : R(0));
   } else {
     const R *dd = DIR == 0 ? I.dxi : (DIR == 1 ? I.dyi : I.dzi);
     av_generic_block<R, DIR>(q, out, I.cc, I.cc, dd, sh, sh.mj, sh.mk, nj, nk, sc, dt,
@@ -487,7 +560,8 @@ static hipError_t launch_batch_av_t(hipStream_t st, const BatchInst<R> *inst,
   return hipGetLastError();
 }
 
-// mode: 0 x-shuffle (dir 0), 1 walk (dir 1, 2), 2 degenerate (dir 1, 2)
+// mode: 0 x-shuffle (dir 0), 1 walk (dir 1, 2), 2 degenerate (dir 1, 2),
+// 3 walk along y with the degenerate z sweep fused (dir 1, 2D runs)
 template <typename R>
 hipError_t launch_batch_av(hipStream_t st, int dir, int mode, const BatchInst<R> *inst,
                            const BatchDyn<R> *dyn, const int *start, int n, int total, int in_set,
@@ -498,6 +572,8 @@ hipError_t launch_batch_av(hipStream_t st, int dir, int mode, const BatchInst<R>
     return launch_batch_av_t<R, 1, 1>(st, inst, dyn, start, n, total, in_set, out_set, sc);
   if (dir == 1 && mode == 2)
     return launch_batch_av_t<R, 1, 2>(st, inst, dyn, start, n, total, in_set, out_set, sc);
+  if (dir == 1 && mode == 3)
+    return launch_batch_av_t<R, 1, 3>(st, inst, dyn, start, n, total, in_set, out_set, sc);
   if (dir == 2 && mode == 1)
     return launch_batch_av_t<R, 2, 1>(st, inst, dyn, start, n, total, in_set, out_set, sc);
   if (dir == 2 && mode == 2)
@@ -518,6 +594,14 @@ template hipError_t launch_av_sweep<double>(hipStream_t, int, const Fld<const do
 template hipError_t launch_av_sweep<float>(hipStream_t, int, const Fld<const float> &,
                                            const Fld<float> &, float *, const float *,
                                            const float *, const Shape &, const AvScalars &);
+template hipError_t launch_av_sweep_y_degen_z<double>(hipStream_t, const Fld<const double> &,
+                                                      const Fld<double> &, const double *,
+                                                      const double *, const double *,
+                                                      const Shape &, const AvScalars &);
+template hipError_t launch_av_sweep_y_degen_z<float>(hipStream_t, const Fld<const float> &,
+                                                     const Fld<float> &, const float *,
+                                                     const float *, const float *, const Shape &,
+                                                     const AvScalars &);
 template hipError_t launch_char_vel<double>(hipStream_t, const Fld<const double> &, double *, int,
                                             const AvScalars &);
 template hipError_t launch_char_vel<float>(hipStream_t, const Fld<const float> &, float *, int,

@@ -237,11 +237,13 @@ template <typename R> hipError_t step_t(hipStream_t st, BatchT<R> *b) {
   av.ep = (R)b->p.ep; av.fh = (R)b->p.fh; av.gm = (R)b->p.gm;
   av.cs_fac = (R)b->p.cs_fac; av.ca_fac = (R)b->p.ca_fac; av.vv_fac = (R)b->p.vv_fac;
   av.chunk = 0;  // per instance (BatchInst::chunk)
-  // x: argm(1) -> rslt(2); y: rslt(2) -> argm(1); z: argm(1) -> qq(0)
-  const int av_in[3] = {1, 2, 1}, av_out[3] = {2, 1, 0};
-  const int mode[3] = {0, b->mode_y, b->mode_z};
+  // x: argm(1) -> rslt(2); y: rslt(2) -> argm(1); z: argm(1) -> qq(0).  2D
+  // runs: the y sweep applies the degenerate z sweep and writes qq(0).
+  const bool fuse_z = b->mode_z == 2;
+  const int av_in[3] = {1, 2, 1}, av_out[3] = {2, fuse_z ? 0 : 1, 0};
+  const int mode[3] = {0, fuse_z ? 3 : b->mode_y, b->mode_z};
   const int ph[3] = {P_AVX, P_AVY, P_AVZ};
-  for (int d = 0; d < 3; ++d) {
+  for (int d = 0; d < (fuse_z ? 2 : 3); ++d) {
     e = launch_batch_av<R>(st, d, mode[d], b->d_inst, b->d_dyn, S(ph[d]), n, b->total[ph[d]],
                            av_in[d], av_out[d], av);
     if (e != hipSuccess) return e;

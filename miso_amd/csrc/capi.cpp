@@ -186,15 +186,29 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
     if (rc) return rc;
     argm = out;
   }
-  // AV: x sweep reads the post-RK state and stores cc; y and z read cc.
+  // AV: x sweep reads the post-RK state and stores cc; y and z read cc.  In
+  // 2D runs the z sweep (extent one, pointwise) is applied by the y sweep to
+  // each of its results before they are stored.
+  const bool fuse_z = g->is && g->js && !g->ks;
   const misohip_fields *cur = argm;
   const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
   for (int dir = 0; dir < 3; ++dir) {
-    const misohip_fields *dst = dir == 2 ? qq : other;
-    int rc = av_sweep_t<R>(ctx, g, p, dir, dt, cur, dst, w->cc);
+    const misohip_fields *dst = (dir == 2 || (dir == 1 && fuse_z)) ? qq : other;
+    int rc;
+    if (dir == 1 && fuse_z) {
+      Timed t_(ctx, MISOHIP_KIND_AV_Y);
+      MISOHIP_TRY(launch_av_sweep_y_degen_z<R>(
+          ctx->stream, cfld<R>(cur), fld<R>(dst), static_cast<const R *>(w->cc),
+          static_cast<const R *>(g->dyi), static_cast<const R *>(g->dzi), sh,
+          av_scalars<R>(p, dt, ctx->tile.sweep_chunk)));
+      rc = MISOHIP_OK;
+    } else {
+      rc = av_sweep_t<R>(ctx, g, p, dir, dt, cur, dst, w->cc);
+    }
     if (rc) return rc;
     rc = ghost_t<R>(ctx, dst, sh, b);
     if (rc) return rc;
+    if (dir == 1 && fuse_z) break;
     other = cur;
     cur = dst;
   }

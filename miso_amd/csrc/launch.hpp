@@ -135,6 +135,10 @@ template <typename R>
 hipError_t launch_av_sweep(hipStream_t, int, const Fld<const R> &, const Fld<R> &, R *, const R *,
                            const R *, const Shape &, const AvScalars &);
 template <typename R>
+hipError_t launch_av_sweep_y_degen_z(hipStream_t, const Fld<const R> &, const Fld<R> &,
+                                     const R *, const R *, const R *, const Shape &,
+                                     const AvScalars &);
+template <typename R>
 hipError_t launch_char_vel(hipStream_t, const Fld<const R> &, R *, int, const AvScalars &);
 template <typename R>
 hipError_t launch_ghost_fill(hipStream_t, const Fld<R> &, const Shape &, const BcDev &);
