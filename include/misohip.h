@@ -266,7 +266,8 @@ typedef struct misohip_tiles {
   int rk_pencil_bx, rk_pencil_by;
   int sweep_chunk;
   int rk_variant; /* 3D RK kernel: 0 = three load phases, 1 = one per stencil
-                     point, 2 = as 1 held to 4 waves per SIMD */
+                     point, 2 = as 1 held to 4 waves per SIMD, 3 = x-y
+                     neighbourhood staged in LDS */
 } misohip_tiles;
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t);
 int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t);

@@ -5,7 +5,8 @@ expose, by timing the candidates on the real state:
 
 - single runs (`tune_single`): the RK stage's pencil shape
   (`rk_pencil_bx` x `rk_pencil_by` blocks of 64 x 4 cells walked along z),
-  its load-phase variant (`rk_variant`, 3D) and the walk length of the y/z
+  its variant (`rk_variant`, 3D: load phases, register cap, LDS-staged x-y
+  neighbourhood) and the walk length of the y/z
   AV sweeps (`sweep_chunk`), timed with the
   library's own HIP events (misohip_profile_*) over a few steps;
 - batches (`tune_ensemble`): the y/z sweep walk length per run size, timed
@@ -110,7 +111,7 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
 
     run()  # warm-up (code objects, caches)
     rk_times = {}
-    variants = (0, 1, 2) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
+    variants = (0, 1, 2, 3) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
     for var in variants:
         for bx, by in pencils:
             set_tiles(ctx, bx, by, base.sweep_chunk, var)

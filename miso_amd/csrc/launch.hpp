@@ -12,7 +12,7 @@ struct RkScalars {
   double dt, ch2, damp, gm, grav;
   int src_kind, grav_axis;
   int pencil_bx, pencil_by;  // pencil shape in blocks of 64 x 4 cells
-  int variant;  // 3D kernel variant: 0 three load phases, 1 one per point, 2 = 1 at 4 waves/SIMD
+  int variant;  // 3D kernel: 0 three load phases, 1 one per point, 2 = 1 at 4 waves/SIMD, 3 LDS tile
 };
 
 // Scalars of one AV sweep (ArtificialViscosity members plus dt).

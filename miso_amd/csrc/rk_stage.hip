@@ -73,37 +73,48 @@ template <typename R> __device__ __forceinline__ void fence17(int &idx, R S[17])
 // All 17 derivatives along D at linear index c (stride s, inverse spacing dd).
 // SPLIT: fence after each stencil point (9 loads in flight per wave instead
 // of 36, fewer registers, more waves per SIMD); the arithmetic is the same.
-template <typename R, int D, int SPLIT>
-__device__ __forceinline__ void dir_derivs(const Fld<const R> &a, int c, int s, R dd, R gm1,
-                                           R S[17]) {
+// Field accessors: global memory (the nine arrays) or an LDS tile holding
+// the nine fields of a block's x-y neighbourhood.
+template <typename R> struct GlobalAcc {
+  const Fld<const R> &a;
+  __device__ __forceinline__ R operator()(int m, int i) const { return a.q[m][i]; }
+};
+constexpr int RK_TX = 68, RK_TY = 8, RK_TILE = RK_TX * RK_TY;  // 64 x 4 cells + 2-cell halo
+template <typename R> struct TileAcc {
+  const R *t;
+  __device__ __forceinline__ R operator()(int m, int i) const { return t[m * RK_TILE + i]; }
+};
+
+template <typename R, int D, int SPLIT, typename Acc>
+__device__ __forceinline__ void dir_derivs(const Acc &a, int c, int s, R dd, R gm1, R S[17]) {
   R f[9], t[17];
 #pragma unroll
-  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + 2 * s];
+  for (int m = 0; m < 9; ++m) f[m] = a(m, c + 2 * s);
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = -t[n];
   if (SPLIT) fence17(c, S);
 #pragma unroll
-  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c + s];
+  for (int m = 0; m < 9; ++m) f[m] = a(m, c + s);
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(8), t[n], S[n]);
   if (SPLIT) fence17(c, S);
 #pragma unroll
-  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - s];
+  for (int m = 0; m < 9; ++m) f[m] = a(m, c - s);
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(-8), t[n], S[n]);
   if (SPLIT) fence17(c, S);
 #pragma unroll
-  for (int m = 0; m < 9; ++m) f[m] = a.q[m][c - 2 * s];
+  for (int m = 0; m < 9; ++m) f[m] = a(m, c - 2 * s);
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
 }
 
-template <typename R, int D, bool ACTIVE, int SPLIT = 0>
-__device__ __forceinline__ void derivs_or_zero(const Fld<const R> &a, int c, int s, R dd, R gm1,
+template <typename R, int D, bool ACTIVE, int SPLIT = 0, typename Acc = GlobalAcc<R>>
+__device__ __forceinline__ void derivs_or_zero(const Acc &a, int c, int s, R dd, R gm1,
                                                R S[17]) {
   if (ACTIVE) {
     dir_derivs<R, D, SPLIT>(a, c, s, dd, gm1, S);
@@ -118,7 +129,9 @@ __device__ __forceinline__ void derivs_or_zero(const Fld<const R> &a, int c, int
 // One 64 x 4 block of cells (x-tile blk_x, y-rows 4*blk_y.., plane blk_z);
 // the stage step dt, ch2 and damp come as arguments so that a batched launch
 // can take them per instance from device memory.
-template <typename R, bool HX, bool HY, bool HZ, int SPLIT = 0>
+// TILE: the x and y derivatives read the block's 68 x 8 neighbourhood of
+// argm from LDS, loaded once by the whole block (3D only: both halos exist).
+template <typename R, bool HX, bool HY, bool HZ, int SPLIT = 0, bool TILE = false>
 __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<const R> &argm,
                                          const Fld<R> &rslt, const R *__restrict__ dxi,
                                          const R *__restrict__ dyi, const R *__restrict__ dzi,
@@ -128,14 +141,33 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   const int i = blk_x * 64 + (threadIdx.x & 63) + sh.mi;
   const int j = blk_y * 4 + (threadIdx.x >> 6) + sh.mj;
   const int k = blk_z + sh.mk;
+  R *tile = nullptr;
+  if (TILE) {
+    __shared__ R lds[9 * RK_TILE];
+    tile = lds;
+    const int i0 = blk_x * 64 + sh.mi - 2, j0 = blk_y * 4 + sh.mj - 2;
+    for (int t = threadIdx.x; t < RK_TILE; t += 256) {
+      const int gi = i0 + t % RK_TX, gj = j0 + t / RK_TX;
+      if (gi < sh.It && gj < sh.Jt) {
+        const int gc = sh.idx(gi, gj, k);
+#pragma unroll
+        for (int m = 0; m < 9; ++m) lds[m * RK_TILE + t] = argm.q[m][gc];
+      }
+    }
+    __syncthreads();
+  }
   if (i >= sh.It - sh.mi || j >= sh.Jt - sh.mj) return;
   const int c = sh.idx(i, j, k);
+  const int tc = ((threadIdx.x >> 6) + 2) * RK_TX + (threadIdx.x & 63) + 2;
   const R gm1 = R(sc.gm) - R(1);
   const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
   R S[17];
 
   // ---- x: start every running sum
-  derivs_or_zero<R, 0, HX, SPLIT>(argm, c, 1, HX ? dxi[i] : R(0), gm1, S);
+  if (TILE)
+    derivs_or_zero<R, 0, HX, SPLIT>(TileAcc<R>{tile}, tc, 1, HX ? dxi[i] : R(0), gm1, S);
+  else
+    derivs_or_zero<R, 0, HX, SPLIT>(GlobalAcc<R>{argm}, c, 1, HX ? dxi[i] : R(0), gm1, S);
   R a_ro = -S[0];
   R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
   R a_vy = -S[2], t_vy = S[7];
@@ -148,13 +180,16 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   // Phase fence: the y loads' index depends (opaquely) on an x result, so
   // the compiler cannot hoist them above the x phase; each phase keeps 36
   // loads in flight instead of 108, which keeps the wave count per SIMD up.
-  int cy = c;
+  int cy = TILE ? tc : c;
   asm volatile("" : "+v"(cy), "+v"(a_ro), "+v"(a_vx), "+v"(d4x), "+v"(d5x), "+v"(t_vx),
                "+v"(a_vy), "+v"(t_vy), "+v"(a_vz), "+v"(t_vz), "+v"(a_by), "+v"(k_by),
                "+v"(a_bz), "+v"(k_bz), "+v"(ph_bx), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
 
   // ---- y
-  derivs_or_zero<R, 1, HY, SPLIT>(argm, cy, sh.It, HY ? dyi[j] : R(0), gm1, S);
+  if (TILE)
+    derivs_or_zero<R, 1, HY, SPLIT>(TileAcc<R>{tile}, cy, RK_TX, HY ? dyi[j] : R(0), gm1, S);
+  else
+    derivs_or_zero<R, 1, HY, SPLIT>(GlobalAcc<R>{argm}, cy, sh.It, HY ? dyi[j] : R(0), gm1, S);
   a_ro = a_ro - S[0];
   a_vx = a_vx - S[1];
   t_vx = t_vx + S[6];
@@ -178,7 +213,8 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
                "+v"(a_bz), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
 
   // ---- z
-  derivs_or_zero<R, 2, HZ, SPLIT>(argm, cz, sh.It * sh.Jt, HZ ? dzi[k] : R(0), gm1, S);
+  derivs_or_zero<R, 2, HZ, SPLIT>(GlobalAcc<R>{argm}, cz, sh.It * sh.Jt, HZ ? dzi[k] : R(0), gm1,
+                                  S);
   a_ro = a_ro - S[0];
   a_vx = a_vx - S[1];
   t_vx = t_vx + S[6];
@@ -261,6 +297,19 @@ rk_stage_kernel_w4(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt, const R *_
                              R(sc.damp), blk_x, blk_y, blk_z);
 }
 
+// The x-y neighbourhood staged in LDS (3D).
+template <typename R>
+__global__ __launch_bounds__(256) void rk_stage_kernel_tile(Fld<const R> orgn, Fld<const R> argm,
+                                                            Fld<R> rslt, const R *__restrict__ dxi,
+                                                            const R *__restrict__ dyi,
+                                                            const R *__restrict__ dzi, Shape sh,
+                                                            RkScalars sc, Pencils pc) {
+  int blk_x, blk_y, blk_z;
+  if (!pc.decode(xcd_virtual_block(blockIdx.x, pc.nb), blk_x, blk_y, blk_z)) return;
+  rk_block<R, true, true, true, 0, true>(orgn, argm, rslt, dxi, dyi, dzi, sh, sc, R(sc.dt),
+                                         R(sc.ch2), R(sc.damp), blk_x, blk_y, blk_z);
+}
+
 template <typename R, bool HX, bool HY, bool HZ>
 static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
                               const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
@@ -275,6 +324,9 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
   else if (HX && HY && HZ && sc.variant == 2)
     hipLaunchKernelGGL((rk_stage_kernel_w4<R, HX, HY, HZ>), dim3(pc.nb), dim3(256), 0, st, o, a,
                        r, dxi, dyi, dzi, sh, sc, pc);
+  else if (HX && HY && HZ && sc.variant == 3)
+    hipLaunchKernelGGL((rk_stage_kernel_tile<R>), dim3(pc.nb), dim3(256), 0, st, o, a, r, dxi,
+                       dyi, dzi, sh, sc, pc);
   else
     hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 0>), dim3(pc.nb), dim3(256), 0, st, o, a,
                        r, dxi, dyi, dzi, sh, sc, pc);

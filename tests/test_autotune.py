@@ -75,7 +75,8 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
     m.integrator.apply_boundary_condition(bc, m.qq)
     start = m.qq.data.clone()
     outs = []
-    for tiles in ((2, 8, 64, 0), (1, 4, 16, 1), (4, 16, 128, 2), (2, 4, 32, 2)):
+    for tiles in ((2, 8, 64, 0), (1, 4, 16, 1), (4, 16, 128, 2), (2, 4, 32, 2), (4, 8, 64, 3),
+                  (1, 4, 64, 3)):
         m.qq.data.copy_(start)
         set_tiles(ctx, *tiles)
         m.integrator.step(bc, m.qq)
