@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--ens-seed", type=int, default=2026)
     ap.add_argument("--ens-graph-steps", type=int, default=10)
     ap.add_argument("--ens-replays", type=int, default=5)
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the Orszag-Tang accuracy leg")
+    ap.add_argument("--accuracy-only", action="store_true")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 timing of the 3D step")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of known size (HBM reference for the "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -145,6 +148,127 @@ def cpu_baseline(target_s: float):
 
 
 BYTES_STEP_2D = {"f64": 1096, "f32": 548}     # algorithmic bytes per 2D cell-update
+
+
+def _div_b(q, dx, dy, dz):
+    """max |div B| dx / max |B| over the interior (4th-order central
+    differences), q in MISO order (9, It, Jt, Kt) with margin 2 (0 on a
+    size-1 axis)."""
+    import numpy as np
+
+    def d(f, axis, h):
+        if f.shape[axis] == 1:
+            return np.zeros_like(f)
+        s = [slice(None)] * 3
+        def sh(o):
+            s2 = list(s)
+            s2[axis] = slice(2 + o, f.shape[axis] - 2 + o)
+            return f[tuple(s2)]
+        return (-sh(2) + 8 * sh(1) - 8 * sh(-1) + sh(-2)) / (12 * h)
+    bx, by, bz = (q[4].astype(np.float64), q[5].astype(np.float64), q[6].astype(np.float64))
+    parts = []
+    for f, ax, h in ((bx, 0, dx), (by, 1, dy), (bz, 2, dz)):
+        g = d(f, ax, h)
+        # trim the other axes to their interiors
+        sl = [slice(2, -2) if (a != ax and f.shape[a] > 1) else slice(None) for a in range(3)]
+        parts.append(g[tuple(sl)])
+    div = parts[0] + parts[1] + parts[2]
+    bmag = np.sqrt(bx ** 2 + by ** 2 + bz ** 2).max()
+    return float(np.abs(div).max() * min(dx, dy, dz) / bmag)
+
+
+def f32_leg(args):
+    """The same 3D turbulence step in FP32 (tuned tiles), cell-updates/s."""
+    import numpy as np
+    import torch
+
+    from miso_amd import mhd, problems
+    n = args.n
+    grid = mhd.Grid(n, n, n, 2, dtype=np.float32, device="cuda:0")
+    ctx = mhd.ExecContext(0)
+    m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), grid, ctx, gm=5.0 / 3.0)
+    bc = mhd.BoundaryCondition((True, True, True))
+    m.qq.load_ijk(problems.turbulence(grid, device="cuda:0"))
+    m.integrator.apply_boundary_condition(bc, m.qq)
+    tiles = None
+    if not args.no_tune:
+        from miso_amd.autotune import summary, tune_single
+        tiles = summary(tune_single(ctx, m, bc, steps=1))
+        tiles.pop("times_ms", None)
+    for _ in range(args.warmup):
+        m.integrator.step(bc, m.qq)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.integrator.step(bc, m.qq)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    v = grid.n_interior * args.steps / el
+    out = {"cell_updates_per_s": v, "ms_per_step": 1e3 * el / args.steps,
+           "hbm_roofline_frac_step": v * BYTES_STEP_3D["f32"] / PEAK_HBM, "tiles": tiles}
+    print(f"[f32] {out}", file=sys.stderr, flush=True)
+    ctx.close()
+    del m
+    torch.cuda.empty_cache()
+    return out
+
+
+def accuracy_leg(args):
+    """Orszag-Tang 256^2 (FP64 and FP32 GPU runs of 200 steps at CFL 0.4)
+    against the FP64 oracle: parity of the FP64 run, per-field error of the
+    FP32 run, div B of both, GPU and CPU times of the same 200 steps."""
+    import numpy as np
+    import torch
+
+    from miso_amd import mhd, problems
+    from oracle import oracle as O
+
+    n, nsteps = 256, 200
+    ctx = mhd.ExecContext(0)
+    cfg = mhd.Config({"mhd": {"cfl_number": 0.4}})
+    bc = mhd.BoundaryCondition((True, True, True))
+    og = O.make_grid((n, n, 1), 2)
+    g64 = mhd.Grid(n, n, 1, 2, device="cuda:0")
+    q0 = problems.to_numpy(problems.orszag_tang(g64))
+    ref = q0.copy()
+    p = O.Params(cfl_number=0.4)
+    O.ghost_fill(og, p, ref)
+    t0 = time.perf_counter()
+    ref_dts = O.run(og, p, ref, nsteps)
+    cpu_s = time.perf_counter() - t0
+    out = {"workload": f"2D Orszag-Tang {n}^2, gamma 5/3, periodic, CFL 0.4, {nsteps} steps",
+           "cpu_oracle_seconds": cpu_s, "cpu_threads": O.get_threads()}
+    sl = (slice(None),) + g64.interior()
+    names = ("ro", "vx", "vy", "vz", "bx", "by", "bz", "ei", "ph")
+    for dt_name, dt in (("f64", np.float64), ("f32", np.float32)):
+        g = g64 if dt == np.float64 else mhd.Grid(n, n, 1, 2, dtype=np.float32, device="cuda:0")
+        m = mhd.MHD(cfg, g, ctx, 5.0 / 3.0)
+        m.apply_initial_condition(q0.astype(dt), bc)
+        div0 = _div_b(m.qq.to_host(ctx), 1.0 / n, 1.0 / n, 1.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dts = [m.integrator.step(bc, m.qq) for _ in range(nsteps)]
+        torch.cuda.synchronize()
+        gpu_s = time.perf_counter() - t0
+        q = m.qq.to_host(ctx).astype(np.float64)
+        err = {}
+        for f, nm in enumerate(names):
+            a, b = q[sl][f], ref[sl][f]
+            scale = max(np.abs(b).max(), 1e-300)
+            err[nm] = {"rel_linf": float(np.abs(a - b).max() / scale),
+                       "rel_l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))}
+        out[dt_name] = {"gpu_seconds": gpu_s, "gpu_cell_updates_per_s": n * n * nsteps / gpu_s,
+                        "dt_equal_to_oracle_every_step": bool(np.array_equal(np.array(dts),
+                                                                             ref_dts)),
+                        "error_vs_fp64_oracle": err,
+                        "max_rel_linf": max(e["rel_linf"] for e in err.values()),
+                        "div_b_rel_start": div0,
+                        "div_b_rel_end": _div_b(q, 1.0 / n, 1.0 / n, 1.0)}
+        del m
+    print(f"[accuracy] fp64 max rel linf {out['f64']['max_rel_linf']:.2e}, "
+          f"fp32 {out['f32']['max_rel_linf']:.2e}", file=sys.stderr, flush=True)
+    ctx.close()
+    return out
 
 
 def ensemble_leg(args):
@@ -374,6 +498,9 @@ def main():
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_leg(args)}), flush=True)
         return
+    if args.accuracy_only:
+        print(json.dumps({"accuracy": accuracy_leg(args)}), flush=True)
+        return
     import numpy as np
     import torch
 
@@ -474,6 +601,10 @@ def main():
     ctx.close()
     del m
     torch.cuda.empty_cache()
+    if not args.no_f32 and args.dtype == "f64":
+        out["f32_step"] = f32_leg(args)
+    if not args.no_accuracy:
+        out["accuracy"] = accuracy_leg(args)
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args)
     if not args.no_rmhd:
