@@ -46,6 +46,8 @@ def parse():
                     help="target length of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--tiles", default=None,
+                    help="bx,by,chunk,variant: set the 3D tile shapes instead of tuning")
     ap.add_argument("--no-rmhd", action="store_true", help="skip the radiative-MHD leg")
     ap.add_argument("--rmhd-only", action="store_true", help="run only the radiative-MHD leg")
     ap.add_argument("--rmhd-n", type=int, default=384, help="learned-RT box, cells per side")
@@ -79,17 +81,25 @@ def parse():
     return ap.parse_args()
 
 
-def rk_traffic(n, dtype):
-    """HBM bytes per rk_stage launch (FETCH_SIZE + WRITE_SIZE, scaled by a
-    plain copy's counters) from the committed rocprofv3 summary of this
-    workload, or None when there is none."""
+def rk_kernel_name(variant):
+    """Name of the RK stage kernel a 3D run launches for a tile variant."""
+    return {1: "rk_stage_kernel<", 2: "rk_stage_kernel_w4<", 3: "rk_stage_kernel_tile<"}.get(
+        variant, "rk_stage_kernel<")
+
+
+def rk_traffic(n, dtype, variant=0):
+    """HBM bytes per launch of the RK stage kernel of this variant
+    (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy's counters) from the
+    committed rocprofv3 summary of this workload, or None when there is
+    none."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"mhd_step_turbulence_{n}cubed_{dtype}.md")
     if not os.path.exists(path):
         return None
+    name = "| `" + rk_kernel_name(variant)
     for line in open(path):
         cols = line.split("|")
-        if line.startswith("| `rk_stage_kernel") and len(cols) >= 10:  # the traffic table
+        if line.startswith(name) and len(cols) >= 10:  # the traffic table
             return float(cols[7]) * 1e6
     return None
 
@@ -520,7 +530,12 @@ def main():
     torch.cuda.synchronize()
     m.integrator.apply_boundary_condition(bc, m.qq)
     tiles = None
-    if not args.no_tune:
+    if args.tiles:
+        from miso_amd.autotune import set_tiles
+        bx, by, ch, var = (int(x) for x in args.tiles.split(","))
+        set_tiles(ctx, bx, by, ch, var)
+        tiles = {"rk_pencil": [bx, by], "sweep_chunk": ch, "rk_variant": var, "given": True}
+    elif not args.no_tune:
         from miso_amd.autotune import summary, tune_single
         tiles = summary(tune_single(ctx, m, bc, steps=1))
         print(f"[tune] {tiles}", file=sys.stderr, flush=True)
@@ -586,12 +601,13 @@ def main():
         "hbm_roofline_frac_step": step_frac,
         "roofline": {
             "bound": "hbm",
-            "kernel": "rk_stage_kernel (fused 9-field RK stage)",
+            "kernel": rk_kernel_name((tiles or {}).get("rk_variant", 0)).rstrip("<") +
+                      " (fused 9-field RK stage)",
             "achieved": rk_achieved / 1e9 if rk else None,
             "peak": PEAK_HBM / 1e9,
             "unit": "GB/s",
             "frac": (rk_achieved / PEAK_HBM) if rk else None,
-            "traffic": rk_traffic(n, args.dtype),
+            "traffic": rk_traffic(n, args.dtype, (tiles or {}).get("rk_variant", 0)),
             "traffic_source": f"profiles/mhd_step_turbulence_{n}cubed_{args.dtype}.md",
         },
         "kernels": kernels,
