@@ -26,7 +26,7 @@ import time
 
 from . import hip
 
-RK_PENCILS = ((1, 4), (1, 8), (2, 4), (2, 8), (2, 16), (4, 8), (4, 16))
+RK_PENCILS = ((1, 4), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (4, 16), (8, 4), (8, 8))
 CHUNKS = (16, 32, 64, 128, 256)
 
 
@@ -113,7 +113,8 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
     rk_times = {}
     variants = (0, 1, 2, 3) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
     for var in variants:
-        for bx, by in pencils:
+        # the register-capped variants are tried on the default pencil only
+        for bx, by in (pencils if var in (0, 3) else ((2, 8),)):
             set_tiles(ctx, bx, by, base.sweep_chunk, var)
             rk_times[(bx, by, var)] = _profile(ctx, run).get("rk_stage", float("inf"))
     best_p = min(rk_times, key=rk_times.get)
