@@ -163,6 +163,18 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                    const misohip_fields *qq, const misohip_work *w,
                    double dt_step);
 
+/* nsteps of ModelBase::update on the same fields (the loop of
+ * ModelBase::run, mhd_model_base.hpp:89-117): from the second step on, the
+ * CFL reduction is carried out by the previous step's last AV sweep on the
+ * values it stores (3D runs and 2D x-y runs; others reduce separately), so
+ * dts[] (optional, nsteps values) equal those of nsteps misohip_step calls.
+ * The caller must not modify qq between the steps -- use misohip_step for
+ * that (e.g. to add a source between steps). */
+int misohip_run(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                const misohip_params *p, const misohip_bc *bc,
+                const misohip_fields *qq, const misohip_work *w, int nsteps,
+                double *dts);
+
 /* ModelBase::update: dt = cfl(qq), then update(dt).  *dt_out receives dt. */
 int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                  const misohip_params *p, const misohip_bc *bc,

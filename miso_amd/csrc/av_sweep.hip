@@ -220,7 +220,8 @@ __device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R
 template <typename R, bool DEGEN = false>
 __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9], const R fup[9],
                                               R dd, R dt, const Fld<R> &out, int c, R cc = R(0),
-                                              R ep = R(0), R fh = R(0), R dz0 = R(0)) {
+                                              R ep = R(0), R fh = R(0), R dz0 = R(0),
+                                              R *stored = nullptr) {
   const R ro_n = a.v[0] - (fup[0] - fdw[0]) * dd * dt;
   const R vxn = (a.v[1] - (fup[1] - fdw[1]) * dd * dt) / ro_n;
   const R vyn = (a.v[2] - (fup[2] - fdw[2]) * dd * dt) / ro_n;
@@ -246,7 +247,15 @@ __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9
     degenerate_point(in, cc, ep, fh, dz0, dt, o);
 #pragma unroll
     for (int f = 0; f < 9; ++f) out.q[f][c] = o[f];
+    if (stored) {
+#pragma unroll
+      for (int f = 0; f < 9; ++f) stored[f] = o[f];
+    }
     return;
+  }
+  if (stored) {
+    stored[0] = ro_n; stored[1] = vxn; stored[2] = vyn; stored[3] = vzn; stored[4] = bxn;
+    stored[5] = byn; stored[6] = bzn; stored[7] = R(ein); stored[8] = phn;
   }
   out.q[0][c] = ro_n;
   out.q[1][c] = vxn;
@@ -259,35 +268,38 @@ __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9
   out.q[8][c] = phn;
 }
 
-template <typename R, int DIR, bool DEGEN = false>
+// CFLF: also reduce the CFL candidates of the stored results into cf.slot
+// (one atomicMin per wave), the next step's dt.
+template <typename R, int DIR, bool DEGEN = false, bool CFLF = false>
 __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<R> &rslt,
                                               const R *__restrict__ cc,
                                               const R *__restrict__ dd_arr, const Shape &sh,
                                               int chunk, int nchunks, int nbx,
                                               const AvScalars &sc, const R dt, int V,
-                                              R dz0 = R(0)) {
+                                              R dz0 = R(0), const CflFuse<R> *cf = nullptr) {
   const int ch = V % nchunks, r = V / nchunks;
   const int bx = r % nbx, bo = r / nbx;
   const int wx = walk_waves_x(DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj);
   const int wave = threadIdx.x >> 6;
   const int i = (bx * wx + wave % wx) * 64 + (threadIdx.x & 63) + sh.mi;
   const int o = bo * (4 / wx) + wave / wx;  // the other transverse axis
-  if (i >= sh.It - sh.mi) return;
+  unsigned best = __float_as_uint(1.e10f);
   int j, k, m_dir, n_dir;
+  bool valid = i < sh.It - sh.mi;
   if (DIR == 1) {
     k = o + sh.mk;
-    if (k >= sh.Kt - sh.mk) return;
+    valid = valid && k < sh.Kt - sh.mk;
     m_dir = sh.mj; n_dir = sh.Jt - 2 * sh.mj;
     j = m_dir + ch * chunk;
   } else {
     j = o + sh.mj;
-    if (j >= sh.Jt - sh.mj) return;
+    valid = valid && j < sh.Jt - sh.mj;
     m_dir = sh.mk; n_dir = sh.Kt - 2 * sh.mk;
     k = m_dir + ch * chunk;
   }
   const int p0 = ch * chunk;  // offset along DIR from the first interior cell
-  const int len = min(chunk, n_dir - p0);
-  if (len <= 0) return;
+  const int len = valid ? min(chunk, n_dir - p0) : 0;
+  if (len > 0) {
   const int s = sh.stride(DIR);
   const R ep = R(sc.ep), fh = R(sc.fh);
   int c = sh.idx(i, j, k);
@@ -325,7 +337,17 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
       d0[f] = d1;
     }
     const R dd = dd_arr[m_dir + p0 + t];
-    av_cell_update<R, DEGEN>(a0, fdw, fup, dd, dt, rslt, c, cc_0, ep, fh, dz0);
+    if (CFLF) {
+      R st[9];
+      av_cell_update<R, DEGEN>(a0, fdw, fup, dd, dt, rslt, c, cc_0, ep, fh, dz0, st);
+      const int jc = DIR == 1 ? m_dir + p0 + t : j, kc = DIR == 2 ? m_dir + p0 + t : k;
+      const R hmin = nmin3(cf->dx[i], cf->dy[jc], cf->dz[kc]);
+      const float cand = cfl_candidate_v(st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7],
+                                         R(cf->gm), R(cf->cfl), hmin);
+      best = min(best, __float_as_uint(cand));
+    } else {
+      av_cell_update<R, DEGEN>(a0, fdw, fup, dd, dt, rslt, c, cc_0, ep, fh, dz0);
+    }
 #pragma unroll
     for (int f = 0; f < 9; ++f) fdw[f] = fup[f];
     a0 = a1;
@@ -334,6 +356,24 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
     cc_1 = cc_2;
     c += s;
   }
+  }
+  if (CFLF) {
+    best = wave_umin(best);
+    if ((threadIdx.x & 63) == 0) atomicMin(cf->slot, best);
+  }
+}
+
+template <typename R, int DIR, bool DEGEN>
+__global__ __launch_bounds__(256) void av_walk_cfl_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                         const R *__restrict__ cc,
+                                                         const R *__restrict__ dd_arr, Shape sh,
+                                                         int chunk, int nchunks, int nbx, int nb,
+                                                         AvScalars sc,
+                                                         const R *__restrict__ dz_arr,
+                                                         CflFuse<R> cf) {
+  av_walk_block<R, DIR, DEGEN, true>(qq, rslt, cc, dd_arr, sh, chunk, nchunks, nbx, sc,
+                                     R(sc.dt), xcd_virtual_block(blockIdx.x, nb),
+                                     DEGEN ? dz_arr[0] : R(0), &cf);
 }
 
 template <typename R, int DIR, bool DEGEN>
@@ -501,6 +541,35 @@ hipError_t launch_av_sweep_y_degen_z(hipStream_t st, const Fld<const R> &q, cons
   return launch_walk_t<R, 1, true>(st, q, r, cc_in, dyi, sh, sc, dzi);
 }
 
+template <typename R, int DIR, bool DEGEN>
+static hipError_t launch_walk_cfl_t(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
+                                    const R *cc_in, const R *dd, const Shape &sh,
+                                    const AvScalars &sc, const R *dz_arr, const CflFuse<R> &cf) {
+  const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
+  const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
+  const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
+  const int nchunks = (n_dir + chunk - 1) / chunk;
+  const int wx = walk_waves_x(n_oth);
+  const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
+  const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
+  const long nbl = (long)nchunks * nbx * nbo;
+  if (nbl > 2147483647L) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((av_walk_cfl_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st,
+                     q, r, cc_in, dd, sh, chunk, nchunks, nbx, (int)nbl, sc, dz_arr, cf);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_av_last_sweep_cfl(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
+                                    const R *cc_in, const R *dyi, const R *dzi, const Shape &sh,
+                                    const AvScalars &sc, const CflFuse<R> &cf) {
+  if (sh.is && sh.js && sh.ks) return launch_walk_cfl_t<R, 2, false>(st, q, r, cc_in, dzi, sh, sc,
+                                                                      nullptr, cf);
+  if (sh.is && sh.js && !sh.ks) return launch_walk_cfl_t<R, 1, true>(st, q, r, cc_in, dyi, sh, sc,
+                                                                      dzi, cf);
+  return hipErrorInvalidValue;
+}
+
 template <typename R>
 hipError_t launch_char_vel(hipStream_t st, const Fld<const R> &q, R *cc, int n, const AvScalars &sc) {
   hipLaunchKernelGGL((char_vel_kernel<R>), dim3((n + 255) / 256), dim3(256), 0, st, q, cc, n, sc);
@@ -602,6 +671,15 @@ template hipError_t launch_av_sweep_y_degen_z<float>(hipStream_t, const Fld<cons
                                                      const Fld<float> &, const float *,
                                                      const float *, const float *, const Shape &,
                                                      const AvScalars &);
+template hipError_t launch_av_last_sweep_cfl<double>(hipStream_t, const Fld<const double> &,
+                                                     const Fld<double> &, const double *,
+                                                     const double *, const double *,
+                                                     const Shape &, const AvScalars &,
+                                                     const CflFuse<double> &);
+template hipError_t launch_av_last_sweep_cfl<float>(hipStream_t, const Fld<const float> &,
+                                                    const Fld<float> &, const float *,
+                                                    const float *, const float *, const Shape &,
+                                                    const AvScalars &, const CflFuse<float> &);
 template hipError_t launch_char_vel<double>(hipStream_t, const Fld<const double> &, double *, int,
                                             const AvScalars &);
 template hipError_t launch_char_vel<float>(hipStream_t, const Fld<const float> &, float *, int,

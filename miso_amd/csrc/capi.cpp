@@ -160,9 +160,12 @@ int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const Bc
 
 // Integrator::update (mhd_integrator_host.hpp:258-300,337-350) with the
 // workspace ping-pong: every stage writes rslt, which becomes the next argm.
+// fuse_cfl: the last AV sweep also reduces the next step's CFL candidates
+// into ctx->d_slot (3D and 2D x-y runs).
 template <typename R>
 int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
-             const misohip_bc *bc, const misohip_fields *qq, const misohip_work *w, R dt) {
+             const misohip_bc *bc, const misohip_fields *qq, const misohip_work *w, R dt,
+             bool fuse_cfl = false) {
   const R cfl_number = (R)p->cfl_number;
   const R min_dxyz = (R)g->min_dxyz;
   // divb_parameters_set (337-341)
@@ -195,7 +198,23 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   for (int dir = 0; dir < 3; ++dir) {
     const misohip_fields *dst = (dir == 2 || (dir == 1 && fuse_z)) ? qq : other;
     int rc;
-    if (dir == 1 && fuse_z) {
+    const bool last = dir == 2 || (dir == 1 && fuse_z);
+    if (last && fuse_cfl) {
+      Timed t_(ctx, fuse_z ? MISOHIP_KIND_AV_Y : MISOHIP_KIND_AV_Z);
+      CflFuse<R> cf;
+      cf.dx = static_cast<const R *>(g->dx);
+      cf.dy = static_cast<const R *>(g->dy);
+      cf.dz = static_cast<const R *>(g->dz);
+      cf.gm = (double)(R)p->gm;
+      cf.cfl = (double)(R)p->cfl_number;
+      cf.slot = ctx->d_slot;
+      MISOHIP_TRY(launch_cfl_init(ctx->stream, ctx->d_slot));
+      MISOHIP_TRY(launch_av_last_sweep_cfl<R>(
+          ctx->stream, cfld<R>(cur), fld<R>(dst), static_cast<const R *>(w->cc),
+          static_cast<const R *>(g->dyi), static_cast<const R *>(g->dzi), sh,
+          av_scalars<R>(p, dt, ctx->tile.sweep_chunk), cf));
+      rc = MISOHIP_OK;
+    } else if (dir == 1 && fuse_z) {
       Timed t_(ctx, MISOHIP_KIND_AV_Y);
       MISOHIP_TRY(launch_av_sweep_y_degen_z<R>(
           ctx->stream, cfld<R>(cur), fld<R>(dst), static_cast<const R *>(w->cc),
@@ -211,6 +230,32 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
     if (dir == 1 && fuse_z) break;
     other = cur;
     cur = dst;
+  }
+  return MISOHIP_OK;
+}
+
+// ModelBase::run's loop body nsteps times; from the second step on, dt comes
+// from the CFL reduction the previous step's last AV sweep carried out.
+template <typename R>
+int run_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p, const misohip_bc *bc,
+          const misohip_fields *qq, const misohip_work *w, int nsteps, double *dts) {
+  const bool can_fuse = g->is && g->js;  // 3D, or 2D in x-y
+  double d = 0.0;
+  for (int s = 0; s < nsteps; ++s) {
+    if (s == 0 || !can_fuse) {
+      int rc = cfl_t<R>(ctx, g, p, qq, &d);
+      if (rc) return rc;
+    } else {
+      MISOHIP_TRY(hipMemcpyAsync(ctx->h_slot, ctx->d_slot, sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+      MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+      float f;
+      std::memcpy(&f, ctx->h_slot, sizeof(float));
+      d = (double)(R)f;
+    }
+    if (dts) dts[s] = d;
+    int rc = update_t<R>(ctx, g, p, bc, qq, w, (R)d, can_fuse && s + 1 < nsteps);
+    if (rc) return rc;
   }
   return MISOHIP_OK;
 }
@@ -438,6 +483,19 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
   if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
   if (dt == MISOHIP_F64) return update_t<double>(ctx, g, p, bc, qq, w, dt_step);
   return update_t<float>(ctx, g, p, bc, qq, w, (float)dt_step);
+}
+
+
+int misohip_run(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                const misohip_params *p, const misohip_bc *bc, const misohip_fields *qq,
+                const misohip_work *w, int nsteps, double *dts) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || bc_invalid(bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
+      check_fields(&w->rslt) || !w->cc || nsteps < 0)
+    return MISOHIP_E_ARG;
+  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (dt == MISOHIP_F64) return run_t<double>(ctx, g, p, bc, qq, w, nsteps, dts);
+  return run_t<float>(ctx, g, p, bc, qq, w, nsteps, dts);
 }
 
 int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,

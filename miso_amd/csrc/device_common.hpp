@@ -120,4 +120,23 @@ __device__ __forceinline__ R flux_core(R qq_dw, R qq_up, R dqq_dw, R dqq_up, R c
   return R(-0.5) * cc * pp * (qqr - qql);
 }
 
+// CFL candidate of one cell (Integrator::cfl, mhd_integrator_host.hpp:303-334):
+// cfl * min(dx, dy, dz) / (cs + |v| + ca), narrowed to float.
+template <typename R>
+__device__ __forceinline__ float cfl_candidate_v(R ro, R vx, R vy, R vz, R bx, R by, R bz, R ei,
+                                                 R gm, R cfl, R hmin) {
+  const R cs = dsqrt(gm * (gm - R(1)) * ei);
+  const R vv = dsqrt(+vx * vx + vy * vy + vz * vz);
+  const R ca = dsqrt((+bx * bx + by * by + bz * bz) / ro * Const<R>::pii4);
+  const R total_vel = cs + vv + ca;
+  return static_cast<float>(cfl * hmin / total_vel);
+}
+
+// minimum over the 64 lanes of a wave (all lanes must be active)
+__device__ __forceinline__ unsigned wave_umin(unsigned v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
 }  // namespace misohip

@@ -125,19 +125,8 @@ __global__ void cfl_init_kernel(unsigned *slot) {
 
 template <typename R>
 __device__ __forceinline__ float cfl_candidate(const Fld<const R> &q, int c, R gm, R cfl, R hmin) {
-  const R ro = q.q[0][c], vx = q.q[1][c], vy = q.q[2][c], vz = q.q[3][c];
-  const R bx = q.q[4][c], by = q.q[5][c], bz = q.q[6][c], ei = q.q[7][c];
-  const R cs = dsqrt(gm * (gm - R(1)) * ei);
-  const R vv = dsqrt(+vx * vx + vy * vy + vz * vz);
-  const R ca = dsqrt((+bx * bx + by * by + bz * bz) / ro * Const<R>::pii4);
-  const R total_vel = cs + vv + ca;
-  return static_cast<float>(cfl * hmin / total_vel);
-}
-
-__device__ __forceinline__ unsigned wave_umin(unsigned v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o, 64));
-  return v;
+  return cfl_candidate_v(q.q[0][c], q.q[1][c], q.q[2][c], q.q[3][c], q.q[4][c], q.q[5][c],
+                         q.q[6][c], q.q[7][c], gm, cfl, hmin);
 }
 
 // Grid-stride over the interior; one atomic per block.
@@ -169,6 +158,11 @@ __global__ __launch_bounds__(256) void cfl_kernel(Fld<const R> q, const R *__res
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = min(b, red[w]);
     atomicMin(slot, b);
   }
+}
+
+hipError_t launch_cfl_init(hipStream_t st, unsigned *slot) {
+  hipLaunchKernelGGL(cfl_init_kernel, dim3(1), dim3(64), 0, st, slot);
+  return hipGetLastError();
 }
 
 template <typename R>

@@ -61,6 +61,15 @@ hipError_t launch_rt_source(hipStream_t, const double *, const double *, double 
 hipError_t launch_rt_heat(hipStream_t, const double *, double *, const double *, const Shape &,
                           double);
 
+// The next step's CFL reduction carried out by the last AV sweep of a step:
+// metric, the run's constants and the reduction slot.
+template <typename R> struct CflFuse {
+  const R *dx, *dy, *dz;
+  double gm, cfl;
+  unsigned *slot;
+};
+hipError_t launch_cfl_init(hipStream_t, unsigned *);
+
 // Waves of a walking-sweep block laid along x: all four when the other
 // transverse axis has fewer than four cells (2D runs), else one (the four
 // waves then take four rows of that axis).
@@ -138,6 +147,12 @@ template <typename R>
 hipError_t launch_av_sweep_y_degen_z(hipStream_t, const Fld<const R> &, const Fld<R> &,
                                      const R *, const R *, const R *, const Shape &,
                                      const AvScalars &);
+// the last AV sweep of a step (z walk in 3D, y walk with the degenerate z
+// sweep in 2D) also reducing the next step's CFL candidates into cf.slot
+template <typename R>
+hipError_t launch_av_last_sweep_cfl(hipStream_t, const Fld<const R> &, const Fld<R> &,
+                                    const R *, const R *, const R *, const Shape &,
+                                    const AvScalars &, const CflFuse<R> &);
 template <typename R>
 hipError_t launch_char_vel(hipStream_t, const Fld<const R> &, R *, int, const AvScalars &);
 template <typename R>

@@ -108,6 +108,8 @@ SIGNATURES = {
                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "misohip_mlp_residual": [C.c_void_p, C.c_void_p, _P(MisohipGrid), C.c_void_p, C.c_void_p,
                              C.c_void_p, C.c_double, C.c_void_p],
+    "misohip_run": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
+                    _P(MisohipFields), _P(MisohipWork), C.c_int, _P(C.c_double)],
     "misohip_batch_create": [C.c_void_p, C.c_int, C.c_int, _P(MisohipGrid), _P(MisohipFields),
                              _P(MisohipWork), _P(MisohipParams), _P(MisohipBc), _P(C.c_void_p)],
     "misohip_batch_destroy": [C.c_void_p],

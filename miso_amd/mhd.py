@@ -310,6 +310,15 @@ class Integrator:
                  self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
                  C.byref(qq.c_struct()), C.byref(self._work), float(dt))
 
+    def run(self, bc: BoundaryCondition, qq: Fields, nsteps: int):
+        """nsteps of ModelBase::update (cfl + update) on qq, the CFL of each
+        later step reduced by the previous step's last sweep; returns the dts."""
+        dts = (C.c_double * max(1, nsteps))()
+        hip.call("misohip_run", self.ctx.handle, C.byref(self.grid.c_struct()), self.dtype_code,
+                 C.byref(self.params), C.byref(bc.c_struct()), C.byref(qq.c_struct()),
+                 C.byref(self._work), int(nsteps), dts)
+        return list(dts)[:nsteps]
+
     def step(self, bc: BoundaryCondition, qq: Fields) -> float:
         out = C.c_double()
         hip.call("misohip_step", self.ctx.handle, C.byref(self.grid.c_struct()), self.dtype_code,

@@ -218,9 +218,12 @@ def test_cfl(ctx, geom):
 
 
 def _run_both(ctx, problem, sizes, nsteps, dtype=np.float64, periodic=(1, 1, 1), sym=None,
-              cfl=0.4, av=None, gm=5.0 / 3.0, bounds=((0.0, 1.0),) * 3, src=None):
+              cfl=0.4, av=None, gm=5.0 / 3.0, bounds=((0.0, 1.0),) * 3, src=None,
+              use_run=False):
     """Oracle and GPU runs of the same initial state; a fixed axis (2) takes
-    its ghost values from the initial state."""
+    its ghost values from the initial state.  use_run: the GPU side advances
+    through one misohip_run call (CFL of later steps fused into the last
+    sweep) instead of nsteps misohip_step calls."""
     from miso_amd import mhd, problems
     from oracle import oracle as O
     src = src or {}
@@ -242,7 +245,10 @@ def _run_both(ctx, problem, sizes, nsteps, dtype=np.float64, periodic=(1, 1, 1),
         fx.copy_from_host(ctx, np.ascontiguousarray(q0.astype(dtype)))
     bc = mhd.BoundaryCondition(tuple(int(p) for p in periodic), sym, fixed=fx)
     m.apply_initial_condition(q0.astype(dtype), bc)
-    dts = [m.integrator.step(bc, m.qq) for _ in range(nsteps)]
+    if use_run:
+        dts = m.integrator.run(bc, m.qq, nsteps)
+    else:
+        dts = [m.integrator.step(bc, m.qq) for _ in range(nsteps)]
     out = m.qq.to_host(ctx).astype(np.float64)
     sl = (slice(None),) + g.interior()
     return out[sl], ref[sl], np.array(dts), ref_dts
@@ -274,6 +280,43 @@ def test_step_fp64(ctx, case):
     err = rel_linf(out, ref)
     assert max(err.values()) <= 1e-12, err
     assert_bitwise(out, ref, case)
+
+
+@pytest.mark.parametrize("case", list(STEP_CASES))
+def test_run_fp64(ctx, case):
+    """misohip_run (the next step's CFL reduced inside the last AV sweep)
+    gives the oracle's fields and dts bit for bit."""
+    problem, sizes, n, kw = STEP_CASES[case]
+    out, ref, dts, ref_dts = _run_both(ctx, problem, sizes, n, use_run=True, **kw)
+    assert np.array_equal(dts, ref_dts), (dts, ref_dts)
+    assert_bitwise(out, ref, f"run {case}")
+
+
+@pytest.mark.parametrize("sizes", [(96, 1, 1), (1, 80, 1), (1, 1, 72), (40, 1, 24)])
+def test_run_lower_dimensions(ctx, sizes):
+    """misohip_run on 1D and x-z runs, where the CFL stays a separate pass."""
+    out, ref, dts, ref_dts = _run_both(ctx, problems.turbulence, sizes, 6, use_run=True)
+    assert np.array_equal(dts, ref_dts), (dts, ref_dts)
+    assert_bitwise(out, ref, f"run {sizes}")
+
+
+def test_run_fp32_equals_steps(ctx):
+    """FP32: one misohip_run call and the same number of misohip_step calls
+    give identical bits and dts."""
+    import torch
+    from miso_amd import mhd
+    for sizes in ((64, 64, 1), (24, 24, 24)):
+        res = []
+        for use_run in (False, True):
+            g = mhd.Grid(*sizes, 2, dtype=np.float32, device="cuda:0")
+            m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
+            bc = mhd.BoundaryCondition((1, 1, 1))
+            m.apply_initial_condition(problems.to_numpy(problems.orszag_tang(g)).astype(np.float32), bc)
+            dts = (m.integrator.run(bc, m.qq, 7) if use_run
+                   else [m.integrator.step(bc, m.qq) for _ in range(7)])
+            res.append((m.qq.data.clone(), dts))
+        assert res[0][1] == res[1][1], (sizes, res[0][1], res[1][1])
+        assert torch.equal(res[0][0], res[1][0]), sizes
 
 
 @pytest.mark.parametrize("axis", [0, 1, 2])
