@@ -6,8 +6,8 @@ expose, by timing the candidates on the real state:
 - single runs (`tune_single`): the RK stage's pencil shape
   (`rk_pencil_bx` x `rk_pencil_by` blocks of 64 x 4 cells walked along z),
   its variant (`rk_variant`, 3D: load phases, register cap, LDS-staged x-y
-  neighbourhood) and the walk length of the y/z
-  AV sweeps (`sweep_chunk`), timed with the
+  neighbourhood, z-march with its planes per block `rk_zchunk`) and the
+  walk length of the y/z AV sweeps (`sweep_chunk`), timed with the
   library's own HIP events (misohip_profile_*) over a few steps;
 - batches (`tune_ensemble`): the y/z sweep walk length per run size, timed
   on a probe batch of that size.
@@ -28,6 +28,7 @@ from . import hip
 
 RK_PENCILS = ((1, 4), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (4, 16), (8, 4), (8, 8))
 CHUNKS = (16, 32, 64, 128, 256)
+ZCHUNKS = (8, 16, 32, 64, 128)
 
 
 def _device_key() -> str:
@@ -66,10 +67,11 @@ def get_tiles(ctx):
     return t
 
 
-def set_tiles(ctx, pencil_bx: int, pencil_by: int, chunk: int, variant: int = 0) -> None:
+def set_tiles(ctx, pencil_bx: int, pencil_by: int, chunk: int, variant: int = 0,
+              zchunk: int = 32) -> None:
     t = hip.MisohipTiles()
     t.rk_pencil_bx, t.rk_pencil_by, t.sweep_chunk = int(pencil_bx), int(pencil_by), int(chunk)
-    t.rk_variant = int(variant)
+    t.rk_variant, t.rk_zchunk = int(variant), int(zchunk)
     hip.call("misohip_set_tiles", ctx.handle, C.byref(t))
 
 
@@ -87,17 +89,19 @@ def _profile(ctx, fn):
 
 
 def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNKS,
-                cache: TileCache | None = None, force: bool = False):
-    """Choose and set the RK pencil and sweep chunk for `model` (a mhd.MHD);
-    returns {"rk_pencil": (bx, by), "sweep_chunk": c, "times_ms": {...},
-    "seconds": tuning time, "cached": bool}."""
+                zchunks=ZCHUNKS, cache: TileCache | None = None, force: bool = False):
+    """Choose and set the RK pencil, variant and z chunk and the sweep chunk
+    for `model` (a mhd.MHD); returns {"rk_pencil": (bx, by), "rk_variant": v,
+    "rk_zchunk": z, "sweep_chunk": c, "times_ms": {...}, "seconds": tuning
+    time, "cached": bool}."""
     import torch
     cache = cache or TileCache()
     g = model.grid
     key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}"
     hit = None if force else cache.get(key)
     if hit:
-        set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"], hit.get("rk_variant", 0))
+        set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"], hit.get("rk_variant", 0),
+                  hit.get("rk_zchunk", 32))
         return dict(hit, cached=True, seconds=0.0)
     t0 = time.perf_counter()
     saved = model.qq.data.clone()
@@ -111,24 +115,32 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
 
     run()  # warm-up (code objects, caches)
     rk_times = {}
-    variants = (0, 1, 2, 3) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
+    variants = (0, 1, 2, 3, 4) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
     for var in variants:
-        # the register-capped variants are tried on the default pencil only
-        for bx, by in (pencils if var in (0, 3) else ((2, 8),)):
-            set_tiles(ctx, bx, by, base.sweep_chunk, var)
-            rk_times[(bx, by, var)] = _profile(ctx, run).get("rk_stage", float("inf"))
+        if var == 4:  # the z-march kernel has no pencils; its knob is the z chunk
+            cands = [(1, 1, z) for z in zchunks]
+        elif var in (0, 3):
+            cands = [(bx, by, 32) for bx, by in pencils]
+        else:  # the register-capped variants are tried on the default pencil only
+            cands = [(2, 8, 32)]
+        for bx, by, z in cands:
+            set_tiles(ctx, bx, by, base.sweep_chunk, var, z)
+            rk_times[(bx, by, var, z)] = _profile(ctx, run).get("rk_stage", float("inf"))
     best_p = min(rk_times, key=rk_times.get)
+    bx, by, var, zc = best_p
     sweep_times = {}
     active = [k for k, s in (("av_y", g.j_size), ("av_z", g.k_size)) if s > 1]
     for c in chunks:
-        set_tiles(ctx, best_p[0], best_p[1], c, best_p[2])
+        set_tiles(ctx, bx, by, c, var, zc)
         t = _profile(ctx, run)
         sweep_times[c] = sum(t.get(k, 0.0) for k in active)
     best_c = min(sweep_times, key=sweep_times.get)
-    set_tiles(ctx, best_p[0], best_p[1], best_c, best_p[2])
+    set_tiles(ctx, bx, by, best_c, var, zc)
     torch.cuda.synchronize()
-    out = {"rk_pencil": list(best_p[:2]), "rk_variant": int(best_p[2]), "sweep_chunk": int(best_c),
-           "times_ms": {"rk_stage": {f"{a}x{b}v{v}": t for (a, b, v), t in rk_times.items()},
+    out = {"rk_pencil": [bx, by], "rk_variant": int(var), "rk_zchunk": int(zc),
+           "sweep_chunk": int(best_c),
+           "times_ms": {"rk_stage": {(f"z{z}v{v}" if v == 4 else f"{a}x{b}v{v}"): t
+                                     for (a, b, v, z), t in rk_times.items()},
                         "av_yz": {str(k): v for k, v in sweep_times.items()}}}
     cache.put(key, out)
     return dict(out, cached=False, seconds=time.perf_counter() - t0)
@@ -178,4 +190,4 @@ def summary(d) -> dict:
 
 
 __all__ = ["TileCache", "tune_single", "tune_ensemble", "set_tiles", "get_tiles", "summary",
-           "RK_PENCILS", "CHUNKS"]
+           "RK_PENCILS", "CHUNKS", "ZCHUNKS"]

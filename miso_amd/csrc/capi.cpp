@@ -107,6 +107,7 @@ int rk_stage_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   sc.pencil_bx = ctx->tile.rk_pencil_bx;
   sc.pencil_by = ctx->tile.rk_pencil_by;
   sc.variant = ctx->tile.rk_variant;
+  sc.zchunk = ctx->tile.rk_zchunk;
   MISOHIP_TRY(launch_rk_stage<R>(ctx->stream, cfld<R>(o), cfld<R>(a), fld<R>(r),
                                  static_cast<const R *>(g->dxi), static_cast<const R *>(g->dyi),
                                  static_cast<const R *>(g->dzi), make_shape(g), sc));
@@ -300,7 +301,7 @@ int misohip_create(int device, void *stream, misohip_ctx **out) {
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t) {
   if (!ctx || !t) return MISOHIP_E_ARG;
   if (t->rk_pencil_bx < 1 || t->rk_pencil_by < 1 || t->sweep_chunk < 4 || t->sweep_chunk > 4096 ||
-      t->rk_variant < 0 || t->rk_variant > 3)
+      t->rk_variant < 0 || t->rk_variant > 4 || t->rk_zchunk < 1 || t->rk_zchunk > 4096)
     return MISOHIP_E_ARG;
   ctx->tile = *t;
   return MISOHIP_OK;

@@ -79,6 +79,22 @@ struct Pencils {
   }
 };
 
+// Blocks of the z-march kernel: (nbx, nby) columns of 64 x 4 cells, each cut
+// into nzc chunks of kz planes; x fastest, then y, then the chunk, so that
+// with xcd_virtual_block every XCD walks compact x-y regions of one slab.
+struct March {
+  int nbx, nby, nzc, kz, nb;
+  __host__ bool init(int nbx_, int nby_, int nk, int kz_) {
+    if (kz_ < 1 || nk < 1) return false;
+    nbx = nbx_; nby = nby_; kz = kz_ > nk ? nk : kz_;
+    nzc = (nk + kz - 1) / kz;
+    const long n = (long)nbx * nby * nzc;
+    if (n > 2147483647L || n <= 0) return false;
+    nb = (int)n;
+    return true;
+  }
+};
+
 // Field set: nine device arrays in MISO's order ro vx vy vz bx by bz ei ph.
 template <typename T> struct Fld {
   T *q[9];

@@ -124,6 +124,59 @@ __device__ __forceinline__ void derivs_or_zero(const Acc &a, int c, int s, R dd,
   }
 }
 
+// The stage update of one cell from its direction sums (the tail of
+// update_sc4, mhd_integrator_host.hpp:113-243); gro is argm's density at the
+// cell (gravity source).
+template <typename R>
+__device__ __forceinline__ void rk_update(const Fld<const R> &orgn, const Fld<R> &rslt, int c,
+                                          R gro, const RkScalars &sc, const R dt, const R ch2,
+                                          const R damp, R a_ro, R a_vx, R d4x, R d5x, R t_vx,
+                                          R a_vy, R d4y, R d5y, R t_vy, R a_vz, R d4z, R d5z,
+                                          R t_vz, R a_bx, R a_by, R a_bz, R a_ph, R a_ei,
+                                          R t_ei) {
+  const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
+  const R o_ro = orgn.q[0][c], o_vx = orgn.q[1][c], o_vy = orgn.q[2][c], o_vz = orgn.q[3][c];
+  const R o_bx = orgn.q[4][c], o_by = orgn.q[5][c], o_bz = orgn.q[6][c];
+  const R o_ei = orgn.q[7][c], o_ph = orgn.q[8][c];
+
+  // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134)
+  R src[3] = {R(0), R(0), R(0)};
+  if (sc.src_kind == 1) src[sc.grav_axis] = -R(sc.grav) * gro;
+
+  // continuity
+  const R ro = o_ro + dt * a_ro;
+  // equations of motion: (ro v)_0 + dt*(flux - grad p - grad bb/8pi + tension/4pi + src)
+  const R vx = (o_ro * o_vx + dt * (a_vx - d4x - d5x + pii4 * t_vx + src[0])) / ro;
+  const R vy = (o_ro * o_vy + dt * (a_vy - d4y - d5y + pii4 * t_vy + src[1])) / ro;
+  const R vz = (o_ro * o_vz + dt * (a_vz - d4z - d5z + pii4 * t_vz + src[2])) / ro;
+  // induction
+  const R bx = o_bx + dt * a_bx;
+  const R by = o_by + dt * a_by;
+  const R bz = o_bz + dt * a_bz;
+  // GLM divergence cleaning; damp = exp(-dt/tau) from the host
+  const R ph = (o_ph + dt * ch2 * a_ph) * damp;
+  // energy: total energy of orgn, flux divergence, recovery of ei
+  const double eo_ro = o_ro;
+  const double Et = +eo_ro * double(o_ei) +
+                    0.5 * eo_ro * (+double(o_vx) * o_vx + double(o_vy) * o_vy + double(o_vz) * o_vz) +
+                    double(pii8) * (+double(o_bx) * o_bx + double(o_by) * o_by + double(o_bz) * o_bz);
+  const R eflux = dt * (a_ei + pii4 * t_ei);
+  const double dro = ro, dvx = vx, dvy = vy, dvz = vz, dbx = bx, dby = by, dbz = bz;
+  const double ei = (Et + double(eflux) - 0.5 * dro * (+dvx * dvx + dvy * dvy + dvz * dvz) -
+                     double(pii8) * (+dbx * dbx + dby * dby + dbz * dbz) + 0.0) /
+                    dro;
+
+  rslt.q[0][c] = ro;
+  rslt.q[1][c] = vx;
+  rslt.q[2][c] = vy;
+  rslt.q[3][c] = vz;
+  rslt.q[4][c] = bx;
+  rslt.q[5][c] = by;
+  rslt.q[6][c] = bz;
+  rslt.q[7][c] = R(ei);
+  rslt.q[8][c] = ph;
+}
+
 // Energy recovery is carried in double for both storage types ("pressure
 // recovery from total energy" is the cancellation-prone step).
 // One 64 x 4 block of cells (x-tile blk_x, y-rows 4*blk_y.., plane blk_z);
@@ -160,7 +213,7 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   const int c = sh.idx(i, j, k);
   const int tc = ((threadIdx.x >> 6) + 2) * RK_TX + (threadIdx.x & 63) + 2;
   const R gm1 = R(sc.gm) - R(1);
-  const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
+  const R pii8 = Const<R>::pii8;
   R S[17];
 
   // ---- x: start every running sum
@@ -230,46 +283,9 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   a_ei = a_ei - S[15];
   t_ei = t_ei + S[16];
 
-  const R o_ro = orgn.q[0][c], o_vx = orgn.q[1][c], o_vy = orgn.q[2][c], o_vz = orgn.q[3][c];
-  const R o_bx = orgn.q[4][c], o_by = orgn.q[5][c], o_bz = orgn.q[6][c];
-  const R o_ei = orgn.q[7][c], o_ph = orgn.q[8][c];
-
-  // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134)
-  R src[3] = {R(0), R(0), R(0)};
-  if (sc.src_kind == 1) src[sc.grav_axis] = -R(sc.grav) * argm.q[0][c];
-
-  // continuity
-  const R ro = o_ro + dt * a_ro;
-  // equations of motion: (ro v)_0 + dt*(flux - grad p - grad bb/8pi + tension/4pi + src)
-  const R vx = (o_ro * o_vx + dt * (a_vx - d4x - d5x + pii4 * t_vx + src[0])) / ro;
-  const R vy = (o_ro * o_vy + dt * (a_vy - d4y_ - d5y_ + pii4 * t_vy + src[1])) / ro;
-  const R vz = (o_ro * o_vz + dt * (a_vz - d4z - d5z + pii4 * t_vz + src[2])) / ro;
-  // induction
-  const R bx = o_bx + dt * a_bx;
-  const R by = o_by + dt * a_by;
-  const R bz = o_bz + dt * a_bz;
-  // GLM divergence cleaning; damp = exp(-dt/tau) from the host
-  const R ph = (o_ph + dt * ch2 * a_ph) * damp;
-  // energy: total energy of orgn, flux divergence, recovery of ei
-  const double eo_ro = o_ro;
-  const double Et = +eo_ro * double(o_ei) +
-                    0.5 * eo_ro * (+double(o_vx) * o_vx + double(o_vy) * o_vy + double(o_vz) * o_vz) +
-                    double(pii8) * (+double(o_bx) * o_bx + double(o_by) * o_by + double(o_bz) * o_bz);
-  const R eflux = dt * (a_ei + pii4 * t_ei);
-  const double dro = ro, dvx = vx, dvy = vy, dvz = vz, dbx = bx, dby = by, dbz = bz;
-  const double ei = (Et + double(eflux) - 0.5 * dro * (+dvx * dvx + dvy * dvy + dvz * dvz) -
-                     double(pii8) * (+dbx * dbx + dby * dby + dbz * dbz) + 0.0) /
-                    dro;
-
-  rslt.q[0][c] = ro;
-  rslt.q[1][c] = vx;
-  rslt.q[2][c] = vy;
-  rslt.q[3][c] = vz;
-  rslt.q[4][c] = bx;
-  rslt.q[5][c] = by;
-  rslt.q[6][c] = bz;
-  rslt.q[7][c] = R(ei);
-  rslt.q[8][c] = ph;
+  const R gro = sc.src_kind == 1 ? argm.q[0][c] : R(0);
+  rk_update<R>(orgn, rslt, c, gro, sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x, t_vx, a_vy, d4y_,
+               d5y_, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph, a_ei, t_ei);
 }
 
 template <typename R, bool HX, bool HY, bool HZ, int SPLIT>
@@ -310,11 +326,210 @@ __global__ __launch_bounds__(256) void rk_stage_kernel_tile(Fld<const R> orgn, F
                                          R(sc.ch2), R(sc.damp), blk_x, blk_y, blk_z);
 }
 
+// ------------------------------------------------------------ z-march (3D)
+// A block owns one 64 x 4 column of cells and walks it up through `kz`
+// planes.  The z stencil reads no global memory: each thread keeps its own
+// cell's fields at planes k-2 and k-1 in registers and at planes k+1 and k+2
+// in two thread-private LDS slots (FP64 has no room for all four in
+// registers at two waves per SIMD); plane k is the centre of the 68 x 8 x-y
+// tile in LDS, which is rebuilt for every plane from the slot of plane k+1
+// and from halo loads issued during the previous plane's update; plane k+3
+// is loaded while plane k is computed.  Global loads per cell: 9 (new plane)
+// + 9 * 288/256 (halo, L2 hits when the neighbour columns run on the same
+// XCD) + 9 (orgn), against 64 for the one-plane-per-block kernels.  The
+// arithmetic is rk_block's, term for term.
+constexpr int RK_HALO = RK_TILE - 256;  // 288 halo cells around a 64 x 4 block
+
+// Tile position of halo slot s (0..287): rows 0-1, rows 6-7, then columns
+// 0, 1, 66, 67 of rows 2-5.
+__device__ __forceinline__ int rk_halo_pos(int s) {
+  if (s < 2 * RK_TX) return s;
+  if (s < 4 * RK_TX) return 6 * RK_TX + (s - 2 * RK_TX);
+  const int q = s - 4 * RK_TX, col = q & 3;
+  return (2 + (q >> 2)) * RK_TX + (col < 2 ? col : 64 + col);
+}
+
+// dir_derivs with the four stencil points' fields given.
+template <typename R, int D>
+__device__ __forceinline__ void derivs_pts(const R (&p2)[9], const R (&p1)[9], const R (&m1)[9],
+                                           const R (&m2)[9], R dd, R gm1, R S[17]) {
+  R t[17];
+  point_terms<R, D>(p2, gm1, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = -t[n];
+  point_terms<R, D>(p1, gm1, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(8), t[n], S[n]);
+  point_terms<R, D>(m1, gm1, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(-8), t[n], S[n]);
+  point_terms<R, D>(m2, gm1, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
+                      const R *__restrict__ dxi, const R *__restrict__ dyi,
+                      const R *__restrict__ dzi, Shape sh, RkScalars sc, March mc) {
+  __shared__ R lds[9 * RK_TILE];
+  __shared__ R slot[2 * 9 * 256];  // [plane parity][field][thread]
+  const int V = xcd_virtual_block(blockIdx.x, mc.nb);
+  const int bx = V % mc.nbx, rr = V / mc.nbx, by = rr % mc.nby, zc = rr / mc.nby;
+  const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  const int i = bx * 64 + tx + sh.mi, j = by * 4 + ty + sh.mj;
+  const int kb = zc * mc.kz + sh.mk;
+  const int ke = min(kb + mc.kz, sh.Kt - sh.mk);
+  const bool valid = i < sh.It - sh.mi && j < sh.Jt - sh.mj;
+  const int plane = sh.It * sh.Jt;
+  // in-plane offsets; a position outside the array reads cell (0, 0) of the
+  // plane instead, whose value no valid cell uses
+  const int pc = (i < sh.It && j < sh.Jt) ? i + sh.It * j : 0;
+  const int i0 = bx * 64 + sh.mi - 2, j0 = by * 4 + sh.mj - 2;
+  const bool has1 = tid + 256 < RK_HALO;
+  const int p0 = rk_halo_pos(tid), p1 = rk_halo_pos(has1 ? tid + 256 : 0);
+  const int g0i = i0 + p0 % RK_TX, g0j = j0 + p0 / RK_TX;
+  const int g1i = i0 + p1 % RK_TX, g1j = j0 + p1 / RK_TX;
+  const int h0 = (g0i < sh.It && g0j < sh.Jt) ? g0i + sh.It * g0j : 0;
+  const int h1 = (g1i < sh.It && g1j < sh.Jt) ? g1i + sh.It * g1j : 0;
+  const int tc = (ty + 2) * RK_TX + tx + 2;
+  const R gm1 = R(sc.gm) - R(1);
+  const R pii8 = Const<R>::pii8;
+  const R ddx = dxi[valid ? i : sh.mi], ddy = dyi[valid ? j : sh.mj];
+  const R dt = R(sc.dt), ch2 = R(sc.ch2), damp = R(sc.damp);
+  auto sl = [&](int k, int m) -> R & { return slot[((k & 1) * 9 + m) * 256 + tid]; };
+
+  R zm2[9], zm1[9];  // own cell at planes k-2, k-1
+  R zn[9];           // own cell at plane k+2, stored to its slot before the z phase
+  {
+    const int c = pc + kb * plane, h = kb * plane;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+      zm2[m] = argm.q[m][c - 2 * plane];
+      zm1[m] = argm.q[m][c - plane];
+      sl(kb + 1, m) = argm.q[m][c + plane];
+      zn[m] = argm.q[m][c + 2 * plane];
+      lds[m * RK_TILE + tc] = argm.q[m][c];
+      lds[m * RK_TILE + p0] = argm.q[m][h + h0];
+      if (has1) lds[m * RK_TILE + p1] = argm.q[m][h + h1];
+    }
+  }
+  __syncthreads();
+
+  for (int k = kb; k < ke; ++k) {
+    const bool more = k + 1 < ke;  // uniform over the block
+    R S[17];
+    // ---- x (LDS tile)
+    dir_derivs<R, 0, 1>(TileAcc<R>{lds}, tc, 1, ddx, gm1, S);
+    R a_ro = -S[0];
+    R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
+    R a_vy = -S[2], t_vy = S[7];
+    R a_vz = -S[3], t_vz = S[8];
+    R a_by = -S[9], k_by = S[10];
+    R a_bz = -S[11], k_bz = S[12];
+    R ph_bx = S[13];
+    R a_ph = -S[14];
+    R a_ei = -S[15], t_ei = S[16];
+    int cy = tc;
+    asm volatile("" : "+v"(cy), "+v"(a_ro), "+v"(a_vx), "+v"(d4x), "+v"(d5x), "+v"(t_vx),
+                 "+v"(a_vy), "+v"(t_vy), "+v"(a_vz), "+v"(t_vz), "+v"(a_by), "+v"(k_by),
+                 "+v"(a_bz), "+v"(k_bz), "+v"(ph_bx), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
+    // ---- y (LDS tile)
+    dir_derivs<R, 1, 1>(TileAcc<R>{lds}, cy, RK_TX, ddy, gm1, S);
+    a_ro = a_ro - S[0];
+    a_vx = a_vx - S[1];
+    t_vx = t_vx + S[6];
+    a_vy = a_vy - S[2];
+    const R d4y = S[4], d5y = S[5] * pii8;
+    t_vy = t_vy + S[7];
+    a_vz = a_vz - S[3];
+    t_vz = t_vz + S[8];
+    R a_bx = -S[9];
+    const R k_bx = S[10];
+    const R ph_by = S[13];
+    a_bz = a_bz - S[11] + k_bz + S[12];
+    a_ph = a_ph - S[14];
+    a_ei = a_ei - S[15];
+    t_ei = t_ei + S[16];
+    // plane k+2 (loaded during the previous plane) into the slot plane k left
+#pragma unroll
+    for (int m = 0; m < 9; ++m) sl(k + 2, m) = zn[m];
+    // ---- z (slots and registers)
+    {
+      R zp2[9], zp1[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+        zp2[m] = sl(k + 2, m);
+        zp1[m] = sl(k + 1, m);
+      }
+      derivs_pts<R, 2>(zp2, zp1, zm1, zm2, dzi[k], gm1, S);
+    }
+    a_ro = a_ro - S[0];
+    a_vx = a_vx - S[1];
+    t_vx = t_vx + S[6];
+    a_vy = a_vy - S[2];
+    t_vy = t_vy + S[7];
+    a_vz = a_vz - S[3];
+    const R d4z = S[4], d5z = S[5] * pii8;
+    t_vz = t_vz + S[8];
+    a_bx = a_bx - S[9] + k_bx + S[10] - ph_bx;
+    a_by = a_by - S[11] + k_by + S[12] - ph_by;
+    a_bz = a_bz - S[13];
+    a_ph = a_ph - S[14];
+    a_ei = a_ei - S[15];
+    t_ei = t_ei + S[16];
+    // plane k+1's halo lands during the update
+    R ha[9], hb[9];
+    if (more) {
+      const int h = (k + 1) * plane;
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+        ha[m] = argm.q[m][h + h0];
+        hb[m] = has1 ? argm.q[m][h + h1] : R(0);
+      }
+    }
+    R ctr[9];  // plane k's own values: the tile centre
+#pragma unroll
+    for (int m = 0; m < 9; ++m) ctr[m] = lds[m * RK_TILE + tc];
+    if (valid)
+      rk_update<R>(orgn, rslt, pc + k * plane, ctr[0], sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x,
+                   t_vx, a_vy, d4y, d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph,
+                   a_ei, t_ei);
+    // plane k+3, used in the next plane's z phase
+    if (more && k + 3 < sh.Kt) {
+      const int cn = pc + (k + 3) * plane;
+#pragma unroll
+      for (int m = 0; m < 9; ++m) zn[m] = argm.q[m][cn];
+    }
+    __syncthreads();  // every wave is done with plane k's tile
+    if (more) {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+        lds[m * RK_TILE + tc] = sl(k + 1, m);
+        lds[m * RK_TILE + p0] = ha[m];
+        if (has1) lds[m * RK_TILE + p1] = hb[m];
+        zm2[m] = zm1[m];
+        zm1[m] = ctr[m];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <typename R, bool HX, bool HY, bool HZ>
 static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
                               const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
                               const Shape &sh, const RkScalars &sc) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  if (HX && HY && HZ && sc.variant == 4) {
+    March mc;
+    if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 || !mc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.zchunk))
+      return hipErrorInvalidValue;
+    hipLaunchKernelGGL((rk_stage_kernel_march<R>), dim3(mc.nb), dim3(256), 0, st, o, a, r, dxi,
+                       dyi, dzi, sh, sc, mc);
+    return hipGetLastError();
+  }
   Pencils pc;
   if (!pc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.pencil_bx, sc.pencil_by))
     return hipErrorInvalidValue;

@@ -68,19 +68,21 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
     import torch
     from miso_amd import mhd, problems
     from miso_amd.autotune import set_tiles
-    g = mhd.Grid(70, 36, 20, 2, device="cuda:0")
-    bc = mhd.BoundaryCondition((True, True, True))
-    m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
-    m.qq.load_ijk(problems.turbulence(g, device="cpu"))
-    m.integrator.apply_boundary_condition(bc, m.qq)
-    start = m.qq.data.clone()
-    outs = []
-    for tiles in ((2, 8, 64, 0), (1, 4, 16, 1), (4, 16, 128, 2), (2, 4, 32, 2), (4, 8, 64, 3),
-                  (1, 4, 64, 3)):
-        m.qq.data.copy_(start)
-        set_tiles(ctx, *tiles)
-        m.integrator.step(bc, m.qq)
-        outs.append(m.qq.data.clone())
-    set_tiles(ctx, 2, 8, 64, 0)
-    for o in outs[1:]:
-        assert torch.equal(o, outs[0])
+    for sizes in ((70, 36, 20), (66, 37, 13)):  # partial x and y tiles, uneven z chunks
+        g = mhd.Grid(*sizes, 2, device="cuda:0")
+        bc = mhd.BoundaryCondition((True, True, True))
+        m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
+        m.qq.load_ijk(problems.turbulence(g, device="cpu"))
+        m.integrator.apply_boundary_condition(bc, m.qq)
+        start = m.qq.data.clone()
+        outs = []
+        for tiles in ((2, 8, 64, 0), (1, 4, 16, 1), (4, 16, 128, 2), (2, 4, 32, 2), (4, 8, 64, 3),
+                      (1, 4, 64, 3), (1, 1, 64, 4, 1), (1, 1, 64, 4, 3), (1, 1, 32, 4, 8),
+                      (1, 1, 64, 4, 64)):
+            m.qq.data.copy_(start)
+            set_tiles(ctx, *tiles)
+            m.integrator.step(bc, m.qq)
+            outs.append(m.qq.data.clone())
+        set_tiles(ctx, 2, 8, 64, 0)
+        for n, o in enumerate(outs[1:], 1):
+            assert torch.equal(o, outs[0]), (sizes, n)
