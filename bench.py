@@ -61,8 +61,13 @@ def parse():
                          "solved values (default: no flags)")
     ap.add_argument("--rmhd-no-correct", action="store_true",
                     help="do not add the correction the last plain solve leaves")
-    ap.add_argument("--rmhd-train-columns", type=int, default=100000)
-    ap.add_argument("--rmhd-epochs", type=int, default=60)
+    ap.add_argument("--rmhd-train-columns", type=int, default=200000)
+    ap.add_argument("--rmhd-train-seeds", type=int, default=4,
+                    help="boxes (other seeds) the training columns come from")
+    ap.add_argument("--rmhd-epochs", type=int, default=200)
+    ap.add_argument("--rmhd-slab-share", type=float, default=0.95,
+                    help="share of the network's residual at a solve whose bottom layers are "
+                         "solved plainly on every step (<= 0: no slab)")
     ap.add_argument("--rmhd-features", type=int, default=4, choices=(3, 4),
                     help="column inputs: 3 = (ln ro, ln T, B^2), 4 = plus the horizontal "
                          "Laplacian of ln T")
@@ -411,7 +416,8 @@ def rmhd_leg(args):
     # training data from other seeds, over the same span of steps
     t0 = time.perf_counter()
     data_box = make_box(n, 101)
-    F, T = collect_columns(data_box, seeds=(101, 102), steps=nsteps + every,
+    F, T = collect_columns(data_box, seeds=tuple(range(101, 101 + args.rmhd_train_seeds)),
+                           steps=nsteps + every,
                            sample_every=4, max_columns=args.rmhd_train_columns, log=log,
                            features=args.rmhd_features)
     torch.cuda.synchronize()
@@ -429,10 +435,11 @@ def rmhd_leg(args):
     del F, T
     torch.cuda.empty_cache()
     log(f"trained: mse {mse:.3e}, validation rel L2 {val:.3e}, {train_s:.1f} s")
+    slab_share = args.rmhd_slab_share if args.rmhd_slab_share > 0 else None
     # timed learned run: one untimed cycle, then args.rmhd_cycles cycles
     box = make_box(n, 7)
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
-                         correct=not args.rmhd_no_correct)
+                         correct=not args.rmhd_no_correct, slab_share=slab_share)
     for _ in range(every):
         box.step_learned()
     torch.cuda.synchronize()
@@ -443,13 +450,18 @@ def rmhd_leg(args):
     el = time.perf_counter() - t0
     learned = {"cells": n ** 3, "steps": nsteps, "rt_every": every,
                "ms_per_step": 1e3 * el / nsteps, "steps_per_s": nsteps / el,
-               "flagged_column_fraction": float(box.flags.float().mean())}
+               "flagged_column_fraction": float(box.flags.float().mean()),
+               "bottom_slab": None if box.slab is None else {
+                   "kept_layers": box.slab.layers, "buffer_layers": box.slab.buffer,
+                   "residual_share": slab_share,
+                   "rt_iterations_mean": float(np.mean(box.slab.iters)) if box.slab.iters
+                   else None}}
     del box
     torch.cuda.empty_cache()
     # accuracy: the same run again, Q_rad checked against converged solves
     box = make_box(n, 7)
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
-                         correct=not args.rmhd_no_correct)
+                         correct=not args.rmhd_no_correct, slab_share=slab_share)
     errs = {"learned": [], "network_only": [], "network_corrected": [], "stale_rt": []}
     by_age = {}
     tmp = torch.empty_like(box.q_rt)
@@ -479,6 +491,7 @@ def rmhd_leg(args):
     learned["qrad_rel_l2_learned_by_steps_since_solve"] = {
         str(a): float(np.mean(v)) for a, v in sorted(by_age.items())}
     learned["training"] = {"columns": ncols, "epochs": args.rmhd_epochs, "train_mse": mse,
+                           "seeds": args.rmhd_train_seeds,
                            "features": args.rmhd_features,
                            "validation_rel_l2": val, "data_seconds": gen_s,
                            "train_seconds": train_s}

@@ -10,9 +10,12 @@ mhd_integrator_host.hpp) and no coupling; this module adds one:
                    heating ei += dt Q_rad / ro and a ghost fill.
   step (learned) : Q_rad from the network on every step; every `rt_every`
                    steps a plain RT solve instead, whose result also sets the
-                   residual flags (columns where the network misses the
-                   solve by more than `flag_tol` in relative L2) that take the
-                   last solved Q_rad until the next solve.
+                   residual indicators: flags (columns where the network
+                   misses the solve by more than `flag_tol` in relative L2)
+                   that take the last solved Q_rad until the next solve, and
+                   the bottom slab (the optically thick layers next to the
+                   bottom boundary that hold most of the network's residual
+                   at the solve) where the plain solver runs on every step.
 
 Workload: a gravity-stratified ideal gas (T falling linearly from T_bot to
 T_top over the box height, hydrostatic pressure with the scale height
@@ -31,13 +34,66 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .mhd import BoundaryCondition, Config, MHD
+from .mhd import BoundaryCondition, Config, Grid, MHD
 from .problems import random_modes_field, _periodic_pad
 from .rt import RT
 
 RO, VX, VY, VZ, BX, BY, BZ, EI, PH = range(9)
 
 RT_FACES = dict(periodic=(True, True, False), face_kind=(0, 0, 0, 0, 1, 2))
+# the bottom slab: thermalised at both z faces (its top face lies deep in the
+# optically thick layers, where the downward intensity is the local source)
+SLAB_FACES = dict(periodic=(True, True, False), face_kind=(0, 0, 0, 0, 1, 1))
+
+
+class BottomSlabRT:
+    """Plain RT solve of the bottom `layers + buffer` interior layers of a
+    box: the fields of those layers and the ghost layers below are the first
+    elements of every (x-fastest) field array, so the slab reads the box's
+    own arrays.  The `buffer` layers above the kept ones carry an optical
+    depth large enough that the approximate top boundary (I = S) does not
+    reach the kept layers; apply() overwrites the kept layers of a Q_rad
+    field with the slab's."""
+
+    def __init__(self, grid, layers: int, buffer: int, num_rays: int = 24):
+        ls = layers + buffer
+        z0, z1 = grid.bounds[2]
+        dz = (z1 - z0) / grid.k_size
+        self.grid = Grid(grid.i_size, grid.j_size, ls, grid.k_margin,
+                         bounds=(grid.bounds[0], grid.bounds[1], (z0, z0 + ls * dz)),
+                         dtype=grid.dtype, device=grid.device)
+        assert (self.grid.i_total, self.grid.j_total) == (grid.i_total, grid.j_total)
+        self.layers, self.buffer = int(layers), int(buffer)
+        self.rt = RT(self.grid, num_rays)
+        self.iters = []
+
+    def apply(self, ctx, qq, gm, kappa0, s_rad, q_full, **rt_kw):
+        self.rt.source_from_fields(ctx, qq, gm, kappa0, s_rad)
+        it, _ = self.rt.solve(ctx, **dict(rt_kw, **SLAB_FACES))
+        self.iters.append(it)
+        q = self.rt.qrad(ctx, **dict(rt_kw, **SLAB_FACES))
+        m = self.grid.k_margin
+        q_full[m:m + self.layers].copy_(q[m:m + self.layers])
+
+
+def slab_layers(resid2, kdz, share: float, tau_buffer: float, max_layers: int):
+    """(kept, buffer) layers of the bottom slab from the network's residual
+    at a solve: kept = the fewest bottom layers holding `share` of the
+    residual's squared norm, buffer = the fewest layers above them whose
+    optical depth (sum of mean kappa dz) reaches `tau_buffer`; (0, 0) when
+    that does not fit in `max_layers` (the residual is not at the bottom)."""
+    r = np.asarray(resid2, dtype=np.float64)
+    tot = r.sum()
+    if tot <= 0.0:
+        return 0, 0
+    kept = int(np.searchsorted(np.cumsum(r) / tot, share) + 1)
+    tau, buf = 0.0, 0
+    while tau < tau_buffer and kept + buf < len(kdz):
+        tau += float(kdz[kept + buf])
+        buf += 1
+    if tau < tau_buffer or kept + buf > max_layers:
+        return 0, 0
+    return kept, buf
 
 
 @dataclass
@@ -138,6 +194,7 @@ class RadiativeMHD:
         self.surrogate = None
         self.flags = None
         self.q_rt = None                     # last solved Q_rad (learned mode)
+        self.slab = None                     # bottom slab solved every step
         self.steps_done = 0
 
     # -- set-up -------------------------------------------------------------
@@ -173,18 +230,25 @@ class RadiativeMHD:
         return r.qrad(self.ctx, **self.rt_kw)
 
     def attach_surrogate(self, surrogate, rt_every: int = 16, flag_tol: float | None = None,
-                         correct: bool = True) -> None:
+                         correct: bool = True, slab_share: float | None = 0.95,
+                         slab_tau: float = 10.0, slab_max_frac: float = 0.25) -> None:
         """Learned RT: the network every step, a plain solve every rt_every
         steps.  correct: add the column correction the last solve left (the
         solve minus the network on the same state), so the network carries
         the change since the solve; flag_tol: columns whose network misses
         the solve by more than this (relative L2) take the solved values
-        until the next solve (None: no flags)."""
+        until the next solve (None: no flags); slab_share: at every solve
+        the bottom layers holding this share of the network's squared
+        residual, topped by slab_tau of optical depth, are handed to a
+        plain slab solve on every step, if they fit in slab_max_frac of the
+        height (None: no slab)."""
         import torch
         self.surrogate = surrogate
         self.rt_every = int(rt_every)
         self.flag_tol = flag_tol
         self.correct = bool(correct)
+        self.slab_share, self.slab_tau, self.slab_max_frac = slab_share, slab_tau, slab_max_frac
+        self.slab = None
         ncol = self.grid.i_size * self.grid.j_size
         self.flags = torch.zeros(ncol, dtype=torch.int32, device=self.grid.device)
         self.q_rt = torch.zeros_like(self.rt.qrad_field)
@@ -227,13 +291,37 @@ class RadiativeMHD:
                                    flags=None, q_rt=None)
             tol = self.flag_tol if self.flag_tol is not None else float("inf")
             self.surrogate.residual(self.grid, self.q_rt, self.flags, tol, self.corr)
+            if self.slab_share is not None:
+                self._choose_slab()
             if probe:
                 probe(self.q_rt, True)
             return self._advance(self.q_rt)
         q = self.learned_qrad()
+        if self.slab is not None:
+            self.slab.apply(self.ctx, self.model.qq, self.strat.gm, self.kappa0, self.s_rad, q,
+                            **self.rt_kw)
         if probe:
             probe(q, False)
         return self._advance(q)
+
+    def _choose_slab(self) -> None:
+        """Bottom slab from the network's residual on the just-solved state
+        (self.q_mlp against self.q_rt, per height) and the layers' mean
+        optical thickness; rebuilt only when the kept layers grow or halve."""
+        g = self.grid
+        m = g.k_margin
+        inner = (slice(m, m + g.k_size), slice(g.j_margin, g.j_margin + g.j_size),
+                 slice(g.i_margin, g.i_margin + g.i_size))
+        d = (self.q_mlp - self.q_rt)[inner]
+        resid2 = (d * d).sum(dim=(1, 2)).cpu().numpy()
+        kdz = (self.rt.abs_coeff[inner].mean(dim=(1, 2)) * float(g.dz[m])).cpu().numpy()
+        kept, buf = slab_layers(resid2, kdz, self.slab_share, self.slab_tau,
+                                int(self.slab_max_frac * g.k_size))
+        cur = self.slab
+        if kept == 0:
+            self.slab = None
+        elif cur is None or kept > cur.layers or 2 * kept < cur.layers or buf != cur.buffer:
+            self.slab = BottomSlabRT(g, kept, buf)
 
 
 def rel_l2(a, b, grid) -> float:

@@ -42,6 +42,27 @@ def test_kappa0_gives_unit_optical_depth_at_mid_height():
     assert abs(tc.min() - s.t_relax) < 1e-12 * s.t_relax
 
 
+def test_slab_layers_rule():
+    """Kept layers: the fewest bottom layers with the asked share of the
+    residual; buffer: the fewest layers above with the asked optical depth;
+    nothing when the slab would not fit or the residual is elsewhere."""
+    from miso_amd.rmhd import slab_layers
+    r = np.zeros(64)
+    r[:3] = [50.0, 30.0, 15.0]
+    r[3:] = 5.0 / 61
+    kdz = np.linspace(8.0, 0.01, 64)
+    kept, buf = slab_layers(r, kdz, 0.95, 10.0, 16)
+    assert kept == 3
+    assert kdz[3:3 + buf].sum() >= 10.0 > kdz[3:3 + buf - 1].sum()
+    assert slab_layers(r, kdz, 0.999, 10.0, 16) == (0, 0)      # needs most of the height
+    mid = np.zeros(64)
+    mid[30:34] = 1.0
+    assert slab_layers(mid, kdz, 0.95, 10.0, 16) == (0, 0)      # residual in the thin layers
+    assert slab_layers(np.zeros(64), kdz, 0.95, 10.0, 16) == (0, 0)
+    thin = np.full(64, 0.01)
+    assert slab_layers(r, thin, 0.95, 10.0, 64) == (0, 0)      # never optically deep
+
+
 # ---------------------------------------------------------------------- GPU
 def _box(ctx, n_xy, nz, seed=3, **kw):
     from miso_amd.mhd import Grid
@@ -169,6 +190,36 @@ def test_learned_rt_beats_stale_rt_between_solves(ctx, features):
           f"stale RT {es:.3e}", flush=True)
     assert ec < es
     sur.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kept", [2, 6])
+def test_bottom_slab_solve_matches_the_full_solve(ctx, kept):
+    """A plain RT solve of the bottom kept + buffer layers (buffer: optical
+    depth 10) reproduces the full solve's Q_rad on the kept layers."""
+    import torch
+    from miso_amd.rmhd import BottomSlabRT, slab_layers
+    box = _box(ctx, 32, 64)
+    for _ in range(3):
+        box.step_plain()
+    full = box.reference_qrad().clone()
+    g = box.grid
+    m = g.k_margin
+    inner = (slice(m, m + g.k_size), slice(m, m + g.j_size), slice(m, m + g.i_size))
+    kdz = (box._rt_ref.abs_coeff[inner].mean(dim=(1, 2)) * float(g.dz[m])).cpu().numpy()
+    r = np.zeros(g.k_size)
+    r[:kept] = 1.0
+    k, buf = slab_layers(r, kdz, 0.999, 10.0, g.k_size // 2)
+    assert k == kept and buf >= 1
+    slab = BottomSlabRT(g, k, buf)
+    q = torch.zeros_like(full)
+    slab.apply(ctx, box.model.qq, box.strat.gm, box.kappa0, box.s_rad, q, **box.rt_kw)
+    a = q[m:m + kept, m:m + g.j_size, m:m + g.i_size]
+    b = full[m:m + kept, m:m + g.j_size, m:m + g.i_size]
+    err = float((a - b).norm() / b.norm())
+    print(f"kept {kept} buffer {buf}: slab vs full rel L2 {err:.2e}", flush=True)
+    assert err < 1e-5
+    assert torch.count_nonzero(q[m + kept:]) == 0
 
 
 @pytest.mark.gpu
