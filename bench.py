@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--rmhd-train-seeds", type=int, default=4,
                     help="boxes (other seeds) the training columns come from")
     ap.add_argument("--rmhd-epochs", type=int, default=200)
-    ap.add_argument("--rmhd-slab-share", type=float, default=0.95,
+    ap.add_argument("--rmhd-slab-share", type=float, default=0.99,
                     help="share of the network's residual at a solve whose bottom layers are "
                          "solved plainly on every step (<= 0: no slab)")
     ap.add_argument("--rmhd-features", type=int, default=4, choices=(3, 4),

@@ -230,7 +230,7 @@ class RadiativeMHD:
         return r.qrad(self.ctx, **self.rt_kw)
 
     def attach_surrogate(self, surrogate, rt_every: int = 16, flag_tol: float | None = None,
-                         correct: bool = True, slab_share: float | None = 0.95,
+                         correct: bool = True, slab_share: float | None = 0.99,
                          slab_tau: float = 10.0, slab_max_frac: float = 0.25) -> None:
         """Learned RT: the network every step, a plain solve every rt_every
         steps.  correct: add the column correction the last solve left (the
