@@ -88,8 +88,8 @@ def parse():
 
 def rk_kernel_name(variant):
     """Name of the RK stage kernel a 3D run launches for a tile variant."""
-    return {1: "rk_stage_kernel<", 2: "rk_stage_kernel_w4<", 3: "rk_stage_kernel_tile<"}.get(
-        variant, "rk_stage_kernel<")
+    return {1: "rk_stage_kernel<", 2: "rk_stage_kernel_w4<", 3: "rk_stage_kernel_tile<",
+            4: "rk_stage_kernel_march<"}.get(variant, "rk_stage_kernel<")
 
 
 def rk_traffic(n, dtype, variant=0):
