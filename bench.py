@@ -86,10 +86,14 @@ def parse():
     return ap.parse_args()
 
 
-def rk_kernel_name(variant):
-    """Name of the RK stage kernel a 3D run launches for a tile variant."""
-    return {1: "rk_stage_kernel<", 2: "rk_stage_kernel_w4<", 3: "rk_stage_kernel_tile<",
-            4: "rk_stage_kernel_march<"}.get(variant, "rk_stage_kernel<")
+def rk_kernel_name(variant, dtype="f64"):
+    """Name (as the profiles list it) of the RK stage kernel a 3D run
+    launches for a tile variant."""
+    t = "double" if dtype == "f64" else "float"
+    return {0: f"rk_stage_kernel<{t}, true, true, true, 0>",
+            1: f"rk_stage_kernel<{t}, true, true, true, 1>",
+            2: f"rk_stage_kernel_w4<{t}, true, true, true>", 3: f"rk_stage_kernel_tile<{t}>",
+            4: f"rk_stage_kernel_march<{t}, 4>", 5: f"rk_stage_kernel_march<{t}, 8>"}[variant]
 
 
 def rk_traffic(n, dtype, variant=0):
@@ -101,7 +105,7 @@ def rk_traffic(n, dtype, variant=0):
                         f"mhd_step_turbulence_{n}cubed_{dtype}.md")
     if not os.path.exists(path):
         return None
-    name = "| `" + rk_kernel_name(variant)
+    name = "| `" + rk_kernel_name(variant, dtype) + "`"
     for line in open(path):
         cols = line.split("|")
         if line.startswith(name) and len(cols) >= 10:  # the traffic table
@@ -615,7 +619,7 @@ def main():
         "hbm_roofline_frac_step": step_frac,
         "roofline": {
             "bound": "hbm",
-            "kernel": rk_kernel_name((tiles or {}).get("rk_variant", 0)).rstrip("<") +
+            "kernel": rk_kernel_name((tiles or {}).get("rk_variant", 0), args.dtype) +
                       " (fused 9-field RK stage)",
             "achieved": rk_achieved / 1e9 if rk else None,
             "peak": PEAK_HBM / 1e9,

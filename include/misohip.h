@@ -281,8 +281,9 @@ typedef struct misohip_tiles {
                      point, 2 = as 1 held to 4 waves per SIMD, 3 = x-y
                      neighbourhood staged in LDS, 4 = z-march (a block walks
                      a 64 x 4 column through rk_zchunk planes, z neighbours
-                     in registers, x-y neighbourhood in LDS) */
-  int rk_zchunk;  /* planes per block of the z-march kernel, 1..4096 */
+                     in registers and LDS, x-y neighbourhood in LDS), 5 = the
+                     z-march over 64 x 8 columns (512 threads) */
+  int rk_zchunk;  /* planes per block of the z-march kernels, 1..4096 */
 } misohip_tiles;
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t);
 int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t);

@@ -28,7 +28,7 @@ from . import hip
 
 RK_PENCILS = ((1, 4), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (4, 16), (8, 4), (8, 8))
 CHUNKS = (16, 32, 64, 128, 256)
-ZCHUNKS = (8, 16, 32, 64, 128)
+ZCHUNKS = (16, 32, 64, 128, 256, 512)
 
 
 def _device_key() -> str:
@@ -115,9 +115,9 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
 
     run()  # warm-up (code objects, caches)
     rk_times = {}
-    variants = (0, 1, 2, 3, 4) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
+    variants = (0, 1, 2, 3, 4, 5) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
     for var in variants:
-        if var == 4:  # the z-march kernel has no pencils; its knob is the z chunk
+        if var in (4, 5):  # the z-march kernels have no pencils; their knob is the z chunk
             cands = [(1, 1, z) for z in zchunks]
         elif var in (0, 3):
             cands = [(bx, by, 32) for bx, by in pencils]
@@ -139,7 +139,7 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
     torch.cuda.synchronize()
     out = {"rk_pencil": [bx, by], "rk_variant": int(var), "rk_zchunk": int(zc),
            "sweep_chunk": int(best_c),
-           "times_ms": {"rk_stage": {(f"z{z}v{v}" if v == 4 else f"{a}x{b}v{v}"): t
+           "times_ms": {"rk_stage": {(f"z{z}v{v}" if v in (4, 5) else f"{a}x{b}v{v}"): t
                                      for (a, b, v, z), t in rk_times.items()},
                         "av_yz": {str(k): v for k, v in sweep_times.items()}}}
     cache.put(key, out)

@@ -194,22 +194,31 @@ template <typename R> struct AvCons {
   R vx, vy, vz, ei;
 };
 
+// conserved values of a cell from its nine fields (ro vx vy vz bx by bz ei ph)
 template <typename R>
-__device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R> &a) {
-  const R ro = q.q[0][c];
-  a.vx = q.q[1][c];
-  a.vy = q.q[2][c];
-  a.vz = q.q[3][c];
-  a.ei = q.q[7][c];
+__device__ __forceinline__ void cons_of(const R f[9], AvCons<R> &a) {
+  const R ro = f[0];
+  a.vx = f[1];
+  a.vy = f[2];
+  a.vz = f[3];
+  a.ei = f[7];
   a.v[0] = ro;
   a.v[1] = ro * a.vx;
   a.v[2] = ro * a.vy;
   a.v[3] = ro * a.vz;
-  a.v[4] = q.q[4][c];
-  a.v[5] = q.q[5][c];
-  a.v[6] = q.q[6][c];
-  a.v[7] = q.q[8][c];
+  a.v[4] = f[4];
+  a.v[5] = f[5];
+  a.v[6] = f[6];
+  a.v[7] = f[8];
   a.v[8] = ro * a.ei;
+}
+
+template <typename R>
+__device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R> &a) {
+  R f[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f[m] = q.q[m][c];
+  cons_of(f, a);
 }
 
 // conservative update of one cell from its two interface fluxes
@@ -324,10 +333,20 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
   // shift so that a0 = p0, a1 = p0+1
   a0 = a1;
   a1 = a2;
+  // the fields and speed of row p+2 arrive one step ahead of their use
+  R nf[9], ncc;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) nf[m] = qq.q[m][c + 2 * s];
+  ncc = cc[c + 2 * s];
   for (int t = 0; t < len; ++t) {
     AvCons<R> an;
-    load_cons(qq, c + 2 * s, an);
-    const R cc_2 = cc[c + 2 * s];
+    cons_of(nf, an);
+    const R cc_2 = ncc;
+    if (t + 1 < len) {  // row p+3 is inside the array when p+1 is an output cell
+#pragma unroll
+      for (int m = 0; m < 9; ++m) nf[m] = qq.q[m][c + 3 * s];
+      ncc = cc[c + 3 * s];
+    }
     const R cup = R(0.5) * (cc_0 + cc_1);
     R fup[9];
 #pragma unroll

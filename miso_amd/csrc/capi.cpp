@@ -301,7 +301,7 @@ int misohip_create(int device, void *stream, misohip_ctx **out) {
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t) {
   if (!ctx || !t) return MISOHIP_E_ARG;
   if (t->rk_pencil_bx < 1 || t->rk_pencil_by < 1 || t->sweep_chunk < 4 || t->sweep_chunk > 4096 ||
-      t->rk_variant < 0 || t->rk_variant > 4 || t->rk_zchunk < 1 || t->rk_zchunk > 4096)
+      t->rk_variant < 0 || t->rk_variant > 5 || t->rk_zchunk < 1 || t->rk_zchunk > 4096)
     return MISOHIP_E_ARG;
   ctx->tile = *t;
   return MISOHIP_OK;

@@ -13,7 +13,7 @@ struct RkScalars {
   int src_kind, grav_axis;
   int pencil_bx, pencil_by;  // pencil shape in blocks of 64 x 4 cells
   int variant;  // 3D kernel: 0 three load phases, 1 one per point, 2 = 1 at 4 waves/SIMD, 3 LDS tile,
-                // 4 z-march
+                // 4 z-march over 64 x 4 columns, 5 over 64 x 8
   int zchunk;   // planes walked per block by the z-march kernel
 };
 

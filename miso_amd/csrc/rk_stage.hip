@@ -80,9 +80,9 @@ template <typename R> struct GlobalAcc {
   __device__ __forceinline__ R operator()(int m, int i) const { return a.q[m][i]; }
 };
 constexpr int RK_TX = 68, RK_TY = 8, RK_TILE = RK_TX * RK_TY;  // 64 x 4 cells + 2-cell halo
-template <typename R> struct TileAcc {
+template <typename R, int TSIZE = RK_TILE> struct TileAcc {
   const R *t;
-  __device__ __forceinline__ R operator()(int m, int i) const { return t[m * RK_TILE + i]; }
+  __device__ __forceinline__ R operator()(int m, int i) const { return t[m * TSIZE + i]; }
 };
 
 template <typename R, int D, int SPLIT, typename Acc>
@@ -327,24 +327,25 @@ __global__ __launch_bounds__(256) void rk_stage_kernel_tile(Fld<const R> orgn, F
 }
 
 // ------------------------------------------------------------ z-march (3D)
-// A block owns one 64 x 4 column of cells and walks it up through `kz`
-// planes.  The z stencil reads no global memory: each thread keeps its own
-// cell's fields at planes k-2 and k-1 in registers and at planes k+1 and k+2
-// in two thread-private LDS slots (FP64 has no room for all four in
-// registers at two waves per SIMD); plane k is the centre of the 68 x 8 x-y
-// tile in LDS, which is rebuilt for every plane from the slot of plane k+1
-// and from halo loads issued during the previous plane's update; plane k+3
-// is loaded while plane k is computed.  Global loads per cell: 9 (new plane)
-// + 9 * 288/256 (halo, L2 hits when the neighbour columns run on the same
-// XCD) + 9 (orgn), against 64 for the one-plane-per-block kernels.  The
-// arithmetic is rk_block's, term for term.
-constexpr int RK_HALO = RK_TILE - 256;  // 288 halo cells around a 64 x 4 block
+// A block owns one 64 x TY column of cells (TY = 4 or 8 rows, 64*TY
+// threads) and walks it up through `kz` planes.  The z stencil reads no
+// global memory: each thread keeps its own cell's fields at planes k-2 and
+// k-1 in registers and at planes k+1 and k+2 in two thread-private LDS slots
+// (FP64 has no room for all four in registers at two waves per SIMD); plane
+// k is the centre of the 68 x (TY+4) x-y tile in LDS, which is rebuilt for
+// every plane from the slot of plane k+1 and from halo loads issued during
+// the previous plane's update; plane k+3 is loaded while plane k is
+// computed.  Global loads per cell: 9 (new plane) + 9 * halo/(64 TY) (halo:
+// 288 cells for TY = 4, 304 for TY = 8) + 9 (orgn), against 64 for the
+// one-plane-per-block kernels.  The arithmetic is rk_block's, term for term.
 
-// Tile position of halo slot s (0..287): rows 0-1, rows 6-7, then columns
-// 0, 1, 66, 67 of rows 2-5.
+// Tile position of halo slot s of a 64 x TY block's 68 x (TY+4) tile: the
+// two rows below, the two rows above, then columns 0, 1, 66, 67 of the TY
+// centre rows.
+template <int TY>
 __device__ __forceinline__ int rk_halo_pos(int s) {
   if (s < 2 * RK_TX) return s;
-  if (s < 4 * RK_TX) return 6 * RK_TX + (s - 2 * RK_TX);
+  if (s < 4 * RK_TX) return (TY + 2) * RK_TX + (s - 2 * RK_TX);
   const int q = s - 4 * RK_TX, col = q & 3;
   return (2 + (q >> 2)) * RK_TX + (col < 2 ? col : 64 + col);
 }
@@ -368,17 +369,21 @@ __device__ __forceinline__ void derivs_pts(const R (&p2)[9], const R (&p1)[9], c
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+template <typename R, int TY>
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
                       const R *__restrict__ dxi, const R *__restrict__ dyi,
                       const R *__restrict__ dzi, Shape sh, RkScalars sc, March mc) {
-  __shared__ R lds[9 * RK_TILE];
-  __shared__ R slot[2 * 9 * 256];  // [plane parity][field][thread]
+  constexpr int NT = 64 * TY;                 // threads
+  constexpr int TS = RK_TX * (TY + 4);        // tile cells
+  constexpr int NH = TS - NT;                 // halo cells
+  constexpr bool TWO = NH > NT;               // some threads own two halo cells
+  __shared__ R lds[9 * TS];
+  __shared__ R slot[2 * 9 * NT];  // [plane parity][field][thread]
   const int V = xcd_virtual_block(blockIdx.x, mc.nb);
   const int bx = V % mc.nbx, rr = V / mc.nbx, by = rr % mc.nby, zc = rr / mc.nby;
   const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-  const int i = bx * 64 + tx + sh.mi, j = by * 4 + ty + sh.mj;
+  const int i = bx * 64 + tx + sh.mi, j = by * TY + ty + sh.mj;
   const int kb = zc * mc.kz + sh.mk;
   const int ke = min(kb + mc.kz, sh.Kt - sh.mk);
   const bool valid = i < sh.It - sh.mi && j < sh.Jt - sh.mj;
@@ -386,9 +391,10 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   // in-plane offsets; a position outside the array reads cell (0, 0) of the
   // plane instead, whose value no valid cell uses
   const int pc = (i < sh.It && j < sh.Jt) ? i + sh.It * j : 0;
-  const int i0 = bx * 64 + sh.mi - 2, j0 = by * 4 + sh.mj - 2;
-  const bool has1 = tid + 256 < RK_HALO;
-  const int p0 = rk_halo_pos(tid), p1 = rk_halo_pos(has1 ? tid + 256 : 0);
+  const int i0 = bx * 64 + sh.mi - 2, j0 = by * TY + sh.mj - 2;
+  const bool has0 = tid < NH;
+  const bool has1 = TWO && tid + NT < NH;
+  const int p0 = rk_halo_pos<TY>(has0 ? tid : 0), p1 = rk_halo_pos<TY>(has1 ? tid + NT : 0);
   const int g0i = i0 + p0 % RK_TX, g0j = j0 + p0 / RK_TX;
   const int g1i = i0 + p1 % RK_TX, g1j = j0 + p1 / RK_TX;
   const int h0 = (g0i < sh.It && g0j < sh.Jt) ? g0i + sh.It * g0j : 0;
@@ -398,7 +404,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   const R pii8 = Const<R>::pii8;
   const R ddx = dxi[valid ? i : sh.mi], ddy = dyi[valid ? j : sh.mj];
   const R dt = R(sc.dt), ch2 = R(sc.ch2), damp = R(sc.damp);
-  auto sl = [&](int k, int m) -> R & { return slot[((k & 1) * 9 + m) * 256 + tid]; };
+  auto sl = [&](int k, int m) -> R & { return slot[((k & 1) * 9 + m) * NT + tid]; };
 
   R zm2[9], zm1[9];  // own cell at planes k-2, k-1
   R zn[9];           // own cell at plane k+2, stored to its slot before the z phase
@@ -410,9 +416,9 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
       zm1[m] = argm.q[m][c - plane];
       sl(kb + 1, m) = argm.q[m][c + plane];
       zn[m] = argm.q[m][c + 2 * plane];
-      lds[m * RK_TILE + tc] = argm.q[m][c];
-      lds[m * RK_TILE + p0] = argm.q[m][h + h0];
-      if (has1) lds[m * RK_TILE + p1] = argm.q[m][h + h1];
+      lds[m * TS + tc] = argm.q[m][c];
+      if (has0) lds[m * TS + p0] = argm.q[m][h + h0];
+      if (has1) lds[m * TS + p1] = argm.q[m][h + h1];
     }
   }
   __syncthreads();
@@ -421,7 +427,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     const bool more = k + 1 < ke;  // uniform over the block
     R S[17];
     // ---- x (LDS tile)
-    dir_derivs<R, 0, 1>(TileAcc<R>{lds}, tc, 1, ddx, gm1, S);
+    dir_derivs<R, 0, 1>(TileAcc<R, TS>{lds}, tc, 1, ddx, gm1, S);
     R a_ro = -S[0];
     R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
     R a_vy = -S[2], t_vy = S[7];
@@ -436,7 +442,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
                  "+v"(a_vy), "+v"(t_vy), "+v"(a_vz), "+v"(t_vz), "+v"(a_by), "+v"(k_by),
                  "+v"(a_bz), "+v"(k_bz), "+v"(ph_bx), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
     // ---- y (LDS tile)
-    dir_derivs<R, 1, 1>(TileAcc<R>{lds}, cy, RK_TX, ddy, gm1, S);
+    dir_derivs<R, 1, 1>(TileAcc<R, TS>{lds}, cy, RK_TX, ddy, gm1, S);
     a_ro = a_ro - S[0];
     a_vx = a_vx - S[1];
     t_vx = t_vx + S[6];
@@ -485,13 +491,13 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
       const int h = (k + 1) * plane;
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
-        ha[m] = argm.q[m][h + h0];
+        ha[m] = has0 ? argm.q[m][h + h0] : R(0);
         hb[m] = has1 ? argm.q[m][h + h1] : R(0);
       }
     }
     R ctr[9];  // plane k's own values: the tile centre
 #pragma unroll
-    for (int m = 0; m < 9; ++m) ctr[m] = lds[m * RK_TILE + tc];
+    for (int m = 0; m < 9; ++m) ctr[m] = lds[m * TS + tc];
     if (valid)
       rk_update<R>(orgn, rslt, pc + k * plane, ctr[0], sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x,
                    t_vx, a_vy, d4y, d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph,
@@ -506,9 +512,9 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     if (more) {
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
-        lds[m * RK_TILE + tc] = sl(k + 1, m);
-        lds[m * RK_TILE + p0] = ha[m];
-        if (has1) lds[m * RK_TILE + p1] = hb[m];
+        lds[m * TS + tc] = sl(k + 1, m);
+        if (has0) lds[m * TS + p0] = ha[m];
+        if (has1) lds[m * TS + p1] = hb[m];
         zm2[m] = zm1[m];
         zm1[m] = ctr[m];
       }
@@ -522,12 +528,18 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
                               const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
                               const Shape &sh, const RkScalars &sc) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
-  if (HX && HY && HZ && sc.variant == 4) {
+  if (HX && HY && HZ && (sc.variant == 4 || sc.variant == 5)) {
+    const int ty = sc.variant == 4 ? 4 : 8;
     March mc;
-    if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 || !mc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.zchunk))
+    if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 ||
+        !mc.init((ni + 63) / 64, (nj + ty - 1) / ty, nk, sc.zchunk))
       return hipErrorInvalidValue;
-    hipLaunchKernelGGL((rk_stage_kernel_march<R>), dim3(mc.nb), dim3(256), 0, st, o, a, r, dxi,
-                       dyi, dzi, sh, sc, mc);
+    if (ty == 4)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4>), dim3(mc.nb), dim3(256), 0, st, o, a, r,
+                         dxi, dyi, dzi, sh, sc, mc);
+    else
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8>), dim3(mc.nb), dim3(512), 0, st, o, a, r,
+                         dxi, dyi, dzi, sh, sc, mc);
     return hipGetLastError();
   }
   Pencils pc;

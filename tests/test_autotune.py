@@ -78,7 +78,7 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
         outs = []
         for tiles in ((2, 8, 64, 0), (1, 4, 16, 1), (4, 16, 128, 2), (2, 4, 32, 2), (4, 8, 64, 3),
                       (1, 4, 64, 3), (1, 1, 64, 4, 1), (1, 1, 64, 4, 3), (1, 1, 32, 4, 8),
-                      (1, 1, 64, 4, 64)):
+                      (1, 1, 64, 4, 64), (1, 1, 64, 5, 1), (1, 1, 64, 5, 7), (1, 1, 64, 5, 32)):
             m.qq.data.copy_(start)
             set_tiles(ctx, *tiles)
             m.integrator.step(bc, m.qq)

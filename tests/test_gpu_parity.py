@@ -294,18 +294,18 @@ def test_run_fp64(ctx, case):
 
 @pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
                                   "stratified_gravity_16"])
-@pytest.mark.parametrize("zchunk", [5, 32])
-def test_step_fp64_z_march(ctx, case, zchunk):
-    """The z-march RK kernel (variant 4) against the oracle, bit for bit."""
+@pytest.mark.parametrize("variant,zchunk", [(4, 5), (4, 32), (5, 5), (5, 32)])
+def test_step_fp64_z_march(ctx, case, variant, zchunk):
+    """The z-march RK kernels (variants 4 and 5) against the oracle, bit for bit."""
     from miso_amd.autotune import set_tiles
     problem, sizes, n, kw = STEP_CASES[case]
-    set_tiles(ctx, 1, 1, 64, 4, zchunk)
+    set_tiles(ctx, 1, 1, 64, variant, zchunk)
     try:
         out, ref, dts, ref_dts = _run_both(ctx, problem, sizes, n, use_run=True, **kw)
     finally:
         set_tiles(ctx, 2, 8, 64, 0)
     assert np.array_equal(dts, ref_dts), (dts, ref_dts)
-    assert_bitwise(out, ref, f"z-march {case} zchunk {zchunk}")
+    assert_bitwise(out, ref, f"z-march {case} variant {variant} zchunk {zchunk}")
 
 
 @pytest.mark.parametrize("sizes", [(96, 1, 1), (1, 80, 1), (1, 1, 72), (40, 1, 24)])
