@@ -88,12 +88,13 @@ def parse():
 
 def rk_kernel_name(variant, dtype="f64"):
     """Name (as the profiles list it) of the RK stage kernel a 3D run
-    launches for a tile variant."""
+    launches for a tile variant; for the z-march variants the prefix shared
+    by the first-stage and later-stage instances."""
     t = "double" if dtype == "f64" else "float"
     return {0: f"rk_stage_kernel<{t}, true, true, true, 0>",
             1: f"rk_stage_kernel<{t}, true, true, true, 1>",
             2: f"rk_stage_kernel_w4<{t}, true, true, true>", 3: f"rk_stage_kernel_tile<{t}>",
-            4: f"rk_stage_kernel_march<{t}, 4>", 5: f"rk_stage_kernel_march<{t}, 8>"}[variant]
+            4: f"rk_stage_kernel_march<{t}, 4", 5: f"rk_stage_kernel_march<{t}, 8"}[variant]
 
 
 def rk_traffic(n, dtype, variant=0):
@@ -105,12 +106,16 @@ def rk_traffic(n, dtype, variant=0):
                         f"mhd_step_turbulence_{n}cubed_{dtype}.md")
     if not os.path.exists(path):
         return None
-    name = "| `" + rk_kernel_name(variant, dtype) + "`"
+    name = "| `" + rk_kernel_name(variant, dtype)
+    tot, cnt = 0.0, 0
     for line in open(path):
         cols = line.split("|")
         if line.startswith(name) and len(cols) >= 10:  # the traffic table
-            return float(cols[7]) * 1e6
-    return None
+            # instances of one variant (the z-march first stage has its own)
+            # are averaged over their dispatches
+            tot += float(cols[7]) * 1e6 * int(cols[2])
+            cnt += int(cols[2])
+    return tot / cnt if cnt else None
 
 
 def calibrate_copy(nbytes=2 << 30, reps=10):
@@ -620,6 +625,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": rk_kernel_name((tiles or {}).get("rk_variant", 0), args.dtype) +
+                      (", *>" if (tiles or {}).get("rk_variant", 0) in (4, 5) else "") +
                       " (fused 9-field RK stage)",
             "achieved": rk_achieved / 1e9 if rk else None,
             "peak": PEAK_HBM / 1e9,

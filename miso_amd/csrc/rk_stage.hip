@@ -125,19 +125,19 @@ __device__ __forceinline__ void derivs_or_zero(const Acc &a, int c, int s, R dd,
 }
 
 // The stage update of one cell from its direction sums (the tail of
-// update_sc4, mhd_integrator_host.hpp:113-243); gro is argm's density at the
-// cell (gravity source).
+// update_sc4, mhd_integrator_host.hpp:113-243); o: orgn's nine fields at the
+// cell; gro: argm's density there (gravity source).
 template <typename R>
-__device__ __forceinline__ void rk_update(const Fld<const R> &orgn, const Fld<R> &rslt, int c,
-                                          R gro, const RkScalars &sc, const R dt, const R ch2,
+__device__ __forceinline__ void rk_update(const R o[9], const Fld<R> &rslt, int c, R gro,
+                                          const RkScalars &sc, const R dt, const R ch2,
                                           const R damp, R a_ro, R a_vx, R d4x, R d5x, R t_vx,
                                           R a_vy, R d4y, R d5y, R t_vy, R a_vz, R d4z, R d5z,
                                           R t_vz, R a_bx, R a_by, R a_bz, R a_ph, R a_ei,
                                           R t_ei) {
   const R pii4 = Const<R>::pii4, pii8 = Const<R>::pii8;
-  const R o_ro = orgn.q[0][c], o_vx = orgn.q[1][c], o_vy = orgn.q[2][c], o_vz = orgn.q[3][c];
-  const R o_bx = orgn.q[4][c], o_by = orgn.q[5][c], o_bz = orgn.q[6][c];
-  const R o_ei = orgn.q[7][c], o_ph = orgn.q[8][c];
+  const R o_ro = o[0], o_vx = o[1], o_vy = o[2], o_vz = o[3];
+  const R o_bx = o[4], o_by = o[5], o_bz = o[6];
+  const R o_ei = o[7], o_ph = o[8];
 
   // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134)
   R src[3] = {R(0), R(0), R(0)};
@@ -283,8 +283,11 @@ __device__ __forceinline__ void rk_block(const Fld<const R> &orgn, const Fld<con
   a_ei = a_ei - S[15];
   t_ei = t_ei + S[16];
 
+  R o[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) o[m] = orgn.q[m][c];
   const R gro = sc.src_kind == 1 ? argm.q[0][c] : R(0);
-  rk_update<R>(orgn, rslt, c, gro, sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x, t_vx, a_vy, d4y_,
+  rk_update<R>(o, rslt, c, gro, sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x, t_vx, a_vy, d4y_,
                d5y_, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph, a_ei, t_ei);
 }
 
@@ -369,7 +372,8 @@ __device__ __forceinline__ void derivs_pts(const R (&p2)[9], const R (&p1)[9], c
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
 }
 
-template <typename R, int TY>
+// SAME: orgn is argm (the first stage); its values are the tile centre.
+template <typename R, int TY, bool SAME>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
                       const R *__restrict__ dxi, const R *__restrict__ dyi,
@@ -498,10 +502,14 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     R ctr[9];  // plane k's own values: the tile centre
 #pragma unroll
     for (int m = 0; m < 9; ++m) ctr[m] = lds[m * TS + tc];
-    if (valid)
-      rk_update<R>(orgn, rslt, pc + k * plane, ctr[0], sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x,
+    if (valid) {
+      R o[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) o[m] = SAME ? ctr[m] : orgn.q[m][pc + k * plane];
+      rk_update<R>(o, rslt, pc + k * plane, ctr[0], sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x,
                    t_vx, a_vy, d4y, d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph,
                    a_ei, t_ei);
+    }
     // plane k+3, used in the next plane's z phase
     if (more && k + 3 < sh.Kt) {
       const int cn = pc + (k + 3) * plane;
@@ -534,12 +542,20 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
     if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 ||
         !mc.init((ni + 63) / 64, (nj + ty - 1) / ty, nk, sc.zchunk))
       return hipErrorInvalidValue;
-    if (ty == 4)
-      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4>), dim3(mc.nb), dim3(256), 0, st, o, a, r,
-                         dxi, dyi, dzi, sh, sc, mc);
+    bool same = true;
+    for (int m = 0; m < 9; ++m) same = same && o.q[m] == a.q[m];
+    if (ty == 4 && same)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, true>), dim3(mc.nb), dim3(256), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
+    else if (ty == 4)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, false>), dim3(mc.nb), dim3(256), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
+    else if (same)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, true>), dim3(mc.nb), dim3(512), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
     else
-      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8>), dim3(mc.nb), dim3(512), 0, st, o, a, r,
-                         dxi, dyi, dzi, sh, sc, mc);
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, false>), dim3(mc.nb), dim3(512), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
     return hipGetLastError();
   }
   Pencils pc;
