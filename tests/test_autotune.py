@@ -19,7 +19,9 @@ def test_tile_cache_round_trip(tmp_path):
 def test_tune_single_keeps_results(ctx, tmp_path):
     import torch
     from miso_amd import mhd, problems
-    from miso_amd.autotune import set_tiles, tune_single
+    from miso_amd.autotune import get_tiles, set_tiles, tune_single
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk)
     g = mhd.Grid(48, 40, 36, 2, device="cuda:0")
     bc = mhd.BoundaryCondition((True, True, True))
     cfg = mhd.Config({"mhd": {"cfl_number": 0.4}})
@@ -38,6 +40,7 @@ def test_tune_single_keeps_results(ctx, tmp_path):
     assert torch.equal(m.qq.data, tuned)
     # a second call hits the cache
     assert tune_single(ctx, m, bc, cache=TileCache(str(tmp_path / "c.json")))["cached"]
+    set_tiles(ctx, *default)
 
 
 @pytest.mark.gpu
@@ -67,7 +70,9 @@ def test_tune_ensemble_keeps_results(ctx, tmp_path):
 def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
     import torch
     from miso_amd import mhd, problems
-    from miso_amd.autotune import set_tiles
+    from miso_amd.autotune import get_tiles, set_tiles
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk)
     for sizes in ((70, 36, 20), (66, 37, 13)):  # partial x and y tiles, uneven z chunks
         g = mhd.Grid(*sizes, 2, device="cuda:0")
         bc = mhd.BoundaryCondition((True, True, True))
@@ -83,6 +88,6 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
             set_tiles(ctx, *tiles)
             m.integrator.step(bc, m.qq)
             outs.append(m.qq.data.clone())
-        set_tiles(ctx, 2, 8, 64, 0)
+        set_tiles(ctx, *default)
         for n, o in enumerate(outs[1:], 1):
             assert torch.equal(o, outs[0]), (sizes, n)

@@ -297,13 +297,15 @@ def test_run_fp64(ctx, case):
 @pytest.mark.parametrize("variant,zchunk", [(4, 5), (4, 32), (5, 5), (5, 32)])
 def test_step_fp64_z_march(ctx, case, variant, zchunk):
     """The z-march RK kernels (variants 4 and 5) against the oracle, bit for bit."""
-    from miso_amd.autotune import set_tiles
+    from miso_amd.autotune import get_tiles, set_tiles
     problem, sizes, n, kw = STEP_CASES[case]
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk)
     set_tiles(ctx, 1, 1, 64, variant, zchunk)
     try:
         out, ref, dts, ref_dts = _run_both(ctx, problem, sizes, n, use_run=True, **kw)
     finally:
-        set_tiles(ctx, 2, 8, 64, 0)
+        set_tiles(ctx, *default)
     assert np.array_equal(dts, ref_dts), (dts, ref_dts)
     assert_bitwise(out, ref, f"z-march {case} variant {variant} zchunk {zchunk}")
 
