@@ -354,6 +354,18 @@ template <typename Real> class Integrator {
     return Real(dt);
   }
 
+  /// nsteps of step() in one call (the loop of ModelBase::run,
+  /// mhd_model_base.hpp:89-117); the dt of every step, the same values.
+  template <typename EOS, typename BC, typename Source>
+  std::vector<double> run(const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq,
+                          int nsteps) {
+    misohip_params p = params(eos, src);
+    std::vector<double> dts(nsteps > 0 ? nsteps : 0);
+    MISO_HIP_CHECK(misohip_run(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
+                               qq.desc(), &w_, nsteps, dts.data()));
+    return dts;
+  }
+
  private:
   template <typename EOS, typename Source> misohip_params params(const EOS &eos, const Source &src) {
     misohip_params p{};
@@ -394,6 +406,11 @@ template <typename Real> struct MHD {
   template <typename EOS, typename BC, typename Source>
   void update(Real dt, const EOS &eos, const BC &bc, const Source &src) {
     integrator.update(dt, eos, bc, src, qq);
+  }
+  /// nsteps of cfl + update (ModelBase::run's loop); the dt of every step
+  template <typename EOS, typename BC, typename Source>
+  std::vector<double> run(const EOS &eos, const BC &bc, const Source &src, int nsteps) {
+    return integrator.run(eos, bc, src, qq, nsteps);
   }
 };
 

@@ -37,6 +37,8 @@
  *                                 (miso/include/miso/mhd_integrator_host.hpp:337-350)
  *   misohip_step                  ModelBase::update = cfl + update
  *                                 (miso/include/miso/mhd_model_base.hpp:57-62)
+ *   misohip_run                   ModelBase::run's stepping loop
+ *                                 (miso/include/miso/mhd_model_base.hpp:89-117)
  */
 #ifndef MISOHIP_H
 #define MISOHIP_H

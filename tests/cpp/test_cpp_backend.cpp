@@ -216,17 +216,16 @@ int cpp_run_steps(int fp64, int ni, int nj, int nk, const double *bounds, const 
       bc.fixed_values(fixed);
     }
     mhd.apply_initial_condition(ic, bc);
-    for (int s = 0; s < nsteps; ++s) {
-      R dt;
-      if (src_kind == 1) {
+    if (src_kind == 1) {  // step by step: cfl, then update
+      for (int s = 0; s < nsteps; ++s) {
         hip::Gravity g{grav_axis, grav};
-        dt = mhd.cfl(eos);
+        const R dt = mhd.cfl(eos);
         mhd.update(dt, eos, bc, g);
-      } else {
-        dt = mhd.cfl(eos);
-        mhd.update(dt, eos, bc, hip::NoSource{});
+        dts[s] = double(dt);
       }
-      dts[s] = double(dt);
+    } else {  // the whole loop in one call
+      const std::vector<double> d = mhd.run(eos, bc, hip::NoSource{}, nsteps);
+      for (int s = 0; s < nsteps; ++s) dts[s] = d[s];
     }
     hip::HostFields<R> out(hg.i_total(), hg.j_total(), hg.k_total());
     mhd.qq.copy_to(ctx, out);
