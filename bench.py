@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--tiles", default=None,
-                    help="bx,by,chunk,variant[,zchunk]: set the 3D tile shapes instead of tuning")
+                    help="bx,by,chunk,variant[,zchunk[,fuse_xy[,xy_chunk]]]: set the 3D tile "
+                         "shapes instead of tuning")
     ap.add_argument("--no-rmhd", action="store_true", help="skip the radiative-MHD leg")
     ap.add_argument("--rmhd-only", action="store_true", help="run only the radiative-MHD leg")
     ap.add_argument("--rmhd-n", type=int, default=384, help="learned-RT box, cells per side")
@@ -553,11 +554,13 @@ def main():
     tiles = None
     if args.tiles:
         from miso_amd.autotune import set_tiles
-        bx, by, ch, var, *zc = (int(x) for x in args.tiles.split(","))
-        zc = zc[0] if zc else 32
-        set_tiles(ctx, bx, by, ch, var, zc)
+        bx, by, ch, var, *rest = (int(x) for x in args.tiles.split(","))
+        zc = rest[0] if rest else 32
+        fz = rest[1] if len(rest) > 1 else 0
+        xc = rest[2] if len(rest) > 2 else 64
+        set_tiles(ctx, bx, by, ch, var, zc, fz, xc)
         tiles = {"rk_pencil": [bx, by], "sweep_chunk": ch, "rk_variant": var, "rk_zchunk": zc,
-                 "given": True}
+                 "av_fuse_xy": fz, "av_xy_chunk": xc, "given": True}
     elif not args.no_tune:
         from miso_amd.autotune import summary, tune_single
         tiles = summary(tune_single(ctx, m, bc, steps=1))

@@ -286,6 +286,10 @@ typedef struct misohip_tiles {
                      in registers and LDS, x-y neighbourhood in LDS), 5 = the
                      z-march over 64 x 8 columns (512 threads) */
   int rk_zchunk;  /* planes per block of the z-march kernels, 1..4096 */
+  int av_fuse_xy; /* 3D runs: 1 = the x and y AV sweeps as one walk that
+                     x-sweeps every row it takes in (the x sweep's ghost fill
+                     drops out), 0 = separate x and y sweeps.  Same results. */
+  int av_xy_chunk; /* rows walked per wave by that walk, 4..4096 */
 } misohip_tiles;
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t);
 int misohip_get_tiles(misohip_ctx *ctx, misohip_tiles *t);

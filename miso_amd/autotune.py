@@ -6,8 +6,10 @@ expose, by timing the candidates on the real state:
 - single runs (`tune_single`): the RK stage's pencil shape
   (`rk_pencil_bx` x `rk_pencil_by` blocks of 64 x 4 cells walked along z),
   its variant (`rk_variant`, 3D: load phases, register cap, LDS-staged x-y
-  neighbourhood, z-march with its planes per block `rk_zchunk`) and the
-  walk length of the y/z AV sweeps (`sweep_chunk`), timed with the
+  neighbourhood, z-march with its planes per block `rk_zchunk`), the walk
+  length of the y/z AV sweeps (`sweep_chunk`) and, in 3D, whether the x and
+  y sweeps run as one walk (`av_fuse_xy`, with its own `av_xy_chunk`),
+  timed with the
   library's own HIP events (misohip_profile_*) over a few steps;
 - batches (`tune_ensemble`): the y/z sweep walk length per run size, timed
   on a probe batch of that size.
@@ -68,15 +70,17 @@ def get_tiles(ctx):
 
 
 def set_tiles(ctx, pencil_bx: int, pencil_by: int, chunk: int, variant: int = 0,
-              zchunk: int = 32) -> None:
+              zchunk: int = 32, fuse_xy: int = 0, xy_chunk: int = 64) -> None:
     t = hip.MisohipTiles()
     t.rk_pencil_bx, t.rk_pencil_by, t.sweep_chunk = int(pencil_bx), int(pencil_by), int(chunk)
-    t.rk_variant, t.rk_zchunk = int(variant), int(zchunk)
+    t.rk_variant, t.rk_zchunk, t.av_fuse_xy = int(variant), int(zchunk), int(fuse_xy)
+    t.av_xy_chunk = int(xy_chunk)
     hip.call("misohip_set_tiles", ctx.handle, C.byref(t))
 
 
-def _profile(ctx, fn):
-    """Run fn with the library's event timing on; returns {kind: ms per launch}."""
+def _profile(ctx, fn, totals=False):
+    """Run fn with the library's event timing on; returns {kind: ms per
+    launch} (totals: {kind: summed ms})."""
     L = hip.lib()
     nk = len(hip.KIND_NAMES)
     hip.check(L.misohip_profile_enable(ctx.handle, 1))
@@ -85,7 +89,7 @@ def _profile(ctx, fn):
     cnt = (C.c_int * nk)()
     hip.check(L.misohip_profile_read(ctx.handle, ms, cnt, nk))
     hip.check(L.misohip_profile_enable(ctx.handle, 0))
-    return {hip.KIND_NAMES[k]: ms[k] / cnt[k] for k in range(nk) if cnt[k]}
+    return {hip.KIND_NAMES[k]: ms[k] / (1 if totals else cnt[k]) for k in range(nk) if cnt[k]}
 
 
 def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNKS,
@@ -101,7 +105,7 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
     hit = None if force else cache.get(key)
     if hit:
         set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"], hit.get("rk_variant", 0),
-                  hit.get("rk_zchunk", 32))
+                  hit.get("rk_zchunk", 32), hit.get("av_fuse_xy", 0), hit.get("av_xy_chunk", 64))
         return dict(hit, cached=True, seconds=0.0)
     t0 = time.perf_counter()
     saved = model.qq.data.clone()
@@ -128,20 +132,33 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
             rk_times[(bx, by, var, z)] = _profile(ctx, run).get("rk_stage", float("inf"))
     best_p = min(rk_times, key=rk_times.get)
     bx, by, var, zc = best_p
-    sweep_times = {}
-    active = [k for k, s in (("av_y", g.j_size), ("av_z", g.k_size)) if s > 1]
+    # AV: summed time of the sweeps and ghost fills, separate x/y sweeps for
+    # every walk length, then (3D) the one-walk x-y sweeps for every walk
+    # length at the best z walk
+    kinds = ("av_x", "av_y", "av_z", "ghost_fill")
+    sweep_times, z_times = {}, {}
     for c in chunks:
         set_tiles(ctx, bx, by, c, var, zc)
-        t = _profile(ctx, run)
-        sweep_times[c] = sum(t.get(k, 0.0) for k in active)
-    best_c = min(sweep_times, key=sweep_times.get)
-    set_tiles(ctx, bx, by, best_c, var, zc)
+        t = _profile(ctx, run, totals=True)
+        sweep_times[(0, c, 0)] = sum(t.get(k, 0.0) for k in kinds)
+        z_times[c] = t.get("av_z", 0.0)
+    if min(g.i_size, g.j_size, g.k_size) > 1:
+        cz = min(z_times, key=z_times.get)
+        for xc in chunks:
+            set_tiles(ctx, bx, by, cz, var, zc, 1, xc)
+            t = _profile(ctx, run, totals=True)
+            sweep_times[(1, cz, xc)] = sum(t.get(k, 0.0) for k in kinds)
+    fz, best_c, xc = min(sweep_times, key=sweep_times.get)
+    xc = xc or 64
+    set_tiles(ctx, bx, by, best_c, var, zc, fz, xc)
     torch.cuda.synchronize()
     out = {"rk_pencil": [bx, by], "rk_variant": int(var), "rk_zchunk": int(zc),
-           "sweep_chunk": int(best_c),
+           "sweep_chunk": int(best_c), "av_fuse_xy": int(fz), "av_xy_chunk": int(xc),
            "times_ms": {"rk_stage": {(f"z{z}v{v}" if v in (4, 5) else f"{a}x{b}v{v}"): t
                                      for (a, b, v, z), t in rk_times.items()},
-                        "av_yz": {str(k): v for k, v in sweep_times.items()}}}
+                        "av_and_ghost_per_run": {
+                            (f"xy{x}z{c}" if f else f"sep{c}"): v
+                            for (f, c, x), v in sweep_times.items()}}}
     cache.put(key, out)
     return dict(out, cached=False, seconds=time.perf_counter() - t0)
 

@@ -15,6 +15,16 @@ namespace misohip {
 // cc = cs*cs_fac + vv*vv_fac + ca*ca_fac with cs = sqrt(gm*(gm-1)*ei)
 // (eos.hpp:35-43, mhd_artificial_viscosity_host.hpp:57-64)
 template <typename R>
+__device__ __forceinline__ R char_speed_v(const R f[9], R gm, R cs_fac, R ca_fac, R vv_fac) {
+  const R ro = f[0], vx = f[1], vy = f[2], vz = f[3];
+  const R bx = f[4], by = f[5], bz = f[6], ei = f[7];
+  const R cs = dsqrt(gm * (gm - R(1)) * ei);
+  const R vv = dsqrt(vx * vx + vy * vy + vz * vz);
+  const R ca = dsqrt((bx * bx + by * by + bz * bz) / ro * Const<R>::pii4);
+  return cs * cs_fac + vv * vv_fac + ca * ca_fac;
+}
+
+template <typename R>
 __device__ __forceinline__ R char_speed(const Fld<const R> &q, int c, R gm, R cs_fac, R ca_fac,
                                         R vv_fac) {
   const R ro = q.q[0][c], vx = q.q[1][c], vy = q.q[2][c], vz = q.q[3][c];
@@ -226,6 +236,33 @@ __device__ __forceinline__ void load_cons(const Fld<const R> &q, int c, AvCons<R
 // DEGEN: apply the sweep of a following direction of extent one (2D runs:
 // z) to the result before storing; dz0 is its inverse spacing, cc the cell's
 // frozen speed.
+// the nine updated fields of one cell from its two interface fluxes
+template <typename R>
+__device__ __forceinline__ void av_cell_values(const AvCons<R> &a, const R fdw[9], const R fup[9],
+                                              R dd, R dt, R o[9]) {
+  const R ro_n = a.v[0] - (fup[0] - fdw[0]) * dd * dt;
+  const R vxn = (a.v[1] - (fup[1] - fdw[1]) * dd * dt) / ro_n;
+  const R vyn = (a.v[2] - (fup[2] - fdw[2]) * dd * dt) / ro_n;
+  const R vzn = (a.v[3] - (fup[3] - fdw[3]) * dd * dt) / ro_n;
+  const R bxn = a.v[4] - (fup[4] - fdw[4]) * dd * dt;
+  const R byn = a.v[5] - (fup[5] - fdw[5]) * dd * dt;
+  const R bzn = a.v[6] - (fup[6] - fdw[6]) * dd * dt;
+  const R phn = a.v[7] - (fup[7] - fdw[7]) * dd * dt;
+  const double pii8 = double(Const<R>::pii8);
+  const double ro = a.v[0];
+  const double Et = ro * double(a.ei) +
+                    0.5 * ro * (double(a.vx) * a.vx + double(a.vy) * a.vy + double(a.vz) * a.vz) +
+                    pii8 * (double(a.v[4]) * a.v[4] + double(a.v[5]) * a.v[5] +
+                            double(a.v[6]) * a.v[6]);
+  const double dro = ro_n, dvx = vxn, dvy = vyn, dvz = vzn, dbx = bxn, dby = byn, dbz = bzn;
+  const double ein = (Et - double((fup[8] - fdw[8]) * dd * dt) -
+                      0.5 * dro * (dvx * dvx + dvy * dvy + dvz * dvz) -
+                      pii8 * (dbx * dbx + dby * dby + dbz * dbz)) /
+                     dro;
+  o[0] = ro_n; o[1] = vxn; o[2] = vyn; o[3] = vzn; o[4] = bxn;
+  o[5] = byn; o[6] = bzn; o[7] = R(ein); o[8] = phn;
+}
+
 template <typename R, bool DEGEN = false>
 __device__ __forceinline__ void av_cell_update(const AvCons<R> &a, const R fdw[9], const R fup[9],
                                               R dd, R dt, const Fld<R> &out, int c, R cc = R(0),
@@ -479,6 +516,154 @@ __global__ __launch_bounds__(256) void char_vel_kernel(Fld<const R> qq, R *__res
   cc[c] = char_speed(qq, c, R(sc.gm), R(sc.cs_fac), R(sc.ca_fac), R(sc.vv_fac));
 }
 
+// ---------------------------------------------------------------------------
+// x and y sweeps in one walk (3D).  A wave owns 64 x lanes (60 outputs, two
+// halo lanes each side, as the x-shuffle kernel) of one plane and walks a
+// chunk of y rows.  Every row entering the y window is x-swept in registers
+// first, with the x-shuffle kernel's arithmetic; a ghost row is what the
+// ghost fill after the x sweep would hold: the x-swept source row for a
+// periodic or mirrored (sign-flipped) boundary, the row itself for a fixed
+// one (its ghost values are the fixed fields both before and after the x
+// sweep).  The frozen speed cc of a row is evaluated from the row's post-RK
+// values (the source row's for a mapped ghost row: the same bits, since cc
+// depends on squares only), and stored for the interior and the z ring,
+// which is all the z sweep reads.  One read pass and one write pass of the
+// separate x sweep, and the ghost fill between the sweeps, drop out.
+
+// the post-RK fields of the row that supplies row jr (the row itself, or
+// the source row of a periodic or mirrored ghost row)
+template <typename R>
+__device__ __forceinline__ void av_xrow_load(const Fld<const R> &qq, const Shape &sh,
+                                             const BcDev &bc, int ic, int jr, int k, R f[9]) {
+  const bool ghost = jr < sh.mj || jr >= sh.Jt - sh.mj;
+  bool mir = false;
+  const int js = ghost ? map_coord(jr, sh.Jt, sh.mj, bc.periodic[1], mir) : jr;
+  const int c = sh.idx(ic, js, k);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) f[m] = qq.q[m][c];
+}
+
+// x-swept fields of row jr (as conserved values) and its frozen speed, from
+// the fields av_xrow_load gave for it
+template <typename R>
+__device__ __forceinline__ void av_xrow_sweep(const R f[9], const Shape &sh, const BcDev &bc,
+                                              int jr, R ddx, const AvScalars &sc, R dt,
+                                              AvCons<R> &out, R &ccv) {
+  const bool ghost = jr < sh.mj || jr >= sh.Jt - sh.mj;
+  const int kind = bc.periodic[1];
+  const bool mir = ghost && kind == 0;
+  ccv = char_speed_v(f, R(sc.gm), R(sc.cs_fac), R(sc.ca_fac), R(sc.vv_fac));
+  if (ghost && kind == 2) {  // uniform per wave
+    cons_of(f, out);
+    return;
+  }
+  AvCons<R> a;
+  cons_of(f, a);
+  const R ep = R(sc.ep), fh = R(sc.fh);
+  const R cup = R(0.5) * (ccv + lane_dn(ccv));
+  R fdw[9], fup[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const R vm = lane_up(a.v[m]), vp = lane_dn(a.v[m]);
+    const R d = dqq_eval(vm, a.v[m], vp, ep);
+    const R dn = lane_dn(d);
+    fup[m] = flux_core(a.v[m], vp, d, dn, cup, fh);
+    fdw[m] = lane_up(fup[m]);
+  }
+  R o[9];
+  av_cell_values(a, fdw, fup, ddx, dt, o);
+  if (mir) {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) o[m] = R(bc.sign[1][m]) * o[m];
+  }
+  cons_of(o, out);
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void av_xy_walk_kernel(Fld<const R> qq, Fld<R> rslt,
+                                                        R *__restrict__ cc_store,
+                                                        const R *__restrict__ dxi,
+                                                        const R *__restrict__ dyi, Shape sh,
+                                                        BcDev bc, int chunk, int nchunks,
+                                                        int nseg, int nwaves, AvScalars sc) {
+  const int w = xcd_virtual_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (w >= nwaves) return;  // whole wave; the kernel has no block barrier
+  const int seg = w % nseg, rr = w / nseg, ch = rr % nchunks, k = sh.mk - 1 + rr / nchunks;
+  const int lane = threadIdx.x & 63;
+  const int i = sh.mi + seg * 60 - 2 + lane;
+  const int ic = min(i, sh.It - 1);
+  const bool out_lane = lane >= 2 && lane < 62 && i < sh.It - sh.mi;
+  const int nj = sh.Jt - 2 * sh.mj;
+  const int p0 = ch * chunk, len = min(chunk, nj - p0);
+  const R gm = R(sc.gm), csf = R(sc.cs_fac), caf = R(sc.ca_fac), vvf = R(sc.vv_fac);
+  if (k < sh.mk || k >= sh.Kt - sh.mk) {  // z ring plane: the frozen speed only
+    for (int t = 0; t < len; ++t) {
+      const int c = sh.idx(ic, sh.mj + p0 + t, k);
+      const R v = char_speed(qq, c, gm, csf, caf, vvf);
+      if (out_lane) cc_store[c] = v;
+    }
+    return;
+  }
+  const R dt = R(sc.dt), ep = R(sc.ep), fh = R(sc.fh);
+  const R ddx = out_lane ? dxi[i] : R(0);
+  const int jb = sh.mj + p0;  // first output row
+  // window: a0 = p, a1 = p+1 (x-swept); slopes at p; flux p-1/2
+  AvCons<R> am, a0, a1, a2;
+  R cc_m, cc_0, cc_1;
+  R nf[9];  // fields of the next row to sweep, loaded one row ahead
+  av_xrow_load(qq, sh, bc, ic, jb - 2, k, nf);
+  av_xrow_sweep(nf, sh, bc, jb - 2, ddx, sc, dt, am, cc_m);
+  av_xrow_load(qq, sh, bc, ic, jb - 1, k, nf);
+  av_xrow_sweep(nf, sh, bc, jb - 1, ddx, sc, dt, a0, cc_m);
+  av_xrow_load(qq, sh, bc, ic, jb, k, nf);
+  av_xrow_sweep(nf, sh, bc, jb, ddx, sc, dt, a1, cc_0);
+  av_xrow_load(qq, sh, bc, ic, jb + 1, k, nf);
+  av_xrow_sweep(nf, sh, bc, jb + 1, ddx, sc, dt, a2, cc_1);
+  av_xrow_load(qq, sh, bc, ic, jb + 2, k, nf);
+  R d0[9], fdw[9];
+  {
+    const R cdw = R(0.5) * (cc_m + cc_0);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+      const R dm = dqq_eval(am.v[f], a0.v[f], a1.v[f], ep);
+      d0[f] = dqq_eval(a0.v[f], a1.v[f], a2.v[f], ep);
+      fdw[f] = flux_core(a0.v[f], a1.v[f], dm, d0[f], cdw, fh);
+    }
+  }
+  a0 = a1;
+  a1 = a2;
+  for (int t = 0; t < len; ++t) {
+    const int j = jb + t;
+    AvCons<R> an;
+    R cc_2;
+    av_xrow_sweep(nf, sh, bc, j + 2, ddx, sc, dt, an, cc_2);
+    // row j+3 exists (at most the last ghost row) when j+1 is an output row
+    if (t + 1 < len) av_xrow_load(qq, sh, bc, ic, j + 3, k, nf);
+    const R cup = R(0.5) * (cc_0 + cc_1);
+    R fup[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) {
+      const R d1 = dqq_eval(a0.v[f], a1.v[f], an.v[f], ep);
+      fup[f] = flux_core(a0.v[f], a1.v[f], d0[f], d1, cup, fh);
+      d0[f] = d1;
+    }
+    const int c = sh.idx(ic, j, k);
+    R o[9];
+    av_cell_values(a0, fdw, fup, dyi[j], dt, o);
+    if (out_lane) {
+#pragma unroll
+      for (int f = 0; f < 9; ++f) rslt.q[f][c] = o[f];
+      cc_store[c] = cc_0;
+    }
+#pragma unroll
+    for (int f = 0; f < 9; ++f) fdw[f] = fup[f];
+    a0 = a1;
+    a1 = an;
+    cc_0 = cc_1;
+    cc_1 = cc_2;
+  }
+}
+
 template <typename R, int DIR>
 static hipError_t launch_av_t(hipStream_t st, const Fld<const R> &q, const Fld<R> &r, R *cc_store,
                               const R *cc_in, const R *dd, const Shape &sh, const AvScalars &sc) {
@@ -587,6 +772,25 @@ hipError_t launch_av_last_sweep_cfl(hipStream_t st, const Fld<const R> &q, const
   if (sh.is && sh.js && !sh.ks) return launch_walk_cfl_t<R, 1, true>(st, q, r, cc_in, dyi, sh, sc,
                                                                       dzi, cf);
   return hipErrorInvalidValue;
+}
+
+// The x and y sweeps of a 3D run in one walk (y extent of at least 2, so
+// every ghost row maps into the interior); cc is written for the z sweep.
+template <typename R>
+hipError_t launch_av_sweep_xy(hipStream_t st, const Fld<const R> &q, const Fld<R> &r,
+                              R *cc_store, const R *dxi, const R *dyi, const Shape &sh,
+                              const BcDev &bc, const AvScalars &sc) {
+  const int nj = sh.Jt - 2 * sh.mj;
+  if (!sh.is || !sh.js || !sh.ks || nj < 2 || sh.mj < 2) return hipErrorInvalidValue;
+  const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
+  const int nchunks = (nj + chunk - 1) / chunk;
+  const int nseg = (sh.It - 2 * sh.mi + 59) / 60;
+  const long nw = (long)nseg * nchunks * (sh.Kt - 2 * sh.mk + 2);
+  const long nbl = (nw + 3) / 4;
+  if (nbl > 2147483647L) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((av_xy_walk_kernel<R>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
+                     cc_store, dxi, dyi, sh, bc, chunk, nchunks, nseg, (int)nw, sc);
+  return hipGetLastError();
 }
 
 template <typename R>
@@ -699,6 +903,14 @@ template hipError_t launch_av_last_sweep_cfl<float>(hipStream_t, const Fld<const
                                                     const Fld<float> &, const float *,
                                                     const float *, const float *, const Shape &,
                                                     const AvScalars &, const CflFuse<float> &);
+template hipError_t launch_av_sweep_xy<double>(hipStream_t, const Fld<const double> &,
+                                               const Fld<double> &, double *, const double *,
+                                               const double *, const Shape &, const BcDev &,
+                                               const AvScalars &);
+template hipError_t launch_av_sweep_xy<float>(hipStream_t, const Fld<const float> &,
+                                              const Fld<float> &, float *, const float *,
+                                              const float *, const Shape &, const BcDev &,
+                                              const AvScalars &);
 template hipError_t launch_char_vel<double>(hipStream_t, const Fld<const double> &, double *, int,
                                             const AvScalars &);
 template hipError_t launch_char_vel<float>(hipStream_t, const Fld<const float> &, float *, int,

@@ -193,14 +193,25 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   // AV: x sweep reads the post-RK state and stores cc; y and z read cc.  In
   // 2D runs the z sweep (extent one, pointwise) is applied by the y sweep to
   // each of its results before they are stored.
+  // In 3D runs the x and y sweeps can run as one walk (tile choice
+  // av_fuse_xy): the y walk x-sweeps every row it takes in.
   const bool fuse_z = g->is && g->js && !g->ks;
+  const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2;
   const misohip_fields *cur = argm;
   const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
   for (int dir = 0; dir < 3; ++dir) {
+    if (dir == 1 && fuse_xy) continue;
     const misohip_fields *dst = (dir == 2 || (dir == 1 && fuse_z)) ? qq : other;
     int rc;
     const bool last = dir == 2 || (dir == 1 && fuse_z);
-    if (last && fuse_cfl) {
+    if (dir == 0 && fuse_xy) {
+      Timed t_(ctx, MISOHIP_KIND_AV_X);
+      MISOHIP_TRY(launch_av_sweep_xy<R>(ctx->stream, cfld<R>(cur), fld<R>(dst),
+                                        static_cast<R *>(w->cc), static_cast<const R *>(g->dxi),
+                                        static_cast<const R *>(g->dyi), sh, b,
+                                        av_scalars<R>(p, dt, ctx->tile.av_xy_chunk)));
+      rc = MISOHIP_OK;
+    } else if (last && fuse_cfl) {
       Timed t_(ctx, fuse_z ? MISOHIP_KIND_AV_Y : MISOHIP_KIND_AV_Z);
       CflFuse<R> cf;
       cf.dx = static_cast<const R *>(g->dx);
@@ -301,7 +312,8 @@ int misohip_create(int device, void *stream, misohip_ctx **out) {
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t) {
   if (!ctx || !t) return MISOHIP_E_ARG;
   if (t->rk_pencil_bx < 1 || t->rk_pencil_by < 1 || t->sweep_chunk < 4 || t->sweep_chunk > 4096 ||
-      t->rk_variant < 0 || t->rk_variant > 5 || t->rk_zchunk < 1 || t->rk_zchunk > 4096)
+      t->rk_variant < 0 || t->rk_variant > 5 || t->rk_zchunk < 1 || t->rk_zchunk > 4096 ||
+      t->av_fuse_xy < 0 || t->av_fuse_xy > 1 || t->av_xy_chunk < 4 || t->av_xy_chunk > 4096)
     return MISOHIP_E_ARG;
   ctx->tile = *t;
   return MISOHIP_OK;

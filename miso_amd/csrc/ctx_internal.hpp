@@ -19,7 +19,7 @@ struct misohip_ctx {
   std::vector<hipEvent_t> pool;
   std::vector<int> rec_kind;
   std::vector<hipEvent_t> rec_start, rec_stop;
-  misohip_tiles tile = {2, 8, 256, 5, 256};
+  misohip_tiles tile = {2, 8, 256, 5, 256, 1, 64};
 };
 
 namespace misohip_detail {

@@ -79,6 +79,25 @@ struct Pencils {
   }
 };
 
+// kind: 0 mirror, 1 periodic, 2 fixed (the ghost cell keeps its coordinate
+// and takes its value from the fixed fields)
+__device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool &mirrored) {
+  mirrored = false;
+  if (m == 0 || kind == 2) return x;
+  const int n = total - 2 * m;
+  if (x < m) {
+    if (kind == 1) return x + n;
+    mirrored = true;
+    return 2 * m - 1 - x;
+  }
+  if (x >= total - m) {
+    if (kind == 1) return x - n;
+    mirrored = true;
+    return 2 * (total - m) - 1 - x;
+  }
+  return x;
+}
+
 // Blocks of the z-march kernel: (nbx, nby) columns of 64 x 4 cells, each cut
 // into nzc chunks of kz planes; x fastest, then y, then the chunk, so that
 // with xcd_virtual_block every XCD walks compact x-y regions of one slab.

@@ -21,24 +21,6 @@
 
 namespace misohip {
 
-// kind: 0 mirror, 1 periodic, 2 fixed (the ghost cell keeps its coordinate
-// and takes its value from the fixed fields)
-__device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool &mirrored) {
-  mirrored = false;
-  if (m == 0 || kind == 2) return x;
-  const int n = total - 2 * m;
-  if (x < m) {
-    if (kind == 1) return x + n;
-    mirrored = true;
-    return 2 * m - 1 - x;
-  }
-  if (x >= total - m) {
-    if (kind == 1) return x - n;
-    mirrored = true;
-    return 2 * (total - m) - 1 - x;
-  }
-  return x;
-}
 
 // Flattened over the shell: x slabs (all j, k), then y slabs (interior i,
 // all k), then z slabs (interior i, j).  The three sets partition the ghosts.

@@ -156,6 +156,9 @@ hipError_t launch_av_last_sweep_cfl(hipStream_t, const Fld<const R> &, const Fld
                                     const R *, const R *, const R *, const Shape &,
                                     const AvScalars &, const CflFuse<R> &);
 template <typename R>
+hipError_t launch_av_sweep_xy(hipStream_t, const Fld<const R> &, const Fld<R> &, R *, const R *,
+                              const R *, const Shape &, const BcDev &, const AvScalars &);
+template <typename R>
 hipError_t launch_char_vel(hipStream_t, const Fld<const R> &, R *, int, const AvScalars &);
 template <typename R>
 hipError_t launch_ghost_fill(hipStream_t, const Fld<R> &, const Shape &, const BcDev &);

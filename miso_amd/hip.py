@@ -45,7 +45,8 @@ class MisohipParams(C.Structure):
 
 class MisohipTiles(C.Structure):
     _fields_ = [("rk_pencil_bx", C.c_int), ("rk_pencil_by", C.c_int), ("sweep_chunk", C.c_int),
-                ("rk_variant", C.c_int), ("rk_zchunk", C.c_int)]
+                ("rk_variant", C.c_int), ("rk_zchunk", C.c_int), ("av_fuse_xy", C.c_int),
+                ("av_xy_chunk", C.c_int)]
 
 
 class MisohipRtConfig(C.Structure):

@@ -21,7 +21,8 @@ def test_tune_single_keeps_results(ctx, tmp_path):
     from miso_amd import mhd, problems
     from miso_amd.autotune import get_tiles, set_tiles, tune_single
     t0 = get_tiles(ctx)
-    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
     g = mhd.Grid(48, 40, 36, 2, device="cuda:0")
     bc = mhd.BoundaryCondition((True, True, True))
     cfg = mhd.Config({"mhd": {"cfl_number": 0.4}})
@@ -72,7 +73,8 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
     from miso_amd import mhd, problems
     from miso_amd.autotune import get_tiles, set_tiles
     t0 = get_tiles(ctx)
-    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
     for sizes in ((70, 36, 20), (66, 37, 13)):  # partial x and y tiles, uneven z chunks
         g = mhd.Grid(*sizes, 2, device="cuda:0")
         bc = mhd.BoundaryCondition((True, True, True))
@@ -91,3 +93,44 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
         set_tiles(ctx, *default)
         for n, o in enumerate(outs[1:], 1):
             assert torch.equal(o, outs[0]), (sizes, n)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ybc", ["periodic", "mirror", "fixed"])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_fused_xy_sweeps_compute_the_same_bits(ctx, ybc, dtype):
+    """The x and y AV sweeps as one walk equal the separate sweeps bit for
+    bit (3D, partial x segments, small and uneven y extents and chunks), for
+    periodic, mirrored (with sign flips) and fixed y boundaries."""
+    import numpy as np
+    import torch
+    from miso_amd import mhd, problems
+    from miso_amd.autotune import get_tiles, set_tiles
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
+    dt = np.float64 if dtype == "f64" else np.float32
+    sym = ((1,) * 9, (1, 1, -1, 1, 1, -1, 1, 1, 1), (1,) * 9)
+    for sizes in ((70, 2, 9), (130, 13, 6), (61, 37, 5)):
+        g = mhd.Grid(*sizes, 2, dtype=dt, device="cuda:0")
+        m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
+        m.qq.load_ijk(problems.turbulence(g, device="cpu"))
+        fx = None
+        if ybc == "fixed":
+            fx = mhd.Fields(g)
+            fx.data.copy_(m.qq.data)
+        per = {"periodic": (1, 1, 1), "mirror": (1, 0, 1), "fixed": (1, 2, 1)}[ybc]
+        bc = mhd.BoundaryCondition(per, sym if ybc == "mirror" else None, fixed=fx)
+        m.integrator.apply_boundary_condition(bc, m.qq)
+        start = m.qq.data.clone()
+        outs = []
+        for tiles in ((2, 8, 64, 0, 32, 0), (2, 8, 64, 0, 32, 1, 64), (2, 8, 4, 0, 32, 1, 4),
+                      (1, 1, 7, 5, 16, 1, 5), (1, 1, 256, 5, 256, 1, 256)):
+            m.qq.data.copy_(start)
+            set_tiles(ctx, *tiles)
+            dts = m.integrator.run(bc, m.qq, 2)
+            outs.append((m.qq.data.clone(), dts))
+        set_tiles(ctx, *default)
+        for n, (o, d) in enumerate(outs[1:], 1):
+            assert d == outs[0][1], (sizes, n)
+            assert torch.equal(o, outs[0][0]), (sizes, n, (o - outs[0][0]).abs().max().item())

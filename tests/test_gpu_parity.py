@@ -300,7 +300,8 @@ def test_step_fp64_z_march(ctx, case, variant, zchunk):
     from miso_amd.autotune import get_tiles, set_tiles
     problem, sizes, n, kw = STEP_CASES[case]
     t0 = get_tiles(ctx)
-    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
     set_tiles(ctx, 1, 1, 64, variant, zchunk)
     try:
         out, ref, dts, ref_dts = _run_both(ctx, problem, sizes, n, use_run=True, **kw)
@@ -308,6 +309,24 @@ def test_step_fp64_z_march(ctx, case, variant, zchunk):
         set_tiles(ctx, *default)
     assert np.array_equal(dts, ref_dts), (dts, ref_dts)
     assert_bitwise(out, ref, f"z-march {case} variant {variant} zchunk {zchunk}")
+
+
+@pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
+                                  "stratified_gravity_16"])
+def test_step_fp64_fused_xy_sweeps(ctx, case):
+    """The one-walk x and y AV sweeps against the oracle, bit for bit."""
+    from miso_amd.autotune import get_tiles, set_tiles
+    problem, sizes, n, kw = STEP_CASES[case]
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
+    set_tiles(ctx, 1, 1, 16, 5, 32, 1, 16)
+    try:
+        out, ref, dts, ref_dts = _run_both(ctx, problem, sizes, n, use_run=True, **kw)
+    finally:
+        set_tiles(ctx, *default)
+    assert np.array_equal(dts, ref_dts), (dts, ref_dts)
+    assert_bitwise(out, ref, f"fused x-y sweeps {case}")
 
 
 @pytest.mark.parametrize("sizes", [(96, 1, 1), (1, 80, 1), (1, 1, 72), (40, 1, 24)])
