@@ -395,6 +395,9 @@ def rmhd_leg(args):
         g = Grid(n, n, n, bounds=((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), device=dev)
         b = RadiativeMHD(g, ctx)
         b.initialize(seed)
+        if not args.no_tune:  # the MHD step's tiles for this size (cached after the first box)
+            from miso_amd.autotune import tune_single
+            tune_single(ctx, b.model, b.bc, steps=1)
         return b
 
     def time_plain(n, steps, seed):

@@ -689,8 +689,8 @@ static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld
                                 const AvScalars &sc, const R *dz_arr = nullptr) {
   const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
   const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
-  const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
-  const int nchunks = (n_dir + chunk - 1) / chunk;
+  const int nchunks = (n_dir + (sc.chunk < 4 ? 4 : sc.chunk) - 1) / (sc.chunk < 4 ? 4 : sc.chunk);
+  const int chunk = (n_dir + nchunks - 1) / nchunks;  // even walk lengths
   const int wx = walk_waves_x(n_oth);
   const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
@@ -751,8 +751,8 @@ static hipError_t launch_walk_cfl_t(hipStream_t st, const Fld<const R> &q, const
                                     const AvScalars &sc, const R *dz_arr, const CflFuse<R> &cf) {
   const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
   const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
-  const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
-  const int nchunks = (n_dir + chunk - 1) / chunk;
+  const int nchunks = (n_dir + (sc.chunk < 4 ? 4 : sc.chunk) - 1) / (sc.chunk < 4 ? 4 : sc.chunk);
+  const int chunk = (n_dir + nchunks - 1) / nchunks;  // even walk lengths
   const int wx = walk_waves_x(n_oth);
   const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
@@ -782,8 +782,8 @@ hipError_t launch_av_sweep_xy(hipStream_t st, const Fld<const R> &q, const Fld<R
                               const BcDev &bc, const AvScalars &sc) {
   const int nj = sh.Jt - 2 * sh.mj;
   if (!sh.is || !sh.js || !sh.ks || nj < 2 || sh.mj < 2) return hipErrorInvalidValue;
-  const int chunk = sc.chunk < 4 ? 4 : sc.chunk;
-  const int nchunks = (nj + chunk - 1) / chunk;
+  const int nchunks = (nj + (sc.chunk < 4 ? 4 : sc.chunk) - 1) / (sc.chunk < 4 ? 4 : sc.chunk);
+  const int chunk = (nj + nchunks - 1) / nchunks;  // even walk lengths
   const int nseg = (sh.It - 2 * sh.mi + 59) / 60;
   const long nw = (long)nseg * nchunks * (sh.Kt - 2 * sh.mk + 2);
   const long nbl = (nw + 3) / 4;

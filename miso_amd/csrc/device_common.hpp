@@ -105,8 +105,9 @@ struct March {
   int nbx, nby, nzc, kz, nb;
   __host__ bool init(int nbx_, int nby_, int nk, int kz_) {
     if (kz_ < 1 || nk < 1) return false;
-    nbx = nbx_; nby = nby_; kz = kz_ > nk ? nk : kz_;
-    nzc = (nk + kz - 1) / kz;
+    nbx = nbx_; nby = nby_;
+    nzc = (nk + kz_ - 1) / kz_;
+    kz = (nk + nzc - 1) / nzc;  // the chunks as even as the count allows
     const long n = (long)nbx * nby * nzc;
     if (n > 2147483647L || n <= 0) return false;
     nb = (int)n;
