@@ -41,6 +41,7 @@ struct RtRay {
   double w00, w10, w01, w11;  // bilinear weights
   double ds;
   double wq;              // quadrature weight
+  int id;                 // the ray's slot in the intensity array
 };
 
 struct RtRays {

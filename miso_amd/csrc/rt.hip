@@ -24,7 +24,7 @@ __device__ __forceinline__ double rt_integrate(double Iu, double Su, double Sd, 
   return Iu * exp_dtau + (u0 - u1) * Su + u1 * Sd;
 }
 
-// plane p of every ray; blockIdx.y = ray
+// plane p of every ray of `rays` (blockIdx.y = entry, R.id = its slot)
 __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict__ I,
                                                              const double *__restrict__ kap,
                                                              const double *__restrict__ S,
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict_
   const int c = R.start + p * R.da + ib * R.db + ic * R.dc;  // cell index (kappa, S)
   const int c00 = c - R.ua;
   const int c10 = c00 - R.ub, c01 = c00 - R.uc, c11 = c00 - R.ub - R.uc;
-  double *Ir = I + ray_stride * blockIdx.y;
+  double *Ir = I + ray_stride * R.id;
   const double iu = R.w00 * Ir[c00] + R.w10 * Ir[c10] + R.w01 * Ir[c01] + R.w11 * Ir[c11];
   const double su = R.w00 * S[c00] + R.w10 * S[c10] + R.w01 * S[c01] + R.w11 * S[c11];
   const double ku = R.w00 * kap[c00] + R.w10 * kap[c10] + R.w01 * kap[c01] + R.w11 * kap[c11];

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "ctx_internal.hpp"
 #include "launch.hpp"
@@ -133,6 +134,7 @@ int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, 
     R.w11 = cx * cy;
     R.ds = ds;
     R.wq = S.w[n];
+    R.id = n;
     S.max_nodes = std::max(S.max_nodes, R.nb * R.nc);
     S.max_planes = std::max(S.max_planes, R.nplanes);
   }
@@ -172,6 +174,35 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
     lo[a] = su.b0[a];
     cnt[a] = su.b1[a] - su.b0[a] + 1;
   }
+  // Sweep phases: plane ranges over which the set of rays that still have
+  // planes is constant; each phase launches only those rays, sized for
+  // their largest plane (a thin slab's z rays end long before its x/y rays).
+  struct Phase {
+    int p0, p1, n, max_nodes;
+    RtRays rays;
+  };
+  std::vector<Phase> phases;
+  {
+    std::vector<int> ends;
+    for (int n = 0; n < su.nrays; ++n) ends.push_back(su.rays.r[n].nplanes);
+    std::sort(ends.begin(), ends.end());
+    int p0 = 0;
+    for (int e : ends) {
+      if (e <= p0) continue;
+      Phase ph;
+      ph.p0 = p0;
+      ph.p1 = e;
+      ph.n = 0;
+      ph.max_nodes = 0;
+      for (int n = 0; n < su.nrays; ++n)
+        if (su.rays.r[n].nplanes > p0) {
+          ph.rays.r[ph.n++] = su.rays.r[n];
+          ph.max_nodes = std::max(ph.max_nodes, su.rays.r[n].nb * su.rays.r[n].nc);
+        }
+      phases.push_back(ph);
+      p0 = e;
+    }
+  }
   int it = 0;
   double md = 0.0;
   for (it = 0; it < rt->max_iters; ++it) {
@@ -197,9 +228,10 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
                                      rt->face_value[2 * a + side], in_mask, su.ray_stride,
                                      su.nrays));
         }
-      for (int p = 0; p < su.max_planes; ++p)
-        MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, su.rays, su.nrays, p,
-                                          su.ray_stride, su.max_nodes));
+      for (const Phase &ph : phases)
+        for (int p = ph.p0; p < ph.p1; ++p)
+          MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, ph.rays, ph.n, p,
+                                            su.ray_stride, ph.max_nodes));
       MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo, cnt, It, Jt, su.ray_stride,
                                     su.nrays, ctx->d_slot + 1));
     }
