@@ -205,6 +205,13 @@ typedef struct misohip_rt_config {
  * narrowed max change is below tolerance (rt_host.hpp:632-651).  Writes the
  * iteration count and the last max change; synchronises once per
  * iteration. */
+/* The angular quadrature the solver uses (AngularQuadrature,
+ * miso/include/miso/rt_quadrature.hpp:14-63): for num_rays 24 the Carlson A4
+ * set, otherwise one +z ray.  Writes the weights and direction cosines of
+ * every ray (arrays of at least 24) and returns the ray count; no device
+ * work. */
+int misohip_rt_quadrature(int num_rays, double *w, double *mu_x, double *mu_y, double *mu_z);
+
 int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
                      const void *kappa, const void *S, void *rint, void *rint_old,
                      int *iters, double *max_diff);

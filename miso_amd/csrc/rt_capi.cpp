@@ -155,6 +155,11 @@ int check_rt(const misohip_grid *g, const misohip_rt_config *rt) {
 
 extern "C" {
 
+int misohip_rt_quadrature(int num_rays, double *w, double *mu_x, double *mu_y, double *mu_z) {
+  if (!w || !mu_x || !mu_y || !mu_z) return MISOHIP_E_ARG;
+  return quadrature(num_rays, w, mu_x, mu_y, mu_z);
+}
+
 int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
                      const void *kappa, const void *S, void *rint, void *rint_old, int *iters,
                      double *max_diff) {

@@ -6,6 +6,9 @@ pymiso and MISO's own tools: the layout of one run directory is
     <save_dir>/time/n_output.txt
     <save_dir>/mpi/coords.csv                mpi::Shape::save  (mpi_util.hpp:111-126)
     <save_dir>/mhd/mhd.<n>.<rank>.bin        Checkpoint::save  (mhd_checkpoint.hpp:29-61)
+    <save_dir>/rt/rank_<rank>.bin            RT::save          (rt_host.hpp:173-199, the
+                                             searchlight demo's name, main.cpp:117-122)
+    <save_dir>/config.yaml                   Config::save      (config.hpp:127-142)
 
 with <n> and <rank> zero-padded to `n_output_digits` (util::zfill,
 utility.hpp:62-66) and the directory names of MISO's io defaults
@@ -13,6 +16,9 @@ utility.hpp:62-66) and the directory names of MISO's io defaults
 element size (4 or 8), then the arrays in MISO's memory order -- the grid
 axes x, y, z with their ghost cells, and the nine fields ro vx vy vz bx by
 bz ei ph, each (i_total, j_total, k_total) with k fastest (array3d.hpp:54-60).
+The RT file has no element-size header: an int32 ray count, the weights and
+the three direction cosines of every ray, then S, kappa and the intensities
+(num_rays, i_total, j_total, k_total), all in the solver's element type.
 This build runs one rank, so coords.csv holds rank 0 at (0, 0, 0).
 """
 from __future__ import annotations
@@ -25,7 +31,7 @@ import numpy as np
 
 FIELD_NAMES = ("ro", "vx", "vy", "vz", "bx", "by", "bz", "ei", "ph")
 IO_DEFAULTS = {"save_dir": "data/", "time_save_dir": "time/", "mpi_save_dir": "mpi/",
-               "mhd_save_dir": "mhd/", "n_output_digits": 8}
+               "mhd_save_dir": "mhd/", "rt_save_dir": "rt/", "n_output_digits": 8}
 
 
 def zfill(num: int, width: int) -> str:
@@ -146,6 +152,51 @@ class RunFiles:
         if out.size != 9 * n:
             raise RuntimeError("Checkpoint file is shorter than nine fields of this grid")
         return out.reshape((9,) + tuple(shape)).astype(dtype)
+
+    # -- RT ----------------------------------------------------------------------
+    def rt_filename(self) -> str:
+        return os.path.join(self._dir("rt_save_dir"), f"rank_{zfill(self.rank, self.digits)}.bin")
+
+    def save_rt(self, rt) -> str:
+        """RT::save of a `miso_amd.rt.RT` (its quadrature, S, kappa and
+        intensities, moved to MISO order on the host)."""
+        w, mx, my, mz = rt.quadrature()
+        n = len(w)
+        ft = _ftype(_elem(rt.grid.dtype))
+        os.makedirs(self._dir("rt_save_dir"), exist_ok=True)
+        name = self.rt_filename()
+        with open(name, "wb") as f:
+            f.write(struct.pack("<i", n))
+            for a in (w, mx, my, mz):
+                np.asarray(a).astype(ft).tofile(f)
+            for t in (rt.src_func, rt.abs_coeff):
+                np.ascontiguousarray(t.permute(2, 1, 0).cpu().numpy()).astype(ft).tofile(f)
+            np.ascontiguousarray(rt.rint[:n].permute(0, 3, 2, 1).cpu().numpy()).astype(ft).tofile(f)
+        return name
+
+    def load_rt(self, shape, dtype=np.float64) -> dict:
+        """The arrays of an RT file, given the grid's (i_total, j_total, k_total)."""
+        ft = _ftype(_elem(dtype))
+        cells = int(np.prod(shape))
+        with open(self.rt_filename(), "rb") as f:
+            (n,) = struct.unpack("<i", f.read(4))
+            w, mx, my, mz = (np.fromfile(f, dtype=ft, count=n) for _ in range(4))
+            src = np.fromfile(f, dtype=ft, count=cells).reshape(shape)
+            kap = np.fromfile(f, dtype=ft, count=cells).reshape(shape)
+            rint = np.fromfile(f, dtype=ft, count=n * cells).reshape((n,) + tuple(shape))
+        return {"num_rays": n, "weights": w, "mu_x": mx, "mu_y": my, "mu_z": mz,
+                "src_func": src, "abs_coeff": kap, "rint": rint}
+
+    # -- config --------------------------------------------------------------------
+    def save_config(self, config: dict) -> None:
+        """config.yaml in save_dir (YAML of the run's configuration, with the
+        io keys this writer used)."""
+        import yaml
+        os.makedirs(self.save_dir, exist_ok=True)
+        c = dict(config)
+        c["io"] = dict(c.get("io", {}), **self.io)
+        with open(os.path.join(self.save_dir, "config.yaml"), "w") as f:
+            yaml.safe_dump(c, f, sort_keys=False)
 
     def save_run(self, grid, t: Time, q_host: np.ndarray) -> str:
         """Everything a MISO run leaves for one output: grid, coords, time, fields."""

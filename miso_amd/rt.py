@@ -53,6 +53,15 @@ class RT:
                  C.byref(it), C.byref(md))
         return it.value, md.value
 
+    def quadrature(self):
+        """(weights, mu_x, mu_y, mu_z) of the solver's rays
+        (AngularQuadrature, rt_quadrature.hpp:14-63)."""
+        bufs = [(C.c_double * 24)() for _ in range(4)]
+        n = hip.lib().misohip_rt_quadrature(self.num_rays, *bufs)
+        if n < 0:
+            hip.check(n)
+        return tuple(np.array(b[:n]) for b in bufs)
+
     def qrad(self, ctx, **kw):
         cfg = self.config(**kw)
         hip.call("misohip_rt_qrad", ctx.handle, C.byref(self.grid.c_struct()), C.byref(cfg),

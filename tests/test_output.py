@@ -86,6 +86,45 @@ def test_time_text_uses_six_significant_digits(tmp_path):
     assert (tmp_path / "data" / "time" / "time.00000002.txt").read_text() == "1.23457e+06\n2\n3\n"
 
 
+def test_rt_file_has_miso_layout(tmp_path):
+    """RT::save layout (rt_host.hpp:173-199): int32 ray count, weights and
+    direction cosines (the reference's own quadrature, committed vectors),
+    S, kappa and the intensities in MISO order; read back equal."""
+    import os.path as op
+
+    import torch
+
+    from miso_amd.rt import RT
+    g = Grid(4, 3, 2, 2, dtype=np.float64, device="cpu")
+    rt = RT(g, 24)
+    rng = np.random.default_rng(3)
+    for t in (rt.src_func, rt.abs_coeff, rt.rint):
+        t.copy_(torch.from_numpy(rng.standard_normal(tuple(t.shape))))
+    rf = RunFiles(str(tmp_path))
+    name = rf.save_rt(rt)
+    assert name == str(tmp_path / "data" / "rt" / "rank_00000000.bin")
+    ref = np.load(op.join(op.dirname(__file__), "golden", "ref_limiter_quadrature.npz"))
+    q = ref["quadrature_24"]  # columns: weight, mu_x, mu_y, mu_z
+    shape = (g.i_total, g.j_total, g.k_total)
+    expect = (struct.pack("<i", 24) + b"".join(q[:, c].astype("<f8").tobytes() for c in range(4))
+              + rt.src_func.permute(2, 1, 0).numpy().astype("<f8").tobytes()
+              + rt.abs_coeff.permute(2, 1, 0).numpy().astype("<f8").tobytes()
+              + rt.rint.permute(0, 3, 2, 1).numpy().astype("<f8").tobytes())
+    assert open(name, "rb").read() == expect
+    back = rf.load_rt(shape)
+    assert back["num_rays"] == 24
+    assert np.array_equal(back["rint"], rt.rint.permute(0, 3, 2, 1).numpy())
+
+
+def test_config_yaml_records_the_io_keys(tmp_path):
+    import yaml
+    rf = RunFiles(str(tmp_path), io={"n_output_digits": 6})
+    rf.save_config({"mhd": {"cfl_number": 0.4}, "grid": {"i_size": 8}})
+    c = yaml.safe_load((tmp_path / "data" / "config.yaml").read_text())
+    assert c["mhd"]["cfl_number"] == 0.4 and c["grid"]["i_size"] == 8
+    assert c["io"]["n_output_digits"] == 6 and c["io"]["mhd_save_dir"] == "mhd/"
+
+
 @pytest.mark.gpu
 def test_gpu_run_checkpoint_round_trip(ctx, tmp_path):
     """Fields of a GPU run written as a MISO checkpoint and read back are the
