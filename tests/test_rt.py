@@ -128,3 +128,28 @@ def test_gpu_stratified_3d(ctx):
     assert np.abs(I - I_ref).max() <= 1e-12 * scale
     lat = (slice(1, -2), slice(1, -2), slice(1, -2))
     assert np.abs(q[lat] - q_ref[lat]).max() <= 1e-11 * np.abs(q_ref[lat]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sizes", [(64, 48, 6), (40, 72, 9)])
+def test_gpu_thin_slab_per_ray_sweep(ctx, sizes):
+    """A thin slab (x, y periodic, thermal bottom, vacuum top; the learned
+    RT's bottom-slab shape): the z rays end after a few planes, the x- and
+    y-dominant rays sweep on alone; against the oracle."""
+    og = O.make_grid(sizes, 2, bounds=_bounds())
+    rng = np.random.default_rng(11)
+    z = og.axis("z")[None, None, :]
+    T = 1.5 - 0.5 * (z + 0.5) + 0.05 * rng.standard_normal(og.shape)
+    kap = 20.0 * (1 + 0.1 * rng.standard_normal(og.shape)) ** 2
+    S = T ** 4
+    I_ref = np.zeros((24,) + og.shape)
+    kw = dict(periodic=(1, 1, 0), face_kind=(0, 0, 0, 0, 1, 2), tolerance=1e-9, max_iters=300)
+    it_ref, md_ref = O.rt_solve(og, kap, S, I_ref, **kw)
+    q_ref = O.rt_qrad(og, kap, S, I_ref)
+    it, md, I, q = _gpu_rt(ctx, og, kap, S, 24, periodic=(True, True, False),
+                           face_kind=kw["face_kind"], tolerance=1e-9, max_iters=300)
+    assert it == it_ref, (it, it_ref)
+    scale = np.abs(I_ref).max()
+    assert np.abs(I - I_ref).max() <= 1e-12 * scale
+    lat = (slice(1, -2), slice(1, -2), slice(1, -2))
+    assert np.abs(q[lat] - q_ref[lat]).max() <= 1e-11 * np.abs(q_ref[lat]).max()
