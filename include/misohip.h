@@ -350,6 +350,19 @@ int misohip_batch_set_chunks(misohip_ctx *ctx, misohip_batch *b, const int *chun
 int misohip_batch_state(misohip_ctx *ctx, misohip_batch *b, double *dt, double *time,
                         long *steps, int *exp_fallback);
 
+/* ---------------------------------------------------------------------
+ * Uniform-table interpolation over device arrays: UniformTableInterpolator1D
+ * and 2D (miso/include/miso/table_interpolator.hpp:22-166) with their
+ * interpolate() over flat or 3D arrays (count elements).  The table has n
+ * (n0 x n1, row-major, the second index contiguous) entries spanning
+ * [x_min, x_max] uniformly; outside the range the nearest entry is taken.
+ * Element type dt for table, x and y.  Asynchronous on the context's stream. */
+int misohip_table1d(misohip_ctx *ctx, misohip_dtype dt, const void *table, int n, double x_min,
+                    double x_max, const void *x, void *y, long count);
+int misohip_table2d(misohip_ctx *ctx, misohip_dtype dt, const void *table, int n0, int n1,
+                    double x0_min, double x0_max, double x1_min, double x1_max, const void *x0,
+                    const void *x1, void *y, long count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,6 +162,11 @@ hipError_t launch_av_sweep_xy(hipStream_t, const Fld<const R> &, const Fld<R> &,
 template <typename R>
 hipError_t launch_char_vel(hipStream_t, const Fld<const R> &, R *, int, const AvScalars &);
 template <typename R>
+hipError_t launch_table1d(hipStream_t, const R *, int, R, R, const R *, R *, long);
+template <typename R>
+hipError_t launch_table2d(hipStream_t, const R *, int, int, R, R, R, R, const R *, const R *, R *,
+                          long);
+template <typename R>
 hipError_t launch_ghost_fill(hipStream_t, const Fld<R> &, const Shape &, const BcDev &);
 template <typename R>
 hipError_t launch_cfl(hipStream_t, const Fld<const R> &, const R *, const R *, const R *,

@@ -83,6 +83,10 @@ SIGNATURES = {
                        _P(MisohipFields), _P(MisohipWork), C.c_double],
     "misohip_step": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
                      _P(MisohipFields), _P(MisohipWork), _P(C.c_double)],
+    "misohip_table1d": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                        C.c_void_p, C.c_void_p, C.c_long],
+    "misohip_table2d": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                        C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long],
     "misohip_rt_quadrature": [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                               _P(C.c_double)],
     "misohip_rt_solve": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,
