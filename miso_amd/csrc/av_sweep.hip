@@ -450,13 +450,28 @@ __global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rs
 // interface flux at its own point once and takes its neighbours' from the
 // adjacent lanes.  It also stores the frozen cc for the y and z sweeps over
 // the interior plus a one-cell ring in y and z.
+// Whole-wave shifts by one lane as DPP moves (VALU, no LDS round trip):
+// wave_shr:1 (0x138) gives lane i the value of lane i-1, wave_shl:1 (0x130)
+// that of lane i+1; the end lanes, which no caller stores, get 0.
+template <int CTRL> __device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_move<CTRL>((int)(unsigned)(b & 0xffffffffLL));
+  const unsigned hi = (unsigned)dpp_move<CTRL>((int)(unsigned)((unsigned long long)b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 template <typename R>
 __device__ __forceinline__ R lane_up(R v) {  // value of lane - 1
-  return __shfl_up(v, 1, 64);
+  return dpp_move<0x138>(v);
 }
 template <typename R>
 __device__ __forceinline__ R lane_dn(R v) {  // value of lane + 1
-  return __shfl_down(v, 1, 64);
+  return dpp_move<0x130>(v);
 }
 
 template <typename R>
@@ -543,44 +558,112 @@ __device__ __forceinline__ void av_xrow_load(const Fld<const R> &qq, const Shape
   for (int m = 0; m < 9; ++m) f[m] = qq.q[m][c];
 }
 
-// x-swept fields of row jr (as conserved values) and its frozen speed, from
-// the fields av_xrow_load gave for it
+// Conserved component m (ro, ro vx, ro vy, ro vz, bx, by, bz, ph, ro ei: the
+// order of AvCons) of a cell from its nine fields; the products are the ones
+// load_cons forms.
+template <typename R, int M> __device__ __forceinline__ R cons_m(const R f[9]) {
+  return M == 0 ? f[0] : (M <= 3 ? f[0] * f[M] : (M <= 6 ? f[M] : (M == 7 ? f[8] : f[0] * f[7])));
+}
+
+// Total energy of a cell (av_cell_values' Et, in double).
+template <typename R> __device__ __forceinline__ double cell_et(const R f[9]) {
+  const double pii8 = double(Const<R>::pii8);
+  const double ro = f[0];
+  return ro * double(f[7]) + 0.5 * ro * (double(f[1]) * f[1] + double(f[2]) * f[2] + double(f[3]) * f[3]) +
+         pii8 * (double(f[4]) * f[4] + double(f[5]) * f[5] + double(f[6]) * f[6]);
+}
+
+// One field of av_cell_values from its flux difference df = (fup - fdw) dd dt:
+// the new value into o (ro first: the velocities divide by it), the
+// energy's term into efl.
+template <typename R, int M>
+__device__ __forceinline__ void cell_field(R v, R df, R o[9], R &efl) {
+  if (M == 0) o[0] = v - df;
+  else if (M <= 3) o[M] = (v - df) / o[0];
+  else if (M <= 6) o[M] = v - df;
+  else if (M == 7) o[8] = v - df;
+  else efl = df;
+}
+
+// The internal energy (av_cell_values' recovery) once every other field is new.
+template <typename R>
+__device__ __forceinline__ void cell_energy(double Et, R efl, R o[9]) {
+  const double pii8 = double(Const<R>::pii8);
+  const double dro = o[0], dvx = o[1], dvy = o[2], dvz = o[3], dbx = o[4], dby = o[5], dbz = o[6];
+  const double ein = (Et - double(efl) - 0.5 * dro * (dvx * dvx + dvy * dvy + dvz * dvz) -
+                      pii8 * (dbx * dbx + dby * dby + dbz * dbz)) /
+                     dro;
+  o[7] = R(ein);
+}
+
+// x-sweep of one field M at every lane (the x-shuffle kernel's slope, flux
+// and update), accumulated into o / efl
+template <typename R, int M>
+__device__ __forceinline__ void xsweep_field(const R f[9], R cup, R ep, R fh, R dd, R dt, R o[9],
+                                             R &efl) {
+  const R v = cons_m<R, M>(f);
+  const R vm = lane_up(v), vp = lane_dn(v);
+  const R d = dqq_eval(vm, v, vp, ep);
+  const R dn = lane_dn(d);
+  const R fup = flux_core(v, vp, d, dn, cup, fh);
+  const R fdw = lane_up(fup);
+  cell_field<R, M>(v, (fup - fdw) * dd * dt, o, efl);
+}
+
+// x-swept fields of row jr (primitive) and its frozen speed, from the fields
+// av_xrow_load gave for it; one field at a time, so only one field's
+// slopes and fluxes are live
 template <typename R>
 __device__ __forceinline__ void av_xrow_sweep(const R f[9], const Shape &sh, const BcDev &bc,
-                                              int jr, R ddx, const AvScalars &sc, R dt,
-                                              AvCons<R> &out, R &ccv) {
+                                              int jr, R ddx, const AvScalars &sc, R dt, R o[9],
+                                              R &ccv) {
   const bool ghost = jr < sh.mj || jr >= sh.Jt - sh.mj;
   const int kind = bc.periodic[1];
   const bool mir = ghost && kind == 0;
   ccv = char_speed_v(f, R(sc.gm), R(sc.cs_fac), R(sc.ca_fac), R(sc.vv_fac));
-  if (ghost && kind == 2) {  // uniform per wave
-    cons_of(f, out);
+  if (ghost && kind == 2) {  // uniform per wave: a fixed ghost row is not swept
+#pragma unroll
+    for (int m = 0; m < 9; ++m) o[m] = f[m];
     return;
   }
-  AvCons<R> a;
-  cons_of(f, a);
   const R ep = R(sc.ep), fh = R(sc.fh);
   const R cup = R(0.5) * (ccv + lane_dn(ccv));
-  R fdw[9], fup[9];
-#pragma unroll
-  for (int m = 0; m < 9; ++m) {
-    const R vm = lane_up(a.v[m]), vp = lane_dn(a.v[m]);
-    const R d = dqq_eval(vm, a.v[m], vp, ep);
-    const R dn = lane_dn(d);
-    fup[m] = flux_core(a.v[m], vp, d, dn, cup, fh);
-    fdw[m] = lane_up(fup[m]);
-  }
-  R o[9];
-  av_cell_values(a, fdw, fup, ddx, dt, o);
+  const double Et = cell_et(f);
+  R efl;
+  xsweep_field<R, 0>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 1>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 2>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 3>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 4>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 5>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 6>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 7>(f, cup, ep, fh, ddx, dt, o, efl);
+  xsweep_field<R, 8>(f, cup, ep, fh, ddx, dt, o, efl);
+  cell_energy<R>(Et, efl, o);
   if (mir) {
 #pragma unroll
     for (int m = 0; m < 9; ++m) o[m] = R(bc.sign[1][m]) * o[m];
   }
-  cons_of(o, out);
+}
+
+// y step of one field M of the window (p, p+1, p+2 as primitive fields):
+// slope at p+1, flux at p+1/2, the update of cell p; d0 and fdw carry the
+// slope at p and the flux at p-1/2
+template <typename R, int M>
+__device__ __forceinline__ void ystep_field(const R w0[9], const R w1[9], const R wn[9], R cup,
+                                            R ep, R fh, R dd, R dt, R d0[9], R fdw[9], R o[9],
+                                            R &efl) {
+  const R v0 = cons_m<R, M>(w0), v1 = cons_m<R, M>(w1), vn = cons_m<R, M>(wn);
+  const R d1 = dqq_eval(v0, v1, vn, ep);
+  const R fup = flux_core(v0, v1, d0[M], d1, cup, fh);
+  d0[M] = d1;
+  const R df = (fup - fdw[M]) * dd * dt;
+  fdw[M] = fup;
+  cell_field<R, M>(v0, df, o, efl);
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void av_xy_walk_kernel(Fld<const R> qq, Fld<R> rslt,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void av_xy_walk_kernel(Fld<const R> qq, Fld<R> rslt,
                                                         R *__restrict__ cc_store,
                                                         const R *__restrict__ dxi,
                                                         const R *__restrict__ dyi, Shape sh,
@@ -607,58 +690,76 @@ __global__ __launch_bounds__(256) void av_xy_walk_kernel(Fld<const R> qq, Fld<R>
   const R dt = R(sc.dt), ep = R(sc.ep), fh = R(sc.fh);
   const R ddx = out_lane ? dxi[i] : R(0);
   const int jb = sh.mj + p0;  // first output row
-  // window: a0 = p, a1 = p+1 (x-swept); slopes at p; flux p-1/2
-  AvCons<R> am, a0, a1, a2;
+  // window: w0 = p, w1 = p+1 (x-swept, primitive); slopes at p; flux p-1/2
+  R wm[9], w0[9], w1[9], w2[9];
   R cc_m, cc_0, cc_1;
   R nf[9];  // fields of the next row to sweep, loaded one row ahead
   av_xrow_load(qq, sh, bc, ic, jb - 2, k, nf);
-  av_xrow_sweep(nf, sh, bc, jb - 2, ddx, sc, dt, am, cc_m);
+  av_xrow_sweep(nf, sh, bc, jb - 2, ddx, sc, dt, wm, cc_m);
   av_xrow_load(qq, sh, bc, ic, jb - 1, k, nf);
-  av_xrow_sweep(nf, sh, bc, jb - 1, ddx, sc, dt, a0, cc_m);
+  av_xrow_sweep(nf, sh, bc, jb - 1, ddx, sc, dt, w0, cc_m);
   av_xrow_load(qq, sh, bc, ic, jb, k, nf);
-  av_xrow_sweep(nf, sh, bc, jb, ddx, sc, dt, a1, cc_0);
+  av_xrow_sweep(nf, sh, bc, jb, ddx, sc, dt, w1, cc_0);
   av_xrow_load(qq, sh, bc, ic, jb + 1, k, nf);
-  av_xrow_sweep(nf, sh, bc, jb + 1, ddx, sc, dt, a2, cc_1);
+  av_xrow_sweep(nf, sh, bc, jb + 1, ddx, sc, dt, w2, cc_1);
   av_xrow_load(qq, sh, bc, ic, jb + 2, k, nf);
   R d0[9], fdw[9];
   {
     const R cdw = R(0.5) * (cc_m + cc_0);
+    R am[9], a0[9], a1[9], a2[9];
+    am[0] = cons_m<R, 0>(wm); a0[0] = cons_m<R, 0>(w0); a1[0] = cons_m<R, 0>(w1); a2[0] = cons_m<R, 0>(w2);
+    am[1] = cons_m<R, 1>(wm); a0[1] = cons_m<R, 1>(w0); a1[1] = cons_m<R, 1>(w1); a2[1] = cons_m<R, 1>(w2);
+    am[2] = cons_m<R, 2>(wm); a0[2] = cons_m<R, 2>(w0); a1[2] = cons_m<R, 2>(w1); a2[2] = cons_m<R, 2>(w2);
+    am[3] = cons_m<R, 3>(wm); a0[3] = cons_m<R, 3>(w0); a1[3] = cons_m<R, 3>(w1); a2[3] = cons_m<R, 3>(w2);
+    am[4] = cons_m<R, 4>(wm); a0[4] = cons_m<R, 4>(w0); a1[4] = cons_m<R, 4>(w1); a2[4] = cons_m<R, 4>(w2);
+    am[5] = cons_m<R, 5>(wm); a0[5] = cons_m<R, 5>(w0); a1[5] = cons_m<R, 5>(w1); a2[5] = cons_m<R, 5>(w2);
+    am[6] = cons_m<R, 6>(wm); a0[6] = cons_m<R, 6>(w0); a1[6] = cons_m<R, 6>(w1); a2[6] = cons_m<R, 6>(w2);
+    am[7] = cons_m<R, 7>(wm); a0[7] = cons_m<R, 7>(w0); a1[7] = cons_m<R, 7>(w1); a2[7] = cons_m<R, 7>(w2);
+    am[8] = cons_m<R, 8>(wm); a0[8] = cons_m<R, 8>(w0); a1[8] = cons_m<R, 8>(w1); a2[8] = cons_m<R, 8>(w2);
 #pragma unroll
     for (int f = 0; f < 9; ++f) {
-      const R dm = dqq_eval(am.v[f], a0.v[f], a1.v[f], ep);
-      d0[f] = dqq_eval(a0.v[f], a1.v[f], a2.v[f], ep);
-      fdw[f] = flux_core(a0.v[f], a1.v[f], dm, d0[f], cdw, fh);
+      const R dm = dqq_eval(am[f], a0[f], a1[f], ep);
+      d0[f] = dqq_eval(a0[f], a1[f], a2[f], ep);
+      fdw[f] = flux_core(a0[f], a1[f], dm, d0[f], cdw, fh);
     }
   }
-  a0 = a1;
-  a1 = a2;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    w0[m] = w1[m];
+    w1[m] = w2[m];
+  }
   for (int t = 0; t < len; ++t) {
     const int j = jb + t;
-    AvCons<R> an;
+    R wn[9];
     R cc_2;
-    av_xrow_sweep(nf, sh, bc, j + 2, ddx, sc, dt, an, cc_2);
+    av_xrow_sweep(nf, sh, bc, j + 2, ddx, sc, dt, wn, cc_2);
     // row j+3 exists (at most the last ghost row) when j+1 is an output row
     if (t + 1 < len) av_xrow_load(qq, sh, bc, ic, j + 3, k, nf);
     const R cup = R(0.5) * (cc_0 + cc_1);
-    R fup[9];
-#pragma unroll
-    for (int f = 0; f < 9; ++f) {
-      const R d1 = dqq_eval(a0.v[f], a1.v[f], an.v[f], ep);
-      fup[f] = flux_core(a0.v[f], a1.v[f], d0[f], d1, cup, fh);
-      d0[f] = d1;
-    }
+    const R dd = dyi[j];
+    const double Et = cell_et(w0);
+    R o[9], efl;
+    ystep_field<R, 0>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 1>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 2>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 3>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 4>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 5>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 6>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 7>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    ystep_field<R, 8>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
+    cell_energy<R>(Et, efl, o);
     const int c = sh.idx(ic, j, k);
-    R o[9];
-    av_cell_values(a0, fdw, fup, dyi[j], dt, o);
     if (out_lane) {
 #pragma unroll
       for (int f = 0; f < 9; ++f) rslt.q[f][c] = o[f];
       cc_store[c] = cc_0;
     }
 #pragma unroll
-    for (int f = 0; f < 9; ++f) fdw[f] = fup[f];
-    a0 = a1;
-    a1 = an;
+    for (int m = 0; m < 9; ++m) {
+      w0[m] = w1[m];
+      w1[m] = wn[m];
+    }
     cc_0 = cc_1;
     cc_1 = cc_2;
   }
