@@ -191,7 +191,12 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
  * hold num_rays such fields back to back.  face_kind per (axis, side) of a
  * non-periodic axis: 0 untouched, 1 incoming I = S (thermal), 2 incoming
  * I = 0, 3 incoming I = face_value (set_incoming_boundary_on_physical_faces,
- * rt_host.hpp:50-101).  num_rays 24 = Carlson A4, otherwise one +z ray. */
+ * rt_host.hpp:50-101).  num_rays 24 = Carlson A4, otherwise one +z ray.
+ * x_rays_yfast 1: a ray whose dominant axis is x (with more than one cell
+ * on x and on y) keeps its field of rint y-fastest, index
+ * ((k * i_total + i) * j_total + j), so the planes it sweeps are contiguous;
+ * misohip_rt_ray_layout names those rays.  0: every ray x-fastest.  Results
+ * are the same bits either way. */
 typedef struct misohip_rt_config {
   int num_rays;
   int periodic[3];
@@ -199,6 +204,7 @@ typedef struct misohip_rt_config {
   double face_value[6];
   double tolerance;
   int max_iters;
+  int x_rays_yfast;
 } misohip_rt_config;
 
 /* RT::solve: iterate halo exchange, boundary, sweep until the float-
@@ -215,6 +221,13 @@ int misohip_rt_quadrature(int num_rays, double *w, double *mu_x, double *mu_y, d
 int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
                      const void *kappa, const void *S, void *rint, void *rint_old,
                      int *iters, double *max_diff);
+
+/* Bit n of *yfast_mask set: ray n's field of rint is y-fastest under this
+ * grid and configuration (see x_rays_yfast); no device work besides reading
+ * the grid spacings.  No counterpart in the reference (its rint is one
+ * layout). */
+int misohip_rt_ray_layout(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
+                          unsigned *yfast_mask);
 
 /* Q_rad = 4 pi kappa (J - S), J = sum_r w_r I_r, on the RT lattice (no
  * counterpart in the reference, which has no MHD coupling). */

@@ -365,6 +365,8 @@ int misohip_destroy(misohip_ctx *ctx) {
   for (hipEvent_t e : ctx->rec_stop) (void)hipEventDestroy(e);
   if (ctx->d_slot) (void)hipFree(ctx->d_slot);
   if (ctx->h_slot) (void)hipHostFree(ctx->h_slot);
+  if (ctx->rt_kap_y) (void)hipFree(ctx->rt_kap_y);
+  if (ctx->rt_src_y) (void)hipFree(ctx->rt_src_y);
   delete ctx;
   return MISOHIP_OK;
 }

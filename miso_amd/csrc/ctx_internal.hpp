@@ -20,6 +20,10 @@ struct misohip_ctx {
   std::vector<int> rec_kind;
   std::vector<hipEvent_t> rec_start, rec_stop;
   misohip_tiles tile = {2, 8, 256, 5, 256, 1, 64};
+  // y-fastest copies of kappa and S for the RT rays kept y-fastest (grown
+  // on demand, freed with the context)
+  double *rt_kap_y = nullptr, *rt_src_y = nullptr;
+  size_t rt_cells = 0;
 };
 
 namespace misohip_detail {

@@ -42,6 +42,7 @@ struct RtRay {
   double ds;
   double wq;              // quadrature weight
   int id;                 // the ray's slot in the intensity array
+  int tr;                 // 1: intensities (and kappa, S) y-fastest, see rt.hip
 };
 
 struct RtRays {
@@ -49,16 +50,18 @@ struct RtRays {
 };
 
 hipError_t launch_rt_sweep_plane(hipStream_t, double *, const double *, const double *,
-                                 const RtRays &, int, int, long, int);
-hipError_t launch_rt_halo(hipStream_t, double *, int, int, int, int, int, int, unsigned, long,
-                          int);
+                                 const double *, const double *, const RtRays &, int, int, long,
+                                 int);
+hipError_t launch_rt_transpose_xy(hipStream_t, const double *, double *, int, int, int);
+hipError_t launch_rt_halo(hipStream_t, double *, int, int, int, int, int, int, unsigned, unsigned,
+                          long, int);
 hipError_t launch_rt_face(hipStream_t, double *, const double *, int, int, int, int, int, int,
                           int, int, int, double, unsigned, long, int);
 hipError_t launch_rt_maxdiff(hipStream_t, const double *, const double *, const int[3],
-                             const int[3], int, int, long, int, unsigned *);
+                             const int[3], int, int, long, int, unsigned, bool, unsigned *);
 hipError_t launch_rt_qrad(hipStream_t, const double *, const double *, const double *, double *,
-                          const int[3], const int[3], int, int, long, int, const RtRays &);
-
+                          const int[3], const int[3], int, int, long, int, unsigned,
+                          const RtRays &);
 hipError_t launch_rt_source(hipStream_t, const double *, const double *, double *, double *, long,
                             double, double, double);
 hipError_t launch_rt_heat(hipStream_t, const double *, double *, const double *, const Shape &,

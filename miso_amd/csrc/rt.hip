@@ -12,6 +12,11 @@
 // Device layout: rint[(n * Kt + k) * Jt + j) * It + i] (x fastest); kappa
 // and S as the MHD fields.  The RT lattice is ib0 = margin - is ...
 // ib1 = ib0 + size on every axis (rt_host.hpp:127-154).
+// A ray whose dominant axis is x sweeps planes of fixed i, whose nodes lie
+// It apart in that layout; such rays ("transposed" rays, R.tr) keep their
+// intensities y-fastest instead, rint[(n * Kt + k) * It + i) * Jt + j], and
+// read kappa and S from copies in the same layout, so their planes are
+// contiguous as well.  Per node the arithmetic does not change.
 #include "launch.hpp"
 
 namespace misohip {
@@ -26,11 +31,15 @@ __device__ __forceinline__ double rt_integrate(double Iu, double Su, double Sd, 
 
 // plane p of every ray of `rays` (blockIdx.y = entry, R.id = its slot)
 __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict__ I,
-                                                             const double *__restrict__ kap,
-                                                             const double *__restrict__ S,
+                                                             const double *__restrict__ kap_x,
+                                                             const double *__restrict__ S_x,
+                                                             const double *__restrict__ kap_y,
+                                                             const double *__restrict__ S_y,
                                                              RtRays rays, int p, long ray_stride) {
   const RtRay &R = rays.r[blockIdx.y];
   if (p >= R.nplanes) return;
+  const double *__restrict__ kap = R.tr ? kap_y : kap_x;
+  const double *__restrict__ S = R.tr ? S_y : S_x;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= R.nb * R.nc) return;
   const int ib = t % R.nb, ic = t / R.nb;
@@ -47,16 +56,18 @@ __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict_
 
 // periodic halo self-exchange along `axis` over the full transverse extent
 // (rt_host.hpp:236-452): + rays copy node b1 into b0, - rays b0 into b1.
+// rays outside ray_mask are left alone (the other layout's launch does them)
 __global__ __launch_bounds__(256) void rt_halo_kernel(double *__restrict__ I, int axis, int b0,
                                                       int b1, int It, int Jt, int Kt,
-                                                      unsigned pos_mask, long ray_stride,
-                                                      int nrays) {
+                                                      unsigned pos_mask, unsigned ray_mask,
+                                                      long ray_stride, int nrays) {
   const int n1 = axis == 0 ? Jt : It, n2 = axis == 2 ? Jt : Kt;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)n1 * n2 * nrays) return;
   const int u = (int)(t % n1);
   const long r = t / n1;
   const int v = (int)(r % n2), n = (int)(r / n2);
+  if (!((ray_mask >> n) & 1u)) return;
   int i, j, k;
   if (axis == 0) { j = u; k = v; i = 0; }
   else if (axis == 1) { i = u; k = v; j = 0; }
@@ -97,7 +108,8 @@ __global__ __launch_bounds__(256) void rt_maxdiff_kernel(const double *__restric
                                                          const double *__restrict__ old, int lo_i,
                                                          int lo_j, int lo_k, int ni, int nj,
                                                          int nk, int It, int Jt, long ray_stride,
-                                                         int nrays, unsigned *slot) {
+                                                         int nrays, unsigned ray_mask,
+                                                         unsigned *slot) {
   const long n_all = (long)ni * nj * nk * nrays;
   unsigned best = 0u;
   for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n_all;
@@ -108,6 +120,7 @@ __global__ __launch_bounds__(256) void rt_maxdiff_kernel(const double *__restric
     r /= nj;
     const int k = lo_k + (int)(r % nk);
     const int n = (int)(r / nk);
+    if (!((ray_mask >> n) & 1u)) continue;
     const long c = ray_stride * n + (long)(k * Jt + j) * It + i;
     const float d = (float)fabs(I[c] - old[c]);
     best = max(best, __float_as_uint(d));
@@ -129,39 +142,68 @@ __global__ void rt_slot_clear_kernel(unsigned *slot) {
 }
 
 // Q_rad = 4 pi kappa (J - S), J = sum_r w_r I_r in ray order, lattice nodes
+// (a ray in tr_mask holds its intensities y-fastest)
 __global__ __launch_bounds__(256) void rt_qrad_kernel(const double *__restrict__ I,
                                                       const double *__restrict__ kap,
                                                       const double *__restrict__ S,
                                                       double *__restrict__ q, int lo_i, int lo_j,
                                                       int lo_k, int ni, int nj, int nk, int It,
                                                       int Jt, long ray_stride, int nrays,
-                                                      RtRays w) {
+                                                      unsigned tr_mask, RtRays w) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ni * nj * nk) return;
   const int i = lo_i + (int)(t % ni);
   const long r = t / ni;
   const int j = lo_j + (int)(r % nj), k = lo_k + (int)(r / nj);
-  const int c = (k * Jt + j) * It + i;
+  const int c = (k * Jt + j) * It + i, ct = (k * It + i) * Jt + j;
   double J = 0.0;
-  for (int n = 0; n < nrays; ++n) J += w.r[n].wq * I[ray_stride * n + c];
+  for (int n = 0; n < nrays; ++n)
+    J += w.r[n].wq * I[ray_stride * n + (((tr_mask >> n) & 1u) ? ct : c)];
   q[c] = 4.0 * 3.14159265358979323846 * kap[c] * (J - S[c]);
 }
 
 hipError_t launch_rt_sweep_plane(hipStream_t st, double *I, const double *kap, const double *S,
-                                 const RtRays &rays, int nrays, int p, long ray_stride,
-                                 int max_nodes) {
+                                 const double *kap_y, const double *S_y, const RtRays &rays,
+                                 int nrays, int p, long ray_stride, int max_nodes) {
   dim3 grid((max_nodes + 255) / 256, nrays);
-  hipLaunchKernelGGL(rt_sweep_plane_kernel, grid, dim3(256), 0, st, I, kap, S, rays, p,
-                     ray_stride);
+  hipLaunchKernelGGL(rt_sweep_plane_kernel, grid, dim3(256), 0, st, I, kap, S, kap_y, S_y, rays,
+                     p, ray_stride);
+  return hipGetLastError();
+}
+
+// (Kt, Jt, It) x-fastest -> (Kt, It, Jt) y-fastest, 32 x 32 tiles through LDS
+__global__ __launch_bounds__(256) void rt_transpose_xy_kernel(const double *__restrict__ src,
+                                                              double *__restrict__ dst, int It,
+                                                              int Jt, int tiles_i) {
+  __shared__ double tile[32][33];
+  const int ti = blockIdx.x % tiles_i, tj = blockIdx.x / tiles_i, k = blockIdx.y;
+  const long plane = (long)It * Jt * k;
+  for (int r = threadIdx.x >> 5; r < 32; r += 8) {
+    const int i = ti * 32 + (threadIdx.x & 31), j = tj * 32 + r;
+    if (i < It && j < Jt) tile[r][threadIdx.x & 31] = src[plane + (long)j * It + i];
+  }
+  __syncthreads();
+  for (int r = threadIdx.x >> 5; r < 32; r += 8) {
+    const int j = tj * 32 + (threadIdx.x & 31), i = ti * 32 + r;
+    if (i < It && j < Jt) dst[plane + (long)i * Jt + j] = tile[threadIdx.x & 31][r];
+  }
+}
+
+hipError_t launch_rt_transpose_xy(hipStream_t st, const double *src, double *dst, int It, int Jt,
+                                  int Kt) {
+  const int ti = (It + 31) / 32, tj = (Jt + 31) / 32;
+  hipLaunchKernelGGL(rt_transpose_xy_kernel, dim3(ti * tj, Kt), dim3(256), 0, st, src, dst, It,
+                     Jt, ti);
   return hipGetLastError();
 }
 
 hipError_t launch_rt_halo(hipStream_t st, double *I, int axis, int b0, int b1, int It, int Jt,
-                          int Kt, unsigned pos_mask, long ray_stride, int nrays) {
+                          int Kt, unsigned pos_mask, unsigned ray_mask, long ray_stride,
+                          int nrays) {
   const int n1 = axis == 0 ? Jt : It, n2 = axis == 2 ? Jt : Kt;
   const long n = (long)n1 * n2 * nrays;
   hipLaunchKernelGGL(rt_halo_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, I, axis,
-                     b0, b1, It, Jt, Kt, pos_mask, ray_stride, nrays);
+                     b0, b1, It, Jt, Kt, pos_mask, ray_mask, ray_stride, nrays);
   return hipGetLastError();
 }
 
@@ -177,20 +219,20 @@ hipError_t launch_rt_face(hipStream_t st, double *I, const double *S, int axis, 
 
 hipError_t launch_rt_maxdiff(hipStream_t st, const double *I, const double *old, const int lo[3],
                              const int n[3], int It, int Jt, long ray_stride, int nrays,
-                             unsigned *slot) {
-  hipLaunchKernelGGL(rt_slot_clear_kernel, dim3(1), dim3(64), 0, st, slot);
+                             unsigned ray_mask, bool clear, unsigned *slot) {
+  if (clear) hipLaunchKernelGGL(rt_slot_clear_kernel, dim3(1), dim3(64), 0, st, slot);
   hipLaunchKernelGGL(rt_maxdiff_kernel, dim3(2048), dim3(256), 0, st, I, old, lo[0], lo[1], lo[2],
-                     n[0], n[1], n[2], It, Jt, ray_stride, nrays, slot);
+                     n[0], n[1], n[2], It, Jt, ray_stride, nrays, ray_mask, slot);
   return hipGetLastError();
 }
 
 hipError_t launch_rt_qrad(hipStream_t st, const double *I, const double *kap, const double *S,
                           double *q, const int lo[3], const int n[3], int It, int Jt,
-                          long ray_stride, int nrays, const RtRays &w) {
+                          long ray_stride, int nrays, unsigned tr_mask, const RtRays &w) {
   const long cnt = (long)n[0] * n[1] * n[2];
   hipLaunchKernelGGL(rt_qrad_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, I,
                      kap, S, q, lo[0], lo[1], lo[2], n[0], n[1], n[2], It, Jt, ray_stride, nrays,
-                     w);
+                     tr_mask, w);
   return hipGetLastError();
 }
 

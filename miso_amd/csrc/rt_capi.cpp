@@ -6,6 +6,12 @@
 // iteration is: copy rint -> rint_old, periodic halo (z, y, x), incoming
 // boundary on physical faces, one launch per plane for all rays, and the
 // max-change reduction read back to decide convergence.
+//
+// Rays whose dominant axis is x run in the y-fastest frame (node (i, j, k)
+// at (k * i_total + i) * j_total + j): their constants are those of the
+// x-fastest frame with the strides of x and y swapped, and each halo, face
+// and reduction launch comes in two, one per frame, with x and y swapped in
+// the second.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -27,6 +33,7 @@ struct RtSetup {
   int b0[3], b1[3], size[3], st[3];
   long ray_stride;
   unsigned pos_mask[3];
+  unsigned tr_mask;  // rays kept y-fastest
 };
 
 // rt_quadrature.hpp:39-62
@@ -65,7 +72,9 @@ int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, 
   const int sizes[3] = {g->i_size, g->j_size, g->k_size};
   const int margins[3] = {g->i_margin, g->j_margin, g->k_margin};
   const int sts[3] = {g->is, g->js, g->ks};
-  const long strides[3] = {1, It, (long)It * Jt};
+  const long strides_x[3] = {1, It, (long)It * Jt};
+  const long strides_y[3] = {Jt, 1, (long)It * Jt};
+  S.tr_mask = 0u;
   for (int a = 0; a < 3; ++a) {
     S.size[a] = sizes[a];
     S.st[a] = sts[a];
@@ -112,6 +121,9 @@ int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, 
     }
     const int b = dom == 0 ? 1 : 0, c = dom == 2 ? 1 : 2;
     RtRay &R = S.rays.r[n];
+    R.tr = (rt->x_rays_yfast && dom == 0 && sizes[0] > 1 && sizes[1] > 1) ? 1 : 0;
+    if (R.tr) S.tr_mask |= 1u << n;
+    const long *strides = R.tr ? strides_y : strides_x;
     const int lo_b = std::min(beg[b], end_[b]), lo_c = std::min(beg[c], end_[c]);
     int idx[3];
     idx[dom] = beg[dom];
@@ -140,6 +152,14 @@ int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, 
   }
   return MISOHIP_OK;
 }
+
+// (lo, cnt, It, Jt, axis) of the y-fastest frame: x and y swapped
+inline void swap_xy(const int in[3], int out[3]) {
+  out[0] = in[1];
+  out[1] = in[0];
+  out[2] = in[2];
+}
+inline int frame_axis(int a, bool y) { return y && a < 2 ? 1 - a : a; }
 
 int check_rt(const misohip_grid *g, const misohip_rt_config *rt) {
   if (!g || !rt) return MISOHIP_E_ARG;
@@ -174,10 +194,32 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
   const double *src = static_cast<const double *>(S);
   const size_t bytes = sizeof(double) * su.nrays * su.ray_stride;
   const int It = g->i_total, Jt = g->j_total, Kt = g->k_total;
-  int lo[3], cnt[3];
+  int lo[3], cnt[3], lo_y[3], cnt_y[3];
   for (int a = 0; a < 3; ++a) {
     lo[a] = su.b0[a];
     cnt[a] = su.b1[a] - su.b0[a] + 1;
+  }
+  swap_xy(lo, lo_y);
+  swap_xy(cnt, cnt_y);
+  const unsigned all = su.nrays >= 32 ? ~0u : ((1u << su.nrays) - 1u);
+  const unsigned x_mask = all & ~su.tr_mask;
+  const double *kap_y = nullptr, *src_y = nullptr;
+  if (su.tr_mask) {
+    const size_t cells = (size_t)su.ray_stride;
+    if (ctx->rt_cells < cells) {
+      if (ctx->rt_kap_y) MISOHIP_TRY(hipFree(ctx->rt_kap_y));
+      if (ctx->rt_src_y) MISOHIP_TRY(hipFree(ctx->rt_src_y));
+      ctx->rt_kap_y = ctx->rt_src_y = nullptr;
+      ctx->rt_cells = 0;
+      MISOHIP_TRY(hipMalloc(&ctx->rt_kap_y, sizeof(double) * cells));
+      MISOHIP_TRY(hipMalloc(&ctx->rt_src_y, sizeof(double) * cells));
+      ctx->rt_cells = cells;
+    }
+    Timed t_(ctx, MISOHIP_KIND_RT);
+    MISOHIP_TRY(launch_rt_transpose_xy(ctx->stream, kap, ctx->rt_kap_y, It, Jt, Kt));
+    MISOHIP_TRY(launch_rt_transpose_xy(ctx->stream, src, ctx->rt_src_y, It, Jt, Kt));
+    kap_y = ctx->rt_kap_y;
+    src_y = ctx->rt_src_y;
   }
   // Sweep phases: plane ranges over which the set of rays that still have
   // planes is constant; each phase launches only those rays, sized for
@@ -215,9 +257,15 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
       Timed t_(ctx, MISOHIP_KIND_RT);
       MISOHIP_TRY(hipMemcpyAsync(old, I, bytes, hipMemcpyDeviceToDevice, ctx->stream));
       for (int a = 2; a >= 0; --a)
-        if (rt->periodic[a] && su.size[a] > 1)
-          MISOHIP_TRY(launch_rt_halo(ctx->stream, I, a, su.b0[a], su.b1[a], It, Jt, Kt,
-                                     su.pos_mask[a], su.ray_stride, su.nrays));
+        if (rt->periodic[a] && su.size[a] > 1) {
+          if (x_mask)
+            MISOHIP_TRY(launch_rt_halo(ctx->stream, I, a, su.b0[a], su.b1[a], It, Jt, Kt,
+                                       su.pos_mask[a], x_mask, su.ray_stride, su.nrays));
+          if (su.tr_mask)
+            MISOHIP_TRY(launch_rt_halo(ctx->stream, I, frame_axis(a, true), su.b0[a], su.b1[a],
+                                       Jt, It, Kt, su.pos_mask[a], su.tr_mask, su.ray_stride,
+                                       su.nrays));
+        }
       for (int a = 0; a < 3; ++a)
         for (int side = 0; side < 2; ++side) {
           const int kind = rt->face_kind[2 * a + side];
@@ -227,18 +275,28 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
             const double mu = su.mu[a][n];
             if (side == 0 ? mu > 0.0 : mu < 0.0) in_mask |= 1u << n;
           }
-          const int t0 = a == 0 ? 1 : 0, t1 = a == 2 ? 1 : 2;
-          MISOHIP_TRY(launch_rt_face(ctx->stream, I, src, a, side == 0 ? su.b0[a] : su.b1[a],
-                                     lo[t0], lo[t1], cnt[t0], cnt[t1], It, Jt, kind,
-                                     rt->face_value[2 * a + side], in_mask, su.ray_stride,
-                                     su.nrays));
+          for (int y = 0; y < 2; ++y) {
+            const unsigned m = in_mask & (y ? su.tr_mask : x_mask);
+            if (!m) continue;
+            const int fa = frame_axis(a, y);
+            const int t0 = fa == 0 ? 1 : 0, t1 = fa == 2 ? 1 : 2;
+            const int *fl = y ? lo_y : lo, *fc = y ? cnt_y : cnt;
+            MISOHIP_TRY(launch_rt_face(ctx->stream, I, y ? src_y : src, fa,
+                                       side == 0 ? su.b0[a] : su.b1[a], fl[t0], fl[t1], fc[t0],
+                                       fc[t1], y ? Jt : It, y ? It : Jt, kind,
+                                       rt->face_value[2 * a + side], m, su.ray_stride,
+                                       su.nrays));
+          }
         }
       for (const Phase &ph : phases)
         for (int p = ph.p0; p < ph.p1; ++p)
-          MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, ph.rays, ph.n, p,
-                                            su.ray_stride, ph.max_nodes));
+          MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, kap_y, src_y, ph.rays,
+                                            ph.n, p, su.ray_stride, ph.max_nodes));
       MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo, cnt, It, Jt, su.ray_stride,
-                                    su.nrays, ctx->d_slot + 1));
+                                    su.nrays, x_mask, true, ctx->d_slot + 1));
+      if (su.tr_mask)
+        MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo_y, cnt_y, Jt, It, su.ray_stride,
+                                      su.nrays, su.tr_mask, false, ctx->d_slot + 1));
     }
     MISOHIP_TRY(hipMemcpyAsync(ctx->h_slot + 1, ctx->d_slot + 1, sizeof(unsigned),
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -273,7 +331,18 @@ int misohip_rt_qrad(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_co
   MISOHIP_TRY(launch_rt_qrad(ctx->stream, static_cast<const double *>(rint),
                              static_cast<const double *>(kappa), static_cast<const double *>(S),
                              static_cast<double *>(qrad), lo, cnt, g->i_total, g->j_total,
-                             su.ray_stride, su.nrays, su.rays));
+                             su.ray_stride, su.nrays, su.tr_mask, su.rays));
+  return MISOHIP_OK;
+}
+
+int misohip_rt_ray_layout(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
+                          unsigned *yfast_mask) {
+  if (!ctx || !yfast_mask) return MISOHIP_E_ARG;
+  if (int rc = check_rt(g, rt)) return rc;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  RtSetup su;
+  if (int rc = setup(ctx, g, rt, su)) return rc;
+  *yfast_mask = su.tr_mask;
   return MISOHIP_OK;
 }
 

@@ -51,7 +51,8 @@ class MisohipTiles(C.Structure):
 
 class MisohipRtConfig(C.Structure):
     _fields_ = [("num_rays", C.c_int), ("periodic", C.c_int * 3), ("face_kind", C.c_int * 6),
-                ("face_value", C.c_double * 6), ("tolerance", C.c_double), ("max_iters", C.c_int)]
+                ("face_value", C.c_double * 6), ("tolerance", C.c_double), ("max_iters", C.c_int),
+                ("x_rays_yfast", C.c_int)]
 
 
 class MisohipWork(C.Structure):
@@ -93,6 +94,7 @@ SIGNATURES = {
                          C.c_void_p, C.c_void_p, _P(C.c_int), _P(C.c_double)],
     "misohip_rt_qrad": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,
                         C.c_void_p, C.c_void_p],
+    "misohip_rt_ray_layout": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), _P(C.c_uint)],
     "misohip_rt_source": [C.c_void_p, _P(MisohipGrid), _P(MisohipFields), C.c_double, C.c_double,
                           C.c_double,
                           C.c_void_p, C.c_void_p],

@@ -171,7 +171,8 @@ class RunFiles:
                 np.asarray(a).astype(ft).tofile(f)
             for t in (rt.src_func, rt.abs_coeff):
                 np.ascontiguousarray(t.permute(2, 1, 0).cpu().numpy()).astype(ft).tofile(f)
-            np.ascontiguousarray(rt.rint[:n].permute(0, 3, 2, 1).cpu().numpy()).astype(ft).tofile(f)
+            rint = rt.intensities()[:n]
+            np.ascontiguousarray(rint.permute(0, 3, 2, 1).cpu().numpy()).astype(ft).tofile(f)
         return name
 
     def load_rt(self, shape, dtype=np.float64) -> dict:

@@ -3,9 +3,13 @@ over the GPU solver in libmisohip.so.
 
 RT(grid, num_rays) owns the device arrays of the reference's struct --
 rint and rint_old (num_rays fields), src_func and abs_coeff -- laid out
-x-fastest like the MHD fields.  solve() is RT::solve with the boundary set
-given per face (periodic axes are self-exchanged); qrad() is the radiative
-heating rate Q_rad = 4 pi kappa (J - S) this backend adds.
+x-fastest like the MHD fields, except that with x_rays_yfast (the default)
+a ray whose dominant axis is x keeps its field of rint y-fastest, so its
+sweep planes are contiguous; `yfast_mask` names those rays as of the last
+solve, and intensities() gives every ray as (k, j, i).
+solve() is RT::solve with the boundary set given per face (periodic axes
+are self-exchanged); qrad() is the radiative heating rate
+Q_rad = 4 pi kappa (J - S) this backend adds.
 """
 from __future__ import annotations
 
@@ -29,9 +33,11 @@ class RT:
         self.src_func = torch.zeros(shp, dtype=torch.float64, device=dev)
         self.abs_coeff = torch.zeros(shp, dtype=torch.float64, device=dev)
         self.qrad_field = torch.zeros(shp, dtype=torch.float64, device=dev)
+        self.yfast_mask = 0
 
     def config(self, periodic=(True, True, False), face_kind=(0, 0, 0, 0, 1, 2),
-               face_value=(0.0,) * 6, tolerance=1e-5, max_iters=100) -> hip.MisohipRtConfig:
+               face_value=(0.0,) * 6, tolerance=1e-5, max_iters=100,
+               x_rays_yfast=True) -> hip.MisohipRtConfig:
         c = hip.MisohipRtConfig()
         c.num_rays = self.num_rays
         for a in range(3):
@@ -41,11 +47,34 @@ class RT:
             c.face_value[f] = float(face_value[f])
         c.tolerance = float(tolerance)
         c.max_iters = int(max_iters)
+        c.x_rays_yfast = 1 if x_rays_yfast else 0
         return c
+
+    def ray_layout(self, ctx, **kw) -> int:
+        """Bit mask of the rays whose field of rint is y-fastest under the
+        configuration `kw` (as given to solve)."""
+        m = C.c_uint()
+        hip.call("misohip_rt_ray_layout", ctx.handle, C.byref(self.grid.c_struct()),
+                 C.byref(self.config(**kw)), C.byref(m))
+        return int(m.value)
+
+    def intensities(self):
+        """rint with every ray's field as (k, j, i), x fastest (a copy when
+        the last solve kept rays y-fastest)."""
+        mask = self.yfast_mask
+        if not mask:
+            return self.rint
+        out = self.rint.clone()
+        kt, jt, it = self.rint.shape[1:]
+        for n in range(self.num_rays):
+            if (mask >> n) & 1:
+                out[n] = self.rint[n].reshape(kt, it, jt).permute(0, 2, 1)
+        return out
 
     def solve(self, ctx, **kw):
         """RT::solve; returns (iterations, last max change)."""
         cfg = self.config(**kw)
+        self.yfast_mask = self.ray_layout(ctx, **kw)
         it, md = C.c_int(), C.c_double()
         hip.call("misohip_rt_solve", ctx.handle, C.byref(self.grid.c_struct()), C.byref(cfg),
                  C.c_void_p(self.abs_coeff.data_ptr()), C.c_void_p(self.src_func.data_ptr()),

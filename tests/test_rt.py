@@ -80,7 +80,7 @@ def _gpu_rt(ctx, og, kap, S, num_rays, **kw):
     it, md = rt.solve(ctx, **kw)
     q = rt.qrad(ctx, **kw)
     torch.cuda.synchronize()
-    I = rt.rint.permute(0, 3, 2, 1).cpu().numpy()
+    I = rt.intensities().permute(0, 3, 2, 1).cpu().numpy()
     return it, md, I, RT.to_host_field(q)
 
 
@@ -153,3 +153,49 @@ def test_gpu_thin_slab_per_ray_sweep(ctx, sizes):
     assert np.abs(I - I_ref).max() <= 1e-12 * scale
     lat = (slice(1, -2), slice(1, -2), slice(1, -2))
     assert np.abs(q[lat] - q_ref[lat]).max() <= 1e-11 * np.abs(q_ref[lat]).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sizes,periodic,faces", [
+    ((24, 20, 16), (True, True, False), (0, 0, 0, 0, 1, 2)),
+    ((20, 28, 12), (False, False, False), (1, 3, 2, 1, 1, 2)),
+    ((36, 30, 1), (False, True, True), (3, 1, 0, 0, 0, 0)),
+])
+def test_gpu_x_rays_yfast_same_bits(ctx, sizes, periodic, faces):
+    """Keeping the x-dominant rays y-fastest changes where their intensities
+    live, not what they are: iterations, last change, every intensity and
+    Q_rad equal bit for bit to the all-x-fastest solve, over periodic and
+    face boundaries of every kind on x and y."""
+    import torch
+
+    from miso_amd import mhd
+    from miso_amd.rt import RT
+    g = mhd.Grid(*sizes, 2, bounds=((-0.5, 0.5), (-0.4, 0.6), (-0.5, 0.5)), device="cuda:0")
+    rng = np.random.default_rng(5)
+    shp = (g.k_total, g.j_total, g.i_total)
+    kap = torch.from_numpy(5.0 * (1 + 0.2 * rng.random(shp))).to("cuda:0")
+    S = torch.from_numpy(1 + 0.3 * rng.random(shp)).to("cuda:0")
+    kw = dict(periodic=periodic, face_kind=faces, face_value=(0.8, 1.3, 0, 0, 0, 0),
+              tolerance=1e-10, max_iters=200)
+    out = []
+    for yf in (False, True):
+        rt = RT(g, 24)
+        rt.abs_coeff.copy_(kap)
+        rt.src_func.copy_(S)
+        it, md = rt.solve(ctx, x_rays_yfast=yf, **kw)
+        q = rt.qrad(ctx, x_rays_yfast=yf, **kw).clone()
+        torch.cuda.synchronize()
+        out.append((it, md, rt.yfast_mask, rt.intensities().cpu(), q.cpu()))
+    (it0, md0, m0, I0, q0), (it1, md1, m1, I1, q1) = out
+    assert m0 == 0
+    # the rays kept y-fastest are those whose shortest step to the next
+    # plane is along x (one cell on z: no step along z)
+    w, mx, my, mz = rt.quadrature()
+    d = [1.0 / sizes[0], 1.0 / sizes[1], 1.0 / sizes[2] if sizes[2] > 1 else 1e30]
+    dom = [int(np.argmin([d[a] / max(abs(m[n]), 1e-30) for a, m in enumerate((mx, my, mz))]))
+           for n in range(24)]
+    assert m1 == sum(1 << n for n in range(24) if dom[n] == 0)
+    assert m1 & 0xFF == 0xFF
+    assert (it0, md0) == (it1, md1)
+    assert torch.equal(I0, I1)
+    assert torch.equal(q0, q1)
