@@ -195,8 +195,11 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
  * x_rays_yfast 1: a ray whose dominant axis is x (with more than one cell
  * on x and on y) keeps its field of rint y-fastest, index
  * ((k * i_total + i) * j_total + j), so the planes it sweeps are contiguous;
- * misohip_rt_ray_layout names those rays.  0: every ray x-fastest.  Results
- * are the same bits either way. */
+ * misohip_rt_ray_layout names those rays.  0: every ray x-fastest.
+ * sweep_planes 2..16: a ray whose shorter transverse axis (c) has at most
+ * 32 lattice nodes (the x and y rays of a thin slab) advances that many
+ * planes per launch; 0 or 1: one launch per plane for every ray.  Results
+ * are the same bits under every setting of both. */
 typedef struct misohip_rt_config {
   int num_rays;
   int periodic[3];
@@ -205,6 +208,7 @@ typedef struct misohip_rt_config {
   double tolerance;
   int max_iters;
   int x_rays_yfast;
+  int sweep_planes;
 } misohip_rt_config;
 
 /* RT::solve: iterate halo exchange, boundary, sweep until the float-

@@ -37,6 +37,7 @@ struct RtRay {
   int da, db, dc;         // linear step of the node along a (signed), b, c
   int ua, ub, uc;         // upwind offsets (linear) along a, b, c (0 if degenerate)
   int nb, nc;             // transverse node counts
+  int sb, sc;             // upwind step along b, c in nodes (+1, -1, or 0 if degenerate)
   int nplanes;
   double w00, w10, w01, w11;  // bilinear weights
   double ds;
@@ -53,6 +54,13 @@ hipError_t launch_rt_sweep_plane(hipStream_t, double *, const double *, const do
                                  const double *, const double *, const RtRays &, int, int, long,
                                  int);
 hipError_t launch_rt_transpose_xy(hipStream_t, const double *, double *, int, int, int);
+// several planes per launch for rays whose c axis has at most RT_MULTI_ROWS - 1 nodes
+constexpr int RT_MULTI_COLS = 32;  // b nodes a workgroup owns
+constexpr int RT_MULTI_PLANES = 16;
+constexpr int RT_MULTI_ROWS = 33;
+hipError_t launch_rt_sweep_multi(hipStream_t, double *, const double *, const double *,
+                                 const double *, const double *, const RtRays &, int, int, int,
+                                 long, int);
 hipError_t launch_rt_halo(hipStream_t, double *, int, int, int, int, int, int, unsigned, unsigned,
                           long, int);
 hipError_t launch_rt_face(hipStream_t, double *, const double *, int, int, int, int, int, int,

@@ -29,6 +29,19 @@ __device__ __forceinline__ double rt_integrate(double Iu, double Su, double Sd, 
   return Iu * exp_dtau + (u0 - u1) * Su + u1 * Sd;
 }
 
+// one node from its four upwind neighbours (00: same transverse position,
+// 10: one step back along b, 01: along c, 11: both) and its own kappa, S
+__device__ __forceinline__ double rt_node(const RtRay &R, double I00, double I10, double I01,
+                                          double I11, double S00, double S10, double S01,
+                                          double S11, double K00, double K10, double K01,
+                                          double K11, double kc, double sc) {
+  const double iu = R.w00 * I00 + R.w10 * I10 + R.w01 * I01 + R.w11 * I11;
+  const double su = R.w00 * S00 + R.w10 * S10 + R.w01 * S01 + R.w11 * S11;
+  const double ku = R.w00 * K00 + R.w10 * K10 + R.w01 * K01 + R.w11 * K11;
+  const double dtau = 0.5 * (ku + kc) * R.ds;
+  return rt_integrate(iu, su, sc, dtau);
+}
+
 // plane p of every ray of `rays` (blockIdx.y = entry, R.id = its slot)
 __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict__ I,
                                                              const double *__restrict__ kap_x,
@@ -47,11 +60,136 @@ __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict_
   const int c00 = c - R.ua;
   const int c10 = c00 - R.ub, c01 = c00 - R.uc, c11 = c00 - R.ub - R.uc;
   double *Ir = I + ray_stride * R.id;
-  const double iu = R.w00 * Ir[c00] + R.w10 * Ir[c10] + R.w01 * Ir[c01] + R.w11 * Ir[c11];
-  const double su = R.w00 * S[c00] + R.w10 * S[c10] + R.w01 * S[c01] + R.w11 * S[c11];
-  const double ku = R.w00 * kap[c00] + R.w10 * kap[c10] + R.w01 * kap[c01] + R.w11 * kap[c11];
-  const double dtau = 0.5 * (ku + kap[c]) * R.ds;
-  Ir[c] = rt_integrate(iu, su, S[c], dtau);
+  Ir[c] = rt_node(R, Ir[c00], Ir[c10], Ir[c01], Ir[c11], S[c00], S[c10], S[c01], S[c11], kap[c00],
+                  kap[c10], kap[c01], kap[c11], kap[c], S[c]);
+}
+
+// Planes p0 .. p0 + P - 1 of every ray of `rays` in one launch, for rays
+// whose c axis is short (nc < RT_MULTI_ROWS; a thin slab's x and y rays).
+// Node coordinates are mirrored so that upwind is always one step back:
+// u = ib (sb > 0) or nb - 1 - ib, likewise v.  A workgroup owns the
+// RT_MULTI_COLS columns u in [U0, U0 + W) of a ray over the whole c range,
+// and computes plane by plane the cone u in [U0 - P, U0 + W) of nodes
+// that its last plane depends on; the plane is kept in LDS, column u = -1
+// and row v = -1 (the upwind boundary nodes, which the sweep never writes)
+// are read from memory.  Every node is computed by rt_node from the same
+// values as in rt_sweep_plane_kernel, so the results are the same bits;
+// cone nodes outside the owned columns are recomputed, not stored, and the
+// cone's stale edge (u below U0 - P + s + 1 at step s) never reaches them.
+constexpr int RT_MULTI_THREADS = 1024;
+
+__global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double *__restrict__ I,
+                                                             const double *__restrict__ kap_x,
+                                                             const double *__restrict__ S_x,
+                                                             const double *__restrict__ kap_y,
+                                                             const double *__restrict__ S_y,
+                                                             RtRays rays, int p0, int P,
+                                                             long ray_stride) {
+  constexpr int W = RT_MULTI_COLS, X = RT_MULTI_COLS + RT_MULTI_PLANES, ROWS = RT_MULTI_ROWS;
+  constexpr int NT = RT_MULTI_THREADS, CPT = (X * ROWS + NT - 1) / NT;
+  __shared__ double Il[ROWS][X], Kl[ROWS][X], Sl[ROWS][X];
+  const RtRay &R = rays.r[blockIdx.y];
+  const int U0 = blockIdx.x * W;
+  // uniform over the workgroup
+  if (p0 >= R.nplanes || U0 >= R.nb || R.nc + 1 > ROWS || P < 1 || P > RT_MULTI_PLANES) return;
+  const double *__restrict__ kap = R.tr ? kap_y : kap_x;
+  const double *__restrict__ S = R.tr ? S_y : S_x;
+  double *Ir = I + ray_stride * R.id;
+  const int ext = W + P, ncell = ext * (R.nc + 1);
+  const int pend = min(p0 + P, R.nplanes);
+  // per cell: memory index at plane 0, role (0 none, 1 boundary, read every
+  // plane; 2 computed; 3 computed and owned; 4 the cone's far edge e = 0,
+  // read for the plane before p0 only: never valid after it, and its node
+  // may be written by another workgroup meanwhile), LDS position
+  int off[CPT], role[CPT], ce[CPT], cr[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int idx = threadIdx.x + q * NT;
+    const int e = idx % ext, r = idx / ext;
+    const int u = U0 - P + e, v = r - 1;
+    ce[q] = e;
+    cr[q] = r;
+    role[q] = 0;
+    off[q] = 0;
+    if (idx >= ncell || u < -1 || u >= R.nb) continue;
+    const int ib = R.sb > 0 ? u : R.nb - 1 - u;
+    const int ic = R.sc > 0 ? v : R.nc - 1 - v;
+    off[q] = R.start + ib * R.db + ic * R.dc;
+    if (u == -1 || v == -1) role[q] = 1;
+    else if (e == 0) role[q] = 4;
+    else role[q] = (u >= U0 && u < U0 + W) ? 3 : 2;
+  }
+  // the plane before p0
+#pragma unroll
+  for (int q = 0; q < CPT; ++q)
+    if (role[q]) {
+      const int c = off[q] + (p0 - 1) * R.da;
+      Il[cr[q]][ce[q]] = Ir[c];
+      Kl[cr[q]][ce[q]] = kap[c];
+      Sl[cr[q]][ce[q]] = S[c];
+    }
+  double kc[CPT], sc[CPT], bnd[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    kc[q] = sc[q] = bnd[q] = 0.0;
+    if (role[q] && role[q] < 4) {
+      const int c = off[q] + p0 * R.da;
+      kc[q] = kap[c];
+      sc[q] = S[c];
+      if (role[q] == 1) bnd[q] = Ir[c];
+    }
+  }
+  __syncthreads();
+  for (int p = p0; p < pend; ++p) {
+    // the next plane's loads are in flight while this plane is computed
+    double kn[CPT], sn[CPT], bn[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      kn[q] = sn[q] = bn[q] = 0.0;
+      if (p + 1 < pend && role[q] && role[q] < 4) {
+        const int c = off[q] + (p + 1) * R.da;
+        kn[q] = kap[c];
+        sn[q] = S[c];
+        if (role[q] == 1) bn[q] = Ir[c];
+      }
+    }
+    double in[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      in[q] = bnd[q];
+      if (role[q] == 2 || role[q] == 3) {
+        const int r = cr[q], e = ce[q];
+        in[q] = rt_node(R, Il[r][e], Il[r][e - 1], Il[r - 1][e], Il[r - 1][e - 1], Sl[r][e],
+                        Sl[r][e - 1], Sl[r - 1][e], Sl[r - 1][e - 1], Kl[r][e], Kl[r][e - 1],
+                        Kl[r - 1][e], Kl[r - 1][e - 1], kc[q], sc[q]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (role[q] && role[q] < 4) {
+        Il[cr[q]][ce[q]] = in[q];
+        Kl[cr[q]][ce[q]] = kc[q];
+        Sl[cr[q]][ce[q]] = sc[q];
+        if (role[q] == 3) Ir[off[q] + p * R.da] = in[q];
+      }
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      kc[q] = kn[q];
+      sc[q] = sn[q];
+      bnd[q] = bn[q];
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t launch_rt_sweep_multi(hipStream_t st, double *I, const double *kap, const double *S,
+                                 const double *kap_y, const double *S_y, const RtRays &rays,
+                                 int nrays, int p0, int P, long ray_stride, int max_nb) {
+  dim3 grid((max_nb + RT_MULTI_COLS - 1) / RT_MULTI_COLS, nrays);
+  hipLaunchKernelGGL(rt_sweep_multi_kernel, grid, dim3(RT_MULTI_THREADS), 0, st, I, kap, S, kap_y,
+                     S_y, rays, p0, P, ray_stride);
+  return hipGetLastError();
 }
 
 // periodic halo self-exchange along `axis` over the full transverse extent

@@ -138,6 +138,8 @@ int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, 
     R.uc = (int)(t[c] * sts[c] * strides[c]);
     R.nb = std::abs(end_[b] - beg[b]) + 1;
     R.nc = std::abs(end_[c] - beg[c]) + 1;
+    R.sb = t[b] * sts[b];
+    R.sc = t[c] * sts[c];
     R.nplanes = std::abs(end_[dom] - beg[dom]) + 1;
     const double cx = cc[b], cy = cc[c];
     R.w00 = (1.0 - cx) * (1.0 - cy);
@@ -165,6 +167,7 @@ int check_rt(const misohip_grid *g, const misohip_rt_config *rt) {
   if (!g || !rt) return MISOHIP_E_ARG;
   if (rt->num_rays != 24 && rt->num_rays != 1) return MISOHIP_E_ARG;
   if (rt->max_iters < 1) return MISOHIP_E_ARG;
+  if (rt->sweep_planes < 0 || rt->sweep_planes > RT_MULTI_PLANES) return MISOHIP_E_ARG;
   const double n = (double)g->i_total * g->j_total * g->k_total;
   if (n >= 2147483647.0) return MISOHIP_E_RANGE;  // int indices within one ray
   if (!g->dx || !g->dy || !g->dz) return MISOHIP_E_ARG;
@@ -221,6 +224,22 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
     kap_y = ctx->rt_kap_y;
     src_y = ctx->rt_src_y;
   }
+  // Rays with a short c axis advance several planes per launch
+  // (rt_sweep_multi_kernel); the others go one plane per launch.
+  const int P = rt->sweep_planes;
+  RtRays mrays;
+  int nm = 0, m_planes = 0, m_nb = 0;
+  std::vector<int> plane_rays;
+  for (int n = 0; n < su.nrays; ++n) {
+    const RtRay &R = su.rays.r[n];
+    if (P >= 2 && R.sb != 0 && R.sc != 0 && R.ua == R.da && R.nc + 1 <= RT_MULTI_ROWS) {
+      mrays.r[nm++] = R;
+      m_planes = std::max(m_planes, R.nplanes);
+      m_nb = std::max(m_nb, R.nb);
+    } else {
+      plane_rays.push_back(n);
+    }
+  }
   // Sweep phases: plane ranges over which the set of rays that still have
   // planes is constant; each phase launches only those rays, sized for
   // their largest plane (a thin slab's z rays end long before its x/y rays).
@@ -231,7 +250,7 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
   std::vector<Phase> phases;
   {
     std::vector<int> ends;
-    for (int n = 0; n < su.nrays; ++n) ends.push_back(su.rays.r[n].nplanes);
+    for (int n : plane_rays) ends.push_back(su.rays.r[n].nplanes);
     std::sort(ends.begin(), ends.end());
     int p0 = 0;
     for (int e : ends) {
@@ -241,7 +260,7 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
       ph.p1 = e;
       ph.n = 0;
       ph.max_nodes = 0;
-      for (int n = 0; n < su.nrays; ++n)
+      for (int n : plane_rays)
         if (su.rays.r[n].nplanes > p0) {
           ph.rays.r[ph.n++] = su.rays.r[n];
           ph.max_nodes = std::max(ph.max_nodes, su.rays.r[n].nb * su.rays.r[n].nc);
@@ -292,6 +311,9 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
         for (int p = ph.p0; p < ph.p1; ++p)
           MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, kap_y, src_y, ph.rays,
                                             ph.n, p, su.ray_stride, ph.max_nodes));
+      for (int p0 = 0; p0 < m_planes; p0 += P)
+        MISOHIP_TRY(launch_rt_sweep_multi(ctx->stream, I, kap, src, kap_y, src_y, mrays, nm, p0,
+                                          P, su.ray_stride, m_nb));
       MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo, cnt, It, Jt, su.ray_stride,
                                     su.nrays, x_mask, true, ctx->d_slot + 1));
       if (su.tr_mask)

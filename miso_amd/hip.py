@@ -52,7 +52,7 @@ class MisohipTiles(C.Structure):
 class MisohipRtConfig(C.Structure):
     _fields_ = [("num_rays", C.c_int), ("periodic", C.c_int * 3), ("face_kind", C.c_int * 6),
                 ("face_value", C.c_double * 6), ("tolerance", C.c_double), ("max_iters", C.c_int),
-                ("x_rays_yfast", C.c_int)]
+                ("x_rays_yfast", C.c_int), ("sweep_planes", C.c_int)]
 
 
 class MisohipWork(C.Structure):

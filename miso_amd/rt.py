@@ -37,7 +37,7 @@ class RT:
 
     def config(self, periodic=(True, True, False), face_kind=(0, 0, 0, 0, 1, 2),
                face_value=(0.0,) * 6, tolerance=1e-5, max_iters=100,
-               x_rays_yfast=True) -> hip.MisohipRtConfig:
+               x_rays_yfast=True, sweep_planes=16) -> hip.MisohipRtConfig:
         c = hip.MisohipRtConfig()
         c.num_rays = self.num_rays
         for a in range(3):
@@ -48,6 +48,7 @@ class RT:
         c.tolerance = float(tolerance)
         c.max_iters = int(max_iters)
         c.x_rays_yfast = 1 if x_rays_yfast else 0
+        c.sweep_planes = int(sweep_planes)
         return c
 
     def ray_layout(self, ctx, **kw) -> int:

@@ -160,12 +160,16 @@ def test_gpu_thin_slab_per_ray_sweep(ctx, sizes):
     ((24, 20, 16), (True, True, False), (0, 0, 0, 0, 1, 2)),
     ((20, 28, 12), (False, False, False), (1, 3, 2, 1, 1, 2)),
     ((36, 30, 1), (False, True, True), (3, 1, 0, 0, 0, 0)),
+    ((72, 40, 6), (True, True, False), (0, 0, 0, 0, 1, 2)),
+    ((40, 70, 31), (False, True, False), (2, 0, 0, 0, 3, 1)),
 ])
-def test_gpu_x_rays_yfast_same_bits(ctx, sizes, periodic, faces):
-    """Keeping the x-dominant rays y-fastest changes where their intensities
-    live, not what they are: iterations, last change, every intensity and
-    Q_rad equal bit for bit to the all-x-fastest solve, over periodic and
-    face boundaries of every kind on x and y."""
+def test_gpu_ray_layouts_and_multi_plane_sweeps_same_bits(ctx, sizes, periodic, faces):
+    """Keeping the x-dominant rays y-fastest, and sweeping the rays of a
+    short c axis several planes per launch, change where intensities live
+    and how the sweep is launched, not what it computes: iterations, last
+    change, every intensity and Q_rad equal bit for bit to the all
+    x-fastest, one-plane-per-launch solve, over periodic and face
+    boundaries of every kind on x and y."""
     import torch
 
     from miso_amd import mhd
@@ -178,15 +182,15 @@ def test_gpu_x_rays_yfast_same_bits(ctx, sizes, periodic, faces):
     kw = dict(periodic=periodic, face_kind=faces, face_value=(0.8, 1.3, 0, 0, 0, 0),
               tolerance=1e-10, max_iters=200)
     out = []
-    for yf in (False, True):
+    for yf, planes in ((False, 0), (True, 0), (True, 8), (False, 5), (True, 16), (True, 2)):
         rt = RT(g, 24)
         rt.abs_coeff.copy_(kap)
         rt.src_func.copy_(S)
-        it, md = rt.solve(ctx, x_rays_yfast=yf, **kw)
-        q = rt.qrad(ctx, x_rays_yfast=yf, **kw).clone()
+        it, md = rt.solve(ctx, x_rays_yfast=yf, sweep_planes=planes, **kw)
+        q = rt.qrad(ctx, x_rays_yfast=yf, sweep_planes=planes, **kw).clone()
         torch.cuda.synchronize()
         out.append((it, md, rt.yfast_mask, rt.intensities().cpu(), q.cpu()))
-    (it0, md0, m0, I0, q0), (it1, md1, m1, I1, q1) = out
+    it0, md0, m0, I0, q0 = out[0]
     assert m0 == 0
     # the rays kept y-fastest are those whose shortest step to the next
     # plane is along x (one cell on z: no step along z)
@@ -194,8 +198,9 @@ def test_gpu_x_rays_yfast_same_bits(ctx, sizes, periodic, faces):
     d = [1.0 / sizes[0], 1.0 / sizes[1], 1.0 / sizes[2] if sizes[2] > 1 else 1e30]
     dom = [int(np.argmin([d[a] / max(abs(m[n]), 1e-30) for a, m in enumerate((mx, my, mz))]))
            for n in range(24)]
-    assert m1 == sum(1 << n for n in range(24) if dom[n] == 0)
-    assert m1 & 0xFF == 0xFF
-    assert (it0, md0) == (it1, md1)
-    assert torch.equal(I0, I1)
-    assert torch.equal(q0, q1)
+    assert out[1][2] == sum(1 << n for n in range(24) if dom[n] == 0)
+    assert out[1][2] & 0xFF == 0xFF
+    for it1, md1, _, I1, q1 in out[1:]:
+        assert (it0, md0) == (it1, md1)
+        assert torch.equal(I0, I1)
+        assert torch.equal(q0, q1)
