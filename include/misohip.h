@@ -211,10 +211,6 @@ typedef struct misohip_rt_config {
   int sweep_planes;
 } misohip_rt_config;
 
-/* RT::solve: iterate halo exchange, boundary, sweep until the float-
- * narrowed max change is below tolerance (rt_host.hpp:632-651).  Writes the
- * iteration count and the last max change; synchronises once per
- * iteration. */
 /* The angular quadrature the solver uses (AngularQuadrature,
  * miso/include/miso/rt_quadrature.hpp:14-63): for num_rays 24 the Carlson A4
  * set, otherwise one +z ray.  Writes the weights and direction cosines of
@@ -222,6 +218,12 @@ typedef struct misohip_rt_config {
  * work. */
 int misohip_rt_quadrature(int num_rays, double *w, double *mu_x, double *mu_y, double *mu_z);
 
+/* RT::solve: iterate halo exchange, boundary, sweep until the float-
+ * narrowed max change is below tolerance (rt_host.hpp:632-651).  Writes the
+ * iteration count and the last max change; synchronises once per
+ * iteration.  rint_old is the solver's scratch: the change is taken inside
+ * the sweep, and rint_old keeps only each ray's upwind lattice faces as
+ * they were at the start of the last iteration. */
 int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt,
                      const void *kappa, const void *S, void *rint, void *rint_old,
                      int *iters, double *max_diff);

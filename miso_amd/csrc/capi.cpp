@@ -367,6 +367,7 @@ int misohip_destroy(misohip_ctx *ctx) {
   if (ctx->h_slot) (void)hipHostFree(ctx->h_slot);
   if (ctx->rt_kap_y) (void)hipFree(ctx->rt_kap_y);
   if (ctx->rt_src_y) (void)hipFree(ctx->rt_src_y);
+  if (ctx->rt_parts) (void)hipFree(ctx->rt_parts);
   delete ctx;
   return MISOHIP_OK;
 }

@@ -24,6 +24,7 @@ struct misohip_ctx {
   // on demand, freed with the context)
   double *rt_kap_y = nullptr, *rt_src_y = nullptr;
   size_t rt_cells = 0;
+  unsigned *rt_parts = nullptr;  // the RT change-reduction slots
 };
 
 namespace misohip_detail {

@@ -50,9 +50,18 @@ struct RtRays {
   RtRay r[24];
 };
 
+// lattice (lo, n per axis), ray directions and layouts for rt_faces_kernel
+struct RtFaces {
+  int lo[3], n[3];
+  unsigned pos[3];  // bit r: ray r goes + along the axis
+  unsigned tr;      // bit r: ray r is kept y-fastest
+  int It, Jt;
+};
+constexpr int RT_PARTS = 64;  // change-reduction slots, 32 words apart
+
 hipError_t launch_rt_sweep_plane(hipStream_t, double *, const double *, const double *,
                                  const double *, const double *, const RtRays &, int, int, long,
-                                 int);
+                                 int, unsigned *);
 hipError_t launch_rt_transpose_xy(hipStream_t, const double *, double *, int, int, int);
 // several planes per launch for rays whose c axis has at most RT_MULTI_ROWS - 1 nodes
 constexpr int RT_MULTI_COLS = 32;  // b nodes a workgroup owns
@@ -60,13 +69,14 @@ constexpr int RT_MULTI_PLANES = 16;
 constexpr int RT_MULTI_ROWS = 33;
 hipError_t launch_rt_sweep_multi(hipStream_t, double *, const double *, const double *,
                                  const double *, const double *, const RtRays &, int, int, int,
-                                 long, int);
+                                 long, int, unsigned *);
 hipError_t launch_rt_halo(hipStream_t, double *, int, int, int, int, int, int, unsigned, unsigned,
                           long, int);
 hipError_t launch_rt_face(hipStream_t, double *, const double *, int, int, int, int, int, int,
                           int, int, int, double, unsigned, long, int);
-hipError_t launch_rt_maxdiff(hipStream_t, const double *, const double *, const int[3],
-                             const int[3], int, int, long, int, unsigned, bool, unsigned *);
+hipError_t launch_rt_faces(hipStream_t, const double *, double *, bool, const RtFaces &, long, int,
+                           unsigned *);
+hipError_t launch_rt_parts_reduce(hipStream_t, unsigned *, unsigned *, bool);
 hipError_t launch_rt_qrad(hipStream_t, const double *, const double *, const double *, double *,
                           const int[3], const int[3], int, int, long, int, unsigned,
                           const RtRays &);

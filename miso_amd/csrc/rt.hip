@@ -42,26 +42,59 @@ __device__ __forceinline__ double rt_node(const RtRay &R, double I00, double I10
   return rt_integrate(iu, su, sc, dtau);
 }
 
+// The solve's convergence measure is the max over lattice nodes of
+// |I_end - I_start| of an iteration, float-narrowed as util::max2 does
+// (rt_host.hpp:612-629); positive floats order like their bit patterns.
+// Every lattice node of a ray is either swept (exactly once per iteration,
+// so the sweep takes the change from the value it overwrites) or on one of
+// the ray's upwind faces (written only by the halo and face kernels; their
+// start values are saved first and compared after).  Workgroup maxima go
+// by atomicMax into one of RT_PARTS slots, 32 words apart, then one small
+// kernel reduces them.
+__device__ __forceinline__ unsigned rt_change(double now, double before) {
+  return __float_as_uint((float)fabs(now - before));
+}
+
+__device__ __forceinline__ void rt_block_max(unsigned best, unsigned *part) {
+  __shared__ unsigned red[16];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned b = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max(b, red[w]);
+    if (b) atomicMax(part + 32 * ((blockIdx.x + 7 * blockIdx.y) % RT_PARTS), b);
+  }
+}
+
 // plane p of every ray of `rays` (blockIdx.y = entry, R.id = its slot)
 __global__ __launch_bounds__(256) void rt_sweep_plane_kernel(double *__restrict__ I,
                                                              const double *__restrict__ kap_x,
                                                              const double *__restrict__ S_x,
                                                              const double *__restrict__ kap_y,
                                                              const double *__restrict__ S_y,
-                                                             RtRays rays, int p, long ray_stride) {
+                                                             RtRays rays, int p, long ray_stride,
+                                                             unsigned *part) {
   const RtRay &R = rays.r[blockIdx.y];
-  if (p >= R.nplanes) return;
+  if (p >= R.nplanes) return;  // uniform over the block
   const double *__restrict__ kap = R.tr ? kap_y : kap_x;
   const double *__restrict__ S = R.tr ? S_y : S_x;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= R.nb * R.nc) return;
-  const int ib = t % R.nb, ic = t / R.nb;
-  const int c = R.start + p * R.da + ib * R.db + ic * R.dc;  // cell index (kappa, S)
-  const int c00 = c - R.ua;
-  const int c10 = c00 - R.ub, c01 = c00 - R.uc, c11 = c00 - R.ub - R.uc;
-  double *Ir = I + ray_stride * R.id;
-  Ir[c] = rt_node(R, Ir[c00], Ir[c10], Ir[c01], Ir[c11], S[c00], S[c10], S[c01], S[c11], kap[c00],
-                  kap[c10], kap[c01], kap[c11], kap[c], S[c]);
+  unsigned best = 0u;
+  if (t < R.nb * R.nc) {
+    const int ib = t % R.nb, ic = t / R.nb;
+    const int c = R.start + p * R.da + ib * R.db + ic * R.dc;  // cell index (kappa, S)
+    const int c00 = c - R.ua;
+    const int c10 = c00 - R.ub, c01 = c00 - R.uc, c11 = c00 - R.ub - R.uc;
+    double *Ir = I + ray_stride * R.id;
+    const double before = Ir[c];
+    const double now = rt_node(R, Ir[c00], Ir[c10], Ir[c01], Ir[c11], S[c00], S[c10], S[c01],
+                               S[c11], kap[c00], kap[c10], kap[c01], kap[c11], kap[c], S[c]);
+    Ir[c] = now;
+    best = rt_change(now, before);
+  }
+  rt_block_max(best, part);
 }
 
 // Planes p0 .. p0 + P - 1 of every ray of `rays` in one launch, for rays
@@ -84,7 +117,7 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
                                                              const double *__restrict__ kap_y,
                                                              const double *__restrict__ S_y,
                                                              RtRays rays, int p0, int P,
-                                                             long ray_stride) {
+                                                             long ray_stride, unsigned *part) {
   constexpr int W = RT_MULTI_COLS, X = RT_MULTI_COLS + RT_MULTI_PLANES, ROWS = RT_MULTI_ROWS;
   constexpr int NT = RT_MULTI_THREADS, CPT = (X * ROWS + NT - 1) / NT;
   __shared__ double Il[ROWS][X], Kl[ROWS][X], Sl[ROWS][X];
@@ -98,7 +131,8 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
   const int ext = W + P, ncell = ext * (R.nc + 1);
   const int pend = min(p0 + P, R.nplanes);
   // per cell: memory index at plane 0, role (0 none, 1 boundary, read every
-  // plane; 2 computed; 3 computed and owned; 4 the cone's far edge e = 0,
+  // plane; 2 computed; 3 computed and owned, its value before the sweep read
+  // for the change; 4 the cone's far edge e = 0,
   // read for the plane before p0 only: never valid after it, and its node
   // may be written by another workgroup meanwhile), LDS position
   int off[CPT], role[CPT], ce[CPT], cr[CPT];
@@ -136,10 +170,11 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
       const int c = off[q] + p0 * R.da;
       kc[q] = kap[c];
       sc[q] = S[c];
-      if (role[q] == 1) bnd[q] = Ir[c];
+      if (role[q] == 1 || role[q] == 3) bnd[q] = Ir[c];
     }
   }
   __syncthreads();
+  unsigned best = 0u;
   for (int p = p0; p < pend; ++p) {
     // the next plane's loads are in flight while this plane is computed
     double kn[CPT], sn[CPT], bn[CPT];
@@ -150,7 +185,7 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
         const int c = off[q] + (p + 1) * R.da;
         kn[q] = kap[c];
         sn[q] = S[c];
-        if (role[q] == 1) bn[q] = Ir[c];
+        if (role[q] == 1 || role[q] == 3) bn[q] = Ir[c];
       }
     }
     double in[CPT];
@@ -171,7 +206,10 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
         Il[cr[q]][ce[q]] = in[q];
         Kl[cr[q]][ce[q]] = kc[q];
         Sl[cr[q]][ce[q]] = sc[q];
-        if (role[q] == 3) Ir[off[q] + p * R.da] = in[q];
+        if (role[q] == 3) {
+          best = max(best, rt_change(in[q], bnd[q]));
+          Ir[off[q] + p * R.da] = in[q];
+        }
       }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
@@ -181,14 +219,16 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
     }
     __syncthreads();
   }
+  rt_block_max(best, part);
 }
 
 hipError_t launch_rt_sweep_multi(hipStream_t st, double *I, const double *kap, const double *S,
                                  const double *kap_y, const double *S_y, const RtRays &rays,
-                                 int nrays, int p0, int P, long ray_stride, int max_nb) {
+                                 int nrays, int p0, int P, long ray_stride, int max_nb,
+                                 unsigned *part) {
   dim3 grid((max_nb + RT_MULTI_COLS - 1) / RT_MULTI_COLS, nrays);
   hipLaunchKernelGGL(rt_sweep_multi_kernel, grid, dim3(RT_MULTI_THREADS), 0, st, I, kap, S, kap_y,
-                     S_y, rays, p0, P, ray_stride);
+                     S_y, rays, p0, P, ray_stride, part);
   return hipGetLastError();
 }
 
@@ -240,43 +280,43 @@ __global__ __launch_bounds__(256) void rt_face_kernel(double *__restrict__ I,
   I[ray_stride * n + c] = kind == 1 ? S[c] : (kind == 2 ? 0.0 : value);
 }
 
-// max over lattice nodes of |I - old|, float-narrowed as util::max2 does
-// (rt_host.hpp:612-629); positive floats order like their bit patterns.
-__global__ __launch_bounds__(256) void rt_maxdiff_kernel(const double *__restrict__ I,
-                                                         const double *__restrict__ old, int lo_i,
-                                                         int lo_j, int lo_k, int ni, int nj,
-                                                         int nk, int It, int Jt, long ray_stride,
-                                                         int nrays, unsigned ray_mask,
-                                                         unsigned *slot) {
-  const long n_all = (long)ni * nj * nk * nrays;
+// The upwind faces of every ray over the lattice (axes with more than one
+// node; blockIdx.y = ray * 3 + axis): the layer x_a = lo_a (ray going +) or
+// lo_a + n_a - 1, all lattice nodes of the other two axes.  save: old = I
+// there; otherwise the change of I against old, into the parts.
+__global__ __launch_bounds__(256) void rt_faces_kernel(const double *__restrict__ I,
+                                                       double *__restrict__ old, int save,
+                                                       RtFaces f, long ray_stride,
+                                                       unsigned *part) {
+  const int n = blockIdx.y / 3, a = blockIdx.y % 3;
+  if (f.n[a] <= 1) return;  // uniform over the block
+  const int t0 = a == 0 ? 1 : 0, t1 = a == 2 ? 1 : 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned best = 0u;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n_all;
-       t += (long)gridDim.x * blockDim.x) {
-    const int i = lo_i + (int)(t % ni);
-    long r = t / ni;
-    const int j = lo_j + (int)(r % nj);
-    r /= nj;
-    const int k = lo_k + (int)(r % nk);
-    const int n = (int)(r / nk);
-    if (!((ray_mask >> n) & 1u)) continue;
-    const long c = ray_stride * n + (long)(k * Jt + j) * It + i;
-    const float d = (float)fabs(I[c] - old[c]);
-    best = max(best, __float_as_uint(d));
+  if (t < f.n[t0] * f.n[t1]) {
+    int x[3];
+    x[a] = f.lo[a] + (((f.pos[a] >> n) & 1u) ? 0 : f.n[a] - 1);
+    x[t0] = f.lo[t0] + t % f.n[t0];
+    x[t1] = f.lo[t1] + t / f.n[t0];
+    const long c = ray_stride * n + (((f.tr >> n) & 1u) ? ((long)x[2] * f.It + x[0]) * f.Jt + x[1]
+                                                         : ((long)x[2] * f.Jt + x[1]) * f.It + x[0]);
+    if (save) old[c] = I[c];
+    else best = rt_change(I[c], old[c]);
   }
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, o, 64));
-  __shared__ unsigned red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned b = red[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max(b, red[w]);
-    atomicMax(slot, b);
-  }
+  if (!save) rt_block_max(best, part);
 }
 
-__global__ void rt_slot_clear_kernel(unsigned *slot) {
-  if (threadIdx.x == 0) slot[0] = 0u;
+// slot = max of the parts; the parts are cleared for the next iteration
+__global__ __launch_bounds__(64) void rt_parts_reduce_kernel(unsigned *part, unsigned *slot,
+                                                             int clear_only) {
+  unsigned b = 0u;
+  for (int s = threadIdx.x; s < RT_PARTS; s += 64) {
+    b = max(b, part[32 * s]);
+    part[32 * s] = 0u;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) b = max(b, (unsigned)__shfl_xor((int)b, o, 64));
+  if (threadIdx.x == 0 && !clear_only) slot[0] = b;
 }
 
 // Q_rad = 4 pi kappa (J - S), J = sum_r w_r I_r in ray order, lattice nodes
@@ -302,10 +342,11 @@ __global__ __launch_bounds__(256) void rt_qrad_kernel(const double *__restrict__
 
 hipError_t launch_rt_sweep_plane(hipStream_t st, double *I, const double *kap, const double *S,
                                  const double *kap_y, const double *S_y, const RtRays &rays,
-                                 int nrays, int p, long ray_stride, int max_nodes) {
+                                 int nrays, int p, long ray_stride, int max_nodes,
+                                 unsigned *part) {
   dim3 grid((max_nodes + 255) / 256, nrays);
   hipLaunchKernelGGL(rt_sweep_plane_kernel, grid, dim3(256), 0, st, I, kap, S, kap_y, S_y, rays,
-                     p, ray_stride);
+                     p, ray_stride, part);
   return hipGetLastError();
 }
 
@@ -355,12 +396,22 @@ hipError_t launch_rt_face(hipStream_t st, double *I, const double *S, int axis, 
   return hipGetLastError();
 }
 
-hipError_t launch_rt_maxdiff(hipStream_t st, const double *I, const double *old, const int lo[3],
-                             const int n[3], int It, int Jt, long ray_stride, int nrays,
-                             unsigned ray_mask, bool clear, unsigned *slot) {
-  if (clear) hipLaunchKernelGGL(rt_slot_clear_kernel, dim3(1), dim3(64), 0, st, slot);
-  hipLaunchKernelGGL(rt_maxdiff_kernel, dim3(2048), dim3(256), 0, st, I, old, lo[0], lo[1], lo[2],
-                     n[0], n[1], n[2], It, Jt, ray_stride, nrays, ray_mask, slot);
+hipError_t launch_rt_faces(hipStream_t st, const double *I, double *old, bool save,
+                          const RtFaces &f, long ray_stride, int nrays, unsigned *part) {
+  int m = 1;
+  for (int a = 0; a < 3; ++a) {
+    const int t0 = a == 0 ? 1 : 0, t1 = a == 2 ? 1 : 2;
+    m = max(m, f.n[t0] * f.n[t1]);
+  }
+  hipLaunchKernelGGL(rt_faces_kernel, dim3((m + 255) / 256, 3 * nrays), dim3(256), 0, st, I, old,
+                     save ? 1 : 0, f, ray_stride, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_rt_parts_reduce(hipStream_t st, unsigned *part, unsigned *slot,
+                                  bool clear_only) {
+  hipLaunchKernelGGL(rt_parts_reduce_kernel, dim3(1), dim3(64), 0, st, part, slot,
+                     clear_only ? 1 : 0);
   return hipGetLastError();
 }
 

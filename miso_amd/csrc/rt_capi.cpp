@@ -3,9 +3,12 @@
 // Host side of RT::solve (miso/include/miso/rt_host.hpp:632-651): per-ray
 // sweep constants are computed here with the reference's expressions
 // (rt_host.hpp:461-508, float-narrowed max/min included), then every
-// iteration is: copy rint -> rint_old, periodic halo (z, y, x), incoming
-// boundary on physical faces, one launch per plane for all rays, and the
-// max-change reduction read back to decide convergence.
+// iteration is: save the rays' upwind faces into rint_old, periodic halo
+// (z, y, x), incoming boundary on physical faces, the sweeps (which take
+// the change of every node they write), the change on the upwind faces,
+// and the max change read back to decide convergence.  That is the
+// reference's max over the lattice of |rint - rint_old| with rint_old a
+// full copy (rt_host.hpp:612-651), without the copy.
 //
 // Rays whose dominant axis is x run in the y-fastest frame (node (i, j, k)
 // at (k * i_total + i) * j_total + j): their constants are those of the
@@ -195,7 +198,6 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
   double *old = static_cast<double *>(rint_old);
   const double *kap = static_cast<const double *>(kappa);
   const double *src = static_cast<const double *>(S);
-  const size_t bytes = sizeof(double) * su.nrays * su.ray_stride;
   const int It = g->i_total, Jt = g->j_total, Kt = g->k_total;
   int lo[3], cnt[3], lo_y[3], cnt_y[3];
   for (int a = 0; a < 3; ++a) {
@@ -269,12 +271,24 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
       p0 = e;
     }
   }
+  RtFaces uf;  // the rays' upwind faces
+  for (int a = 0; a < 3; ++a) {
+    uf.lo[a] = lo[a];
+    uf.n[a] = cnt[a];
+    uf.pos[a] = su.pos_mask[a];
+  }
+  uf.tr = su.tr_mask;
+  uf.It = It;
+  uf.Jt = Jt;
+  if (!ctx->rt_parts) MISOHIP_TRY(hipMalloc(&ctx->rt_parts, sizeof(unsigned) * 32 * RT_PARTS));
+  unsigned *parts = ctx->rt_parts;
+  MISOHIP_TRY(launch_rt_parts_reduce(ctx->stream, parts, nullptr, true));
   int it = 0;
   double md = 0.0;
   for (it = 0; it < rt->max_iters; ++it) {
     {
       Timed t_(ctx, MISOHIP_KIND_RT);
-      MISOHIP_TRY(hipMemcpyAsync(old, I, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      MISOHIP_TRY(launch_rt_faces(ctx->stream, I, old, true, uf, su.ray_stride, su.nrays, parts));
       for (int a = 2; a >= 0; --a)
         if (rt->periodic[a] && su.size[a] > 1) {
           if (x_mask)
@@ -310,15 +324,12 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
       for (const Phase &ph : phases)
         for (int p = ph.p0; p < ph.p1; ++p)
           MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, kap_y, src_y, ph.rays,
-                                            ph.n, p, su.ray_stride, ph.max_nodes));
+                                            ph.n, p, su.ray_stride, ph.max_nodes, parts));
       for (int p0 = 0; p0 < m_planes; p0 += P)
         MISOHIP_TRY(launch_rt_sweep_multi(ctx->stream, I, kap, src, kap_y, src_y, mrays, nm, p0,
-                                          P, su.ray_stride, m_nb));
-      MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo, cnt, It, Jt, su.ray_stride,
-                                    su.nrays, x_mask, true, ctx->d_slot + 1));
-      if (su.tr_mask)
-        MISOHIP_TRY(launch_rt_maxdiff(ctx->stream, I, old, lo_y, cnt_y, Jt, It, su.ray_stride,
-                                      su.nrays, su.tr_mask, false, ctx->d_slot + 1));
+                                          P, su.ray_stride, m_nb, parts));
+      MISOHIP_TRY(launch_rt_faces(ctx->stream, I, old, false, uf, su.ray_stride, su.nrays, parts));
+      MISOHIP_TRY(launch_rt_parts_reduce(ctx->stream, parts, ctx->d_slot + 1, false));
     }
     MISOHIP_TRY(hipMemcpyAsync(ctx->h_slot + 1, ctx->d_slot + 1, sizeof(unsigned),
                                hipMemcpyDeviceToHost, ctx->stream));
