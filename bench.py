@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--rmhd-train-seeds", type=int, default=4,
                     help="boxes (other seeds) the training columns come from")
     ap.add_argument("--rmhd-epochs", type=int, default=200)
+    ap.add_argument("--rmhd-slab-tol", type=float, default=None,
+                    help="tolerance of the bottom-slab solves (default: the RT tolerance)")
     ap.add_argument("--rmhd-slab-share", type=float, default=0.99,
                     help="share of the network's residual at a solve whose bottom layers are "
                          "solved plainly on every step (<= 0: no slab)")
@@ -406,13 +408,25 @@ def rmhd_leg(args):
             box.step_plain()
         torch.cuda.synchronize()
         i0 = len(box.rt_iters)
+        evs = []
         t0 = time.perf_counter()
-        for _ in range(steps):
-            box.step_plain()
+        for _ in range(steps):  # box.step_plain(), with events between its two parts
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record()
+            q = box.solve_rt()
+            e[1].record()
+            box._advance(q)
+            e[2].record()
+            evs.append(e)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        rt_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+        mhd_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+        # a learned RT can at best remove the solve: (MHD + RT) / MHD
         r = {"cells": n ** 3, "steps": steps, "ms_per_step": 1e3 * el / steps,
-             "steps_per_s": steps / el, "rt_iterations_mean": float(np.mean(box.rt_iters[i0:]))}
+             "steps_per_s": steps / el, "rt_iterations_mean": float(np.mean(box.rt_iters[i0:])),
+             "rt_solve_ms": rt_ms, "mhd_step_ms": mhd_ms,
+             "learned_speedup_bound": (rt_ms + mhd_ms) / mhd_ms}
         log(f"plain RT {n}^3: {r}")
         del box
         torch.cuda.empty_cache()
@@ -452,7 +466,8 @@ def rmhd_leg(args):
     # timed learned run: one untimed cycle, then args.rmhd_cycles cycles
     box = make_box(n, 7)
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
-                         correct=not args.rmhd_no_correct, slab_share=slab_share)
+                         correct=not args.rmhd_no_correct, slab_share=slab_share,
+                         slab_tolerance=args.rmhd_slab_tol)
     for _ in range(every):
         box.step_learned()
     torch.cuda.synchronize()
@@ -466,7 +481,7 @@ def rmhd_leg(args):
                "flagged_column_fraction": float(box.flags.float().mean()),
                "bottom_slab": None if box.slab is None else {
                    "kept_layers": box.slab.layers, "buffer_layers": box.slab.buffer,
-                   "residual_share": slab_share,
+                   "residual_share": slab_share, "tolerance": box.slab_kw["tolerance"],
                    "rt_iterations_mean": float(np.mean(box.slab.iters)) if box.slab.iters
                    else None}}
     del box
@@ -474,7 +489,8 @@ def rmhd_leg(args):
     # accuracy: the same run again, Q_rad checked against converged solves
     box = make_box(n, 7)
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
-                         correct=not args.rmhd_no_correct, slab_share=slab_share)
+                         correct=not args.rmhd_no_correct, slab_share=slab_share,
+                         slab_tolerance=args.rmhd_slab_tol)
     errs = {"learned": [], "network_only": [], "network_corrected": [], "stale_rt": []}
     by_age = {}
     tmp = torch.empty_like(box.q_rt)

@@ -263,19 +263,22 @@ __global__ __launch_bounds__(256) void column_features_kernel(
     if (i >= ni || kz >= nz) continue;
     const int k = sh.mk + kz;
     const int c = sh.idx(sh.mi + i, sh.mj + j, k);
+    // the features end in FP32 (then BF16): logarithms in FP32 of the FP64
+    // arguments, sums in FP64
     const double ro = q.q[0][c], ei = q.q[7][c];
     const double bx = q.q[4][c], by = q.q[5][c], bz = q.q[6][c];
-    const double lnT = log((gm - 1.0) * ei);
-    tile[0][r][lane] = (float)log(ro);
+    const double lnT = (double)logf((float)((gm - 1.0) * ei));
+    tile[0][r][lane] = logf((float)ro);
     tile[1][r][lane] = (float)lnT;
     tile[2][r][lane] = (float)(bx * bx + by * by + bz * bz);
     if (nfeat == 4) {
       const int ip = (i + h) % ni, im = (i - h % ni + ni) % ni;
       const int jp = (j + h) % nj, jm = (j - h % nj + nj) % nj;
-      const double s = log((gm - 1.0) * q.q[7][sh.idx(sh.mi + ip, sh.mj + j, k)]) +
-                       log((gm - 1.0) * q.q[7][sh.idx(sh.mi + im, sh.mj + j, k)]) +
-                       log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jp, k)]) +
-                       log((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jm, k)]);
+      const double s =
+          (double)logf((float)((gm - 1.0) * q.q[7][sh.idx(sh.mi + ip, sh.mj + j, k)])) +
+          (double)logf((float)((gm - 1.0) * q.q[7][sh.idx(sh.mi + im, sh.mj + j, k)])) +
+          (double)logf((float)((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jp, k)])) +
+          (double)logf((float)((gm - 1.0) * q.q[7][sh.idx(sh.mi + i, sh.mj + jm, k)]));
       tile[3][r][lane] = (float)(s - 4.0 * lnT);
     }
   }

@@ -231,7 +231,8 @@ class RadiativeMHD:
 
     def attach_surrogate(self, surrogate, rt_every: int = 16, flag_tol: float | None = None,
                          correct: bool = True, slab_share: float | None = 0.99,
-                         slab_tau: float = 10.0, slab_max_frac: float = 0.25) -> None:
+                         slab_tau: float = 10.0, slab_max_frac: float = 0.25,
+                         slab_tolerance: float | None = None) -> None:
         """Learned RT: the network every step, a plain solve every rt_every
         steps.  correct: add the column correction the last solve left (the
         solve minus the network on the same state), so the network carries
@@ -240,7 +241,8 @@ class RadiativeMHD:
         until the next solve (None: no flags); slab_share: at every solve
         the bottom layers holding this share of the network's squared
         residual, topped by slab_tau of optical depth, are handed to a
-        plain slab solve on every step, if they fit in slab_max_frac of the
+        plain slab solve on every step (to slab_tolerance, default the
+        run's RT tolerance), if they fit in slab_max_frac of the
         height (None: no slab)."""
         import torch
         self.surrogate = surrogate
@@ -248,6 +250,9 @@ class RadiativeMHD:
         self.flag_tol = flag_tol
         self.correct = bool(correct)
         self.slab_share, self.slab_tau, self.slab_max_frac = slab_share, slab_tau, slab_max_frac
+        self.slab_kw = dict(self.rt_kw)
+        if slab_tolerance is not None:
+            self.slab_kw["tolerance"] = float(slab_tolerance)
         self.slab = None
         ncol = self.grid.i_size * self.grid.j_size
         self.flags = torch.zeros(ncol, dtype=torch.int32, device=self.grid.device)
@@ -299,7 +304,7 @@ class RadiativeMHD:
         q = self.learned_qrad()
         if self.slab is not None:
             self.slab.apply(self.ctx, self.model.qq, self.strat.gm, self.kappa0, self.s_rad, q,
-                            **self.rt_kw)
+                            **self.slab_kw)
         if probe:
             probe(q, False)
         return self._advance(q)
