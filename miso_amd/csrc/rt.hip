@@ -319,25 +319,50 @@ __global__ __launch_bounds__(64) void rt_parts_reduce_kernel(unsigned *part, uns
   if (threadIdx.x == 0 && !clear_only) slot[0] = b;
 }
 
-// Q_rad = 4 pi kappa (J - S), J = sum_r w_r I_r in ray order, lattice nodes
-// (a ray in tr_mask holds its intensities y-fastest)
+// Q_rad = 4 pi kappa (J - S), J = sum_r w_r I_r in ray order, lattice nodes.
+// A block covers 32 x 32 (i, j) nodes of one plane k; a ray kept y-fastest
+// (tr_mask) is read along j into LDS and taken from there, the others along
+// i, so every read is coalesced and each node sums its rays in order.
 __global__ __launch_bounds__(256) void rt_qrad_kernel(const double *__restrict__ I,
                                                       const double *__restrict__ kap,
                                                       const double *__restrict__ S,
                                                       double *__restrict__ q, int lo_i, int lo_j,
-                                                      int lo_k, int ni, int nj, int nk, int It,
-                                                      int Jt, long ray_stride, int nrays,
+                                                      int lo_k, int ni, int nj, int It, int Jt,
+                                                      long ray_stride, int nrays, int tiles_i,
                                                       unsigned tr_mask, RtRays w) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)ni * nj * nk) return;
-  const int i = lo_i + (int)(t % ni);
-  const long r = t / ni;
-  const int j = lo_j + (int)(r % nj), k = lo_k + (int)(r / nj);
-  const int c = (k * Jt + j) * It + i, ct = (k * It + i) * Jt + j;
-  double J = 0.0;
-  for (int n = 0; n < nrays; ++n)
-    J += w.r[n].wq * I[ray_stride * n + (((tr_mask >> n) & 1u) ? ct : c)];
-  q[c] = 4.0 * 3.14159265358979323846 * kap[c] * (J - S[c]);
+  __shared__ double tile[32][33];
+  const int ti = blockIdx.x % tiles_i, tj = blockIdx.x / tiles_i, k = lo_k + blockIdx.y;
+  const int l = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+  double J[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int n = 0; n < nrays; ++n) {
+    const double *In = I + ray_stride * n;
+    const double wq = w.r[n].wq;
+    if ((tr_mask >> n) & 1u) {  // uniform over the block
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int il = r0 + 8 * m, i = ti * 32 + il, j = tj * 32 + l;
+        if (i < ni && j < nj) tile[il][l] = In[((long)k * It + lo_i + i) * Jt + lo_j + j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < 4; ++m) J[m] += wq * tile[l][r0 + 8 * m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int i = ti * 32 + l, j = tj * 32 + r0 + 8 * m;
+        if (i < ni && j < nj) J[m] += wq * In[((long)k * Jt + lo_j + j) * It + lo_i + i];
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int i = ti * 32 + l, j = tj * 32 + r0 + 8 * m;
+    if (i < ni && j < nj) {
+      const long c = ((long)k * Jt + lo_j + j) * It + lo_i + i;
+      q[c] = 4.0 * 3.14159265358979323846 * kap[c] * (J[m] - S[c]);
+    }
+  }
 }
 
 hipError_t launch_rt_sweep_plane(hipStream_t st, double *I, const double *kap, const double *S,
@@ -418,10 +443,9 @@ hipError_t launch_rt_parts_reduce(hipStream_t st, unsigned *part, unsigned *slot
 hipError_t launch_rt_qrad(hipStream_t st, const double *I, const double *kap, const double *S,
                           double *q, const int lo[3], const int n[3], int It, int Jt,
                           long ray_stride, int nrays, unsigned tr_mask, const RtRays &w) {
-  const long cnt = (long)n[0] * n[1] * n[2];
-  hipLaunchKernelGGL(rt_qrad_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, I,
-                     kap, S, q, lo[0], lo[1], lo[2], n[0], n[1], n[2], It, Jt, ray_stride, nrays,
-                     tr_mask, w);
+  const int ti = (n[0] + 31) / 32, tj = (n[1] + 31) / 32;
+  hipLaunchKernelGGL(rt_qrad_kernel, dim3(ti * tj, n[2]), dim3(256), 0, st, I, kap, S, q, lo[0],
+                     lo[1], lo[2], n[0], n[1], It, Jt, ray_stride, nrays, ti, tr_mask, w);
   return hipGetLastError();
 }
 
