@@ -13,7 +13,7 @@ struct misohip_ctx {
   int device;
   hipStream_t stream;
   unsigned *d_slot;  // CFL reduction slot (device)
-  unsigned *h_slot;  // pinned host copy
+  unsigned *h_slot;  // pinned host words: [0] CFL, [1] RT change, bytes 64..87 RT spacings
   // timing: pairs of events recorded around launches, tagged by kind
   bool profiling = false;
   std::vector<hipEvent_t> pool;

@@ -162,32 +162,28 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
       Kl[cr[q]][ce[q]] = kap[c];
       Sl[cr[q]][ce[q]] = S[c];
     }
-  double kc[CPT], sc[CPT], bnd[CPT];
+  // kappa, S and the boundary (role 1) or before-sweep (role 3) intensity of
+  // a plane's cells; loads run two planes ahead of the computation
+  auto load = [&](int p, double *k, double *s, double *b) {
 #pragma unroll
-  for (int q = 0; q < CPT; ++q) {
-    kc[q] = sc[q] = bnd[q] = 0.0;
-    if (role[q] && role[q] < 4) {
-      const int c = off[q] + p0 * R.da;
-      kc[q] = kap[c];
-      sc[q] = S[c];
-      if (role[q] == 1 || role[q] == 3) bnd[q] = Ir[c];
+    for (int q = 0; q < CPT; ++q) {
+      k[q] = s[q] = b[q] = 0.0;
+      if (p < pend && role[q] && role[q] < 4) {
+        const int c = off[q] + p * R.da;
+        k[q] = kap[c];
+        s[q] = S[c];
+        if (role[q] == 1 || role[q] == 3) b[q] = Ir[c];
+      }
     }
-  }
+  };
+  double kc[CPT], sc[CPT], bnd[CPT], k1[CPT], s1[CPT], b1[CPT];
+  load(p0, kc, sc, bnd);
+  load(p0 + 1, k1, s1, b1);
   __syncthreads();
   unsigned best = 0u;
   for (int p = p0; p < pend; ++p) {
-    // the next plane's loads are in flight while this plane is computed
     double kn[CPT], sn[CPT], bn[CPT];
-#pragma unroll
-    for (int q = 0; q < CPT; ++q) {
-      kn[q] = sn[q] = bn[q] = 0.0;
-      if (p + 1 < pend && role[q] && role[q] < 4) {
-        const int c = off[q] + (p + 1) * R.da;
-        kn[q] = kap[c];
-        sn[q] = S[c];
-        if (role[q] == 1 || role[q] == 3) bn[q] = Ir[c];
-      }
-    }
+    load(p + 2, kn, sn, bn);
     double in[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
@@ -213,9 +209,12 @@ __global__ __launch_bounds__(RT_MULTI_THREADS) void rt_sweep_multi_kernel(double
       }
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
-      kc[q] = kn[q];
-      sc[q] = sn[q];
-      bnd[q] = bn[q];
+      kc[q] = k1[q];
+      sc[q] = s1[q];
+      bnd[q] = b1[q];
+      k1[q] = kn[q];
+      s1[q] = sn[q];
+      b1[q] = bn[q];
     }
     __syncthreads();
   }

@@ -87,12 +87,21 @@ int setup(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_config *rt, 
     S.b1[a] = S.b0[a] + (sizes[a] > 1 ? sizes[a] : 0);
   }
   S.ray_stride = (long)It * Jt * Kt;
-  // uniform spacings (rt_host.hpp:467-469 uses dx[0] etc.)
+  // uniform spacings (rt_host.hpp:467-469 uses dx[0] etc.), read in stream
+  // order into the context's pinned words and waited for once
   double d[3] = {1.e30, 1.e30, 1.e30};
   const void *dptr[3] = {g->dx, g->dy, g->dz};
+  double *hd = reinterpret_cast<double *>(reinterpret_cast<char *>(ctx->h_slot) + 64);
+  bool any = false;
   for (int a = 0; a < 3; ++a)
-    if (sizes[a] > 1)
-      MISOHIP_TRY(hipMemcpy(&d[a], dptr[a], sizeof(double), hipMemcpyDeviceToHost));
+    if (sizes[a] > 1) {
+      MISOHIP_TRY(hipMemcpyAsync(hd + a, dptr[a], sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+      any = true;
+    }
+  if (any) MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int a = 0; a < 3; ++a)
+    if (sizes[a] > 1) d[a] = hd[a];
   S.max_nodes = 0;
   S.max_planes = 0;
   for (int a = 0; a < 3; ++a) S.pos_mask[a] = 0u;

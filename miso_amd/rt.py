@@ -34,6 +34,7 @@ class RT:
         self.abs_coeff = torch.zeros(shp, dtype=torch.float64, device=dev)
         self.qrad_field = torch.zeros(shp, dtype=torch.float64, device=dev)
         self.yfast_mask = 0
+        self._layouts = {}                   # x_rays_yfast -> mask (the grid is fixed)
 
     def config(self, periodic=(True, True, False), face_kind=(0, 0, 0, 0, 1, 2),
                face_value=(0.0,) * 6, tolerance=1e-5, max_iters=100,
@@ -75,7 +76,10 @@ class RT:
     def solve(self, ctx, **kw):
         """RT::solve; returns (iterations, last max change)."""
         cfg = self.config(**kw)
-        self.yfast_mask = self.ray_layout(ctx, **kw)
+        yf = bool(kw.get("x_rays_yfast", True))
+        if yf not in self._layouts:
+            self._layouts[yf] = self.ray_layout(ctx, **kw)
+        self.yfast_mask = self._layouts[yf]
         it, md = C.c_int(), C.c_double()
         hip.call("misohip_rt_solve", ctx.handle, C.byref(self.grid.c_struct()), C.byref(cfg),
                  C.c_void_p(self.abs_coeff.data_ptr()), C.c_void_p(self.src_func.data_ptr()),
