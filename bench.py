@@ -51,6 +51,8 @@ def parse():
                          "shapes instead of tuning")
     ap.add_argument("--no-rmhd", action="store_true", help="skip the radiative-MHD leg")
     ap.add_argument("--rmhd-only", action="store_true", help="run only the radiative-MHD leg")
+    ap.add_argument("--rt-only", action="store_true",
+                    help="run only the plain-RT part of the radiative leg (args.rmhd_n^3)")
     ap.add_argument("--rmhd-n", type=int, default=384, help="learned-RT box, cells per side")
     ap.add_argument("--rmhd-plain-n", type=int, default=256, help="plain-RT box, cells per side")
     ap.add_argument("--rmhd-plain-steps", type=int, default=8)
@@ -435,8 +437,12 @@ def rmhd_leg(args):
     out = {"workload": "stratified radiative MHD box, FP64 fields, T 2 -> 0.5, scale height "
                        "0.1 box, kappa = kappa0 ro T^3 (tau = 1 at mid-height), 24 rays, "
                        "1e-2 seeded v and B perturbations, x/y periodic, fixed z ghosts"}
-    out["plain_rt_small"] = time_plain(args.rmhd_plain_n, args.rmhd_plain_steps, seed=7)
     n = args.rmhd_n
+    if args.rt_only:
+        out["plain_rt"] = time_plain(n, args.rmhd_plain_steps, seed=7)
+        ctx.close()
+        return out
+    out["plain_rt_small"] = time_plain(args.rmhd_plain_n, args.rmhd_plain_steps, seed=7)
     out["plain_rt"] = time_plain(n, args.rmhd_plain_steps, seed=7)
     every = args.rmhd_rt_every
     nsteps = every * args.rmhd_cycles
@@ -542,6 +548,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.rt_only:
+        print(json.dumps({"rt": rmhd_leg(args)}), flush=True)
         return
     if args.rmhd_only:
         print(json.dumps({"rmhd": rmhd_leg(args)}), flush=True)
