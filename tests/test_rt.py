@@ -204,3 +204,42 @@ def test_gpu_ray_layouts_and_multi_plane_sweeps_same_bits(ctx, sizes, periodic, 
         assert (it0, md0) == (it1, md1)
         assert torch.equal(I0, I1)
         assert torch.equal(q0, q1)
+
+
+@pytest.mark.gpu
+def test_gpu_single_ray_and_option_checks(ctx):
+    """The one-ray quadrature (a +z ray) through an absorber with a constant
+    incoming face, on a grid whose y axis is short enough for the
+    multi-plane sweep: against the oracle (intensities to 1e-13), the same
+    bits with one plane per launch, and sweep_planes outside 0..16 refused."""
+    import torch
+
+    from miso_amd import hip, mhd
+    from miso_amd.rt import RT
+    og = O.make_grid((10, 6, 40), 2, bounds=_bounds())
+    rng = np.random.default_rng(2)
+    kap = 2.0 + rng.random(og.shape)
+    S = 0.3 * rng.random(og.shape)
+    kw = dict(periodic=(1, 1, 0), face_kind=(0, 0, 0, 0, 3, 2), face_value=(0, 0, 0, 0, 1.0, 0),
+              tolerance=1e-12, max_iters=50)
+    I_ref = np.zeros((1,) + og.shape)
+    it_ref, _ = O.rt_solve(og, kap, S, I_ref, num_rays=1, **kw)
+    q_ref = O.rt_qrad(og, kap, S, I_ref, num_rays=1)
+    g = mhd.Grid(10, 6, 40, 2, bounds=_bounds(), device="cuda:0")
+    outs = []
+    for planes in (16, 0):
+        rt = RT(g, 1)
+        rt.abs_coeff.copy_(RT.to_dev_field(kap, "cuda:0"))
+        rt.src_func.copy_(RT.to_dev_field(S, "cuda:0"))
+        it, md = rt.solve(ctx, sweep_planes=planes, **kw)
+        q = rt.qrad(ctx, sweep_planes=planes, **kw)
+        torch.cuda.synchronize()
+        outs.append((it, rt.intensities().permute(0, 3, 2, 1).cpu().numpy(), RT.to_host_field(q)))
+    (it1, I1, q1), (it0, I0, q0) = outs
+    assert it1 == it0 == it_ref
+    assert np.array_equal(I1, I0) and np.array_equal(q1, q0)
+    assert np.allclose(I1, I_ref, rtol=1e-13, atol=1e-15)
+    lat = (slice(1, -2), slice(1, -2), slice(1, -2))
+    assert np.allclose(q1[lat], q_ref[lat], rtol=1e-12, atol=1e-13)
+    with pytest.raises(hip.MisohipError):
+        RT(g, 1).solve(ctx, sweep_planes=17, **kw)
