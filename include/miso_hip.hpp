@@ -17,6 +17,8 @@
 //                                    apply_boundary_condition
 //                                    (miso/include/miso/mhd_integrator_host.hpp:13-357)
 //   miso::hip::MHD<Real>             mhd::MHD<Real, Backend> (miso/include/miso/mhd.hpp:15-50)
+//   miso::hip::RT                    rt::RT<double> (miso/include/miso/rt_host.hpp:104-689),
+//                                    CPU-only in the reference
 //
 // User boundary and source functors cannot cross a C ABI; they are given as
 // descriptors: Boundary (per axis periodic / mirror with per-field signs /
@@ -412,6 +414,130 @@ template <typename Real> struct MHD {
   std::vector<double> run(const EOS &eos, const BC &bc, const Source &src, int nsteps) {
     return integrator.run(eos, bc, src, qq, nsteps);
   }
+};
+
+/// The RT boundary as descriptors (the reference passes a functor that calls
+/// set_incoming_boundary_on_physical_faces, rt_host.hpp:50-101): per axis
+/// periodic (the halo self-exchange, rt_host.hpp:229-452), otherwise per
+/// face (x inner, x outer, y inner, ...) untouched, thermal I = S, vacuum
+/// I = 0 or a constant value on the incoming rays.
+struct RtBoundary {
+  enum Kind { UNTOUCHED = 0, THERMAL = 1, VACUUM = 2, VALUE = 3 };
+  std::array<int, 3> periodic{{1, 1, 0}};
+  std::array<int, 6> face{{UNTOUCHED, UNTOUCHED, UNTOUCHED, UNTOUCHED, THERMAL, VACUUM}};
+  std::array<double, 6> value{{0.0, 0.0, 0.0, 0.0, 0.0, 0.0}};
+};
+
+/// rt::RT<double> on the device: rint and rint_old (num_rays fields of the
+/// grid), src_func and abs_coeff (x-fastest fields), the angular quadrature
+/// on the host and the lattice bounds ib0..kb1 (rt_host.hpp:127-154).  With
+/// x_rays_yfast (default) the rays whose dominant axis is x keep their field
+/// of rint y-fastest; intensity() returns any ray in MISO's order.
+class RT {
+ public:
+  RT(ExecContext<double> &ctx, const Grid<double> &grid, int num_rays)
+      : ctx_(&ctx), grid_(&grid), n_(grid.host().n_total()) {
+    weights.resize(24);
+    mu_x.resize(24);
+    mu_y.resize(24);
+    mu_z.resize(24);
+    const int n = misohip_rt_quadrature(num_rays, weights.data(), mu_x.data(), mu_y.data(),
+                                        mu_z.data());
+    if (n < 1) MISO_HIP_CHECK(n);
+    this->num_rays = n;
+    for (auto *v : {&weights, &mu_x, &mu_y, &mu_z}) v->resize(n);
+    rint = DeviceArray<double>(n_ * n);
+    rint_old = DeviceArray<double>(n_ * n);
+    src_func = DeviceArray<double>(n_);
+    abs_coeff = DeviceArray<double>(n_);
+    const HostGrid<double> &h = grid.host();
+    const int m[3] = {h.i_margin(), h.j_margin(), h.k_margin()};
+    const int st[3] = {h.xg.stride, h.yg.stride, h.zg.stride};
+    const int sz[3] = {h.i_size(), h.j_size(), h.k_size()};
+    int b0[3], b1[3];
+    for (int a = 0; a < 3; ++a) {
+      b0[a] = m[a] - st[a];
+      b1[a] = b0[a] + (sz[a] > 1 ? sz[a] : 0);
+    }
+    ib0 = b0[0]; ib1 = b1[0]; jb0 = b0[1]; jb1 = b1[1]; kb0 = b0[2]; kb1 = b1[2];
+  }
+
+  /// kappa and S from host fields in MISO order ((i * Jt + j) * Kt + k)
+  void set_fields(const std::vector<double> &kappa, const std::vector<double> &S) {
+    if (kappa.size() != n_ || S.size() != n_) throw std::runtime_error("RT field size");
+    MISO_HIP_CHECK(misohip_copy_to_device(ctx_->handle(), grid_->desc(), MISOHIP_F64,
+                                          kappa.data(), abs_coeff.data()));
+    MISO_HIP_CHECK(misohip_copy_to_device(ctx_->handle(), grid_->desc(), MISOHIP_F64, S.data(),
+                                          src_func.data()));
+  }
+
+  /// RT::solve (rt_host.hpp:632-651): iterate until the float-narrowed max
+  /// change is below tolerance; returns the iteration count and prints the
+  /// reference's message when max_iters are used up without converging.
+  int solve(double tolerance, int max_iters, const RtBoundary &bc) {
+    const misohip_rt_config c = config(tolerance, max_iters, bc);
+    int it = 0;
+    double md = 0.0;
+    MISO_HIP_CHECK(misohip_rt_solve(ctx_->handle(), grid_->desc(), &c, abs_coeff.data(),
+                                    src_func.data(), rint.data(), rint_old.data(), &it, &md));
+    yfast_mask_ = 0u;
+    MISO_HIP_CHECK(misohip_rt_ray_layout(ctx_->handle(), grid_->desc(), &c, &yfast_mask_));
+    last_diff = md;
+    if (!(md < tolerance)) std::printf("  RT did not converge in %d iterations.\n", max_iters);
+    return it;
+  }
+
+  /// Q_rad = 4 pi kappa (J - S) on the lattice into a device field of the grid
+  void qrad(DeviceArray<double> &out) const {
+    if (out.size() < n_) throw std::runtime_error("RT qrad field size");
+    const misohip_rt_config c = config(0.0, 1, RtBoundary{});
+    MISO_HIP_CHECK(misohip_rt_qrad(ctx_->handle(), grid_->desc(), &c, abs_coeff.data(),
+                                   src_func.data(), rint.data(), out.data()));
+  }
+
+  /// Ray n's intensities in MISO order ((i * Jt + j) * Kt + k), host copy
+  std::vector<double> intensity(int n) const {
+    const HostGrid<double> &h = grid_->host();
+    const std::size_t it = h.i_total(), jt = h.j_total(), kt = h.k_total();
+    std::vector<double> dev(n_), out(n_);
+    ctx_->synchronize();
+    MISO_HIP_RT_CHECK(hipMemcpy(dev.data(), rint.data() + n_ * std::size_t(n),
+                                sizeof(double) * n_, hipMemcpyDeviceToHost));
+    const bool yf = (yfast_mask_ >> n) & 1u;
+    for (std::size_t i = 0; i < it; ++i)
+      for (std::size_t j = 0; j < jt; ++j)
+        for (std::size_t k = 0; k < kt; ++k)
+          out[(i * jt + j) * kt + k] = yf ? dev[(k * it + i) * jt + j] : dev[(k * jt + j) * it + i];
+    return out;
+  }
+
+  DeviceArray<double> rint, rint_old, src_func, abs_coeff;
+  std::vector<double> weights, mu_x, mu_y, mu_z;
+  int num_rays = 0;
+  int ib0 = 0, ib1 = 0, jb0 = 0, jb1 = 0, kb0 = 0, kb1 = 0;
+  bool x_rays_yfast = true;
+  int sweep_planes = 16;
+  double last_diff = 0.0;
+
+ private:
+  misohip_rt_config config(double tolerance, int max_iters, const RtBoundary &bc) const {
+    misohip_rt_config c{};
+    c.num_rays = num_rays;
+    for (int a = 0; a < 3; ++a) c.periodic[a] = bc.periodic[a];
+    for (int f = 0; f < 6; ++f) {
+      c.face_kind[f] = bc.face[f];
+      c.face_value[f] = bc.value[f];
+    }
+    c.tolerance = tolerance;
+    c.max_iters = max_iters;
+    c.x_rays_yfast = x_rays_yfast ? 1 : 0;
+    c.sweep_planes = sweep_planes;
+    return c;
+  }
+  ExecContext<double> *ctx_;
+  const Grid<double> *grid_;
+  std::size_t n_;
+  unsigned yfast_mask_ = 0u;
 };
 
 }  // namespace hip

@@ -10,6 +10,10 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and libmisohip.so")
+    # torch's HIP runtime comes up first, whatever test runs first: the C++
+    # layer's test library links the system runtime, and a process whose
+    # first HIP user is that library finds torch.cuda unavailable after it
+    gpu_available()
 
 
 def gpu_available() -> bool:

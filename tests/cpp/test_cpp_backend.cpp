@@ -241,4 +241,43 @@ int cpp_run_steps(int fp64, int ni, int nj, int nk, const double *bounds, const 
   }
 }
 
+// RT::solve through miso::hip::RT on kappa and S given in MISO order (24
+// rays); writes the iteration count, every ray's intensities in MISO order
+// and Q_rad (MISO order, computed on the device).
+int cpp_rt_solve(int ni, int nj, int nk, const double *bounds, const double *kappa,
+                 const double *S, const int *periodic, const int *face, const double *value,
+                 double tol, int max_iters, int yfast, int sweep_planes, int *iters,
+                 double *I_out, double *q_out) {
+  try {
+    hip::HostGrid<double> hg(ni, nj, nk, 2, bounds[0], bounds[1], bounds[2], bounds[3],
+                             bounds[4], bounds[5]);
+    hip::ExecContext<double> ctx(0);
+    hip::Grid<double> grid(hg);
+    hip::RT rt(ctx, grid, 24);
+    rt.x_rays_yfast = yfast != 0;
+    rt.sweep_planes = sweep_planes;
+    const std::size_t N = hg.n_total();
+    rt.set_fields(std::vector<double>(kappa, kappa + N), std::vector<double>(S, S + N));
+    hip::RtBoundary bc;
+    for (int a = 0; a < 3; ++a) bc.periodic[a] = periodic[a];
+    for (int f = 0; f < 6; ++f) {
+      bc.face[f] = face[f];
+      bc.value[f] = value[f];
+    }
+    *iters = rt.solve(tol, max_iters, bc);
+    for (int n = 0; n < rt.num_rays; ++n) {
+      const std::vector<double> I = rt.intensity(n);
+      std::memcpy(I_out + N * n, I.data(), sizeof(double) * N);
+    }
+    hip::DeviceArray<double> q(N);
+    rt.qrad(q);
+    MISO_HIP_CHECK(misohip_copy_to_host(ctx.handle(), grid.desc(), MISOHIP_F64, q.data(), q_out));
+    ctx.synchronize();
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "cpp_rt_solve: %s\n", e.what());
+    return 2;
+  }
+}
+
 }  // extern "C"

@@ -24,12 +24,17 @@ def _lib():
                                 C.c_int, C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
                                 C.POINTER(C.c_double)]
     L.cpp_run_steps.restype = C.c_int
+    D = C.POINTER(C.c_double)
+    I = C.POINTER(C.c_int)
+    L.cpp_rt_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, I, I, D, C.c_double, C.c_int,
+                               C.c_int, C.c_int, I, D, D]
+    L.cpp_rt_solve.restype = C.c_int
     return L
 
 
 def test_cpp_test_library_exports():
     L = _lib()
-    assert L.cpp_test_hd1d_shock_tube and L.cpp_run_steps
+    assert L.cpp_test_hd1d_shock_tube and L.cpp_run_steps and L.cpp_rt_solve
 
 
 @pytest.mark.gpu
@@ -98,3 +103,47 @@ def test_cpp_layer_matches_oracle(case):
         err = np.abs(out[sl].astype(np.float64) - ref[sl]).max(axis=(1, 2, 3)) / \
             np.maximum(np.abs(ref[sl]).max(axis=(1, 2, 3)), 1e-30)
         assert err.max() < 2e-4, err
+
+
+@pytest.mark.gpu
+def test_cpp_rt_matches_oracle():
+    """miso::hip::RT (rt::RT<double>'s C++ mirror) on the stratified box of
+    test_rt.py: x and y periodic, thermal bottom, vacuum top; iterations,
+    intensities and Q_rad against the RT oracle, and the same bits with
+    every ray x-fastest and one plane per launch."""
+    from oracle import oracle as O
+    sizes = (16, 12, 20)
+    bounds = ((-0.5, 0.5),) * 3
+    og = O.make_grid(sizes, 2, bounds=bounds)
+    rng = np.random.default_rng(7)
+    z = og.axis("z")[None, None, :]
+    T = 2.0 - 1.5 * (z + 0.5) + 0.02 * rng.standard_normal(og.shape)
+    ro = np.exp(-(z + 0.5) / 0.1) * (1 + 0.02 * rng.standard_normal(og.shape))
+    kap = np.ascontiguousarray(50.0 * ro * T ** 3)
+    S = np.ascontiguousarray(T ** 4)
+    kw = dict(periodic=(1, 1, 0), face_kind=(0, 0, 0, 0, 1, 2), tolerance=1e-9, max_iters=300)
+    I_ref = np.zeros((24,) + og.shape)
+    it_ref, _ = O.rt_solve(og, kap, S, I_ref, **kw)
+    q_ref = O.rt_qrad(og, kap, S, I_ref)
+    L = _lib()
+    D = C.POINTER(C.c_double)
+    b = np.array([v for ab in bounds for v in ab], dtype=np.float64)
+    per = (C.c_int * 3)(1, 1, 0)
+    face = (C.c_int * 6)(0, 0, 0, 0, 1, 2)
+    val = (C.c_double * 6)()
+    outs = []
+    for yfast, planes in ((1, 16), (0, 0)):
+        it = C.c_int()
+        I = np.zeros((24,) + og.shape)
+        q = np.zeros(og.shape)
+        rc = L.cpp_rt_solve(*sizes, b.ctypes.data_as(D), kap.ctypes.data_as(D),
+                            S.ctypes.data_as(D), per, face, val, 1e-9, 300, yfast, planes,
+                            C.byref(it), I.ctypes.data_as(D), q.ctypes.data_as(D))
+        assert rc == 0
+        outs.append((it.value, I, q))
+    (it1, I1, q1), (it0, I0, q0) = outs
+    assert it1 == it0 == it_ref
+    assert np.array_equal(I1, I0) and np.array_equal(q1, q0)
+    assert np.abs(I1 - I_ref).max() <= 1e-12 * np.abs(I_ref).max()
+    lat = (slice(1, -2), slice(1, -2), slice(1, -2))
+    assert np.abs(q1[lat] - q_ref[lat]).max() <= 1e-11 * np.abs(q_ref[lat]).max()
