@@ -35,6 +35,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
+#include <string>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -431,12 +432,13 @@ struct RtBoundary {
 /// rt::RT<double> on the device: rint and rint_old (num_rays fields of the
 /// grid), src_func and abs_coeff (x-fastest fields), the angular quadrature
 /// on the host and the lattice bounds ib0..kb1 (rt_host.hpp:127-154).  With
-/// x_rays_yfast (default) the rays whose dominant axis is x keep their field
-/// of rint y-fastest; intensity() returns any ray in MISO's order.
+/// x_rays_yfast (default; fixed for the object's life) the rays whose
+/// dominant axis is x keep their field of rint y-fastest; intensity(),
+/// save() and load() move any ray in MISO's order.
 class RT {
  public:
-  RT(ExecContext<double> &ctx, const Grid<double> &grid, int num_rays)
-      : ctx_(&ctx), grid_(&grid), n_(grid.host().n_total()) {
+  RT(ExecContext<double> &ctx, const Grid<double> &grid, int num_rays, bool x_rays_yfast = true)
+      : ctx_(&ctx), grid_(&grid), n_(grid.host().n_total()), yfast_(x_rays_yfast) {
     weights.resize(24);
     mu_x.resize(24);
     mu_y.resize(24);
@@ -460,6 +462,8 @@ class RT {
       b1[a] = b0[a] + (sz[a] > 1 ? sz[a] : 0);
     }
     ib0 = b0[0]; ib1 = b1[0]; jb0 = b0[1]; jb1 = b1[1]; kb0 = b0[2]; kb1 = b1[2];
+    const misohip_rt_config c = config(0.0, 1, RtBoundary{});
+    MISO_HIP_CHECK(misohip_rt_ray_layout(ctx_->handle(), grid_->desc(), &c, &yfast_mask_));
   }
 
   /// kappa and S from host fields in MISO order ((i * Jt + j) * Kt + k)
@@ -480,8 +484,6 @@ class RT {
     double md = 0.0;
     MISO_HIP_CHECK(misohip_rt_solve(ctx_->handle(), grid_->desc(), &c, abs_coeff.data(),
                                     src_func.data(), rint.data(), rint_old.data(), &it, &md));
-    yfast_mask_ = 0u;
-    MISO_HIP_CHECK(misohip_rt_ray_layout(ctx_->handle(), grid_->desc(), &c, &yfast_mask_));
     last_diff = md;
     if (!(md < tolerance)) std::printf("  RT did not converge in %d iterations.\n", max_iters);
     return it;
@@ -497,29 +499,93 @@ class RT {
 
   /// Ray n's intensities in MISO order ((i * Jt + j) * Kt + k), host copy
   std::vector<double> intensity(int n) const {
-    const HostGrid<double> &h = grid_->host();
-    const std::size_t it = h.i_total(), jt = h.j_total(), kt = h.k_total();
     std::vector<double> dev(n_), out(n_);
     ctx_->synchronize();
     MISO_HIP_RT_CHECK(hipMemcpy(dev.data(), rint.data() + n_ * std::size_t(n),
                                 sizeof(double) * n_, hipMemcpyDeviceToHost));
-    const bool yf = (yfast_mask_ >> n) & 1u;
-    for (std::size_t i = 0; i < it; ++i)
-      for (std::size_t j = 0; j < jt; ++j)
-        for (std::size_t k = 0; k < kt; ++k)
-          out[(i * jt + j) * kt + k] = yf ? dev[(k * it + i) * jt + j] : dev[(k * jt + j) * it + i];
+    reorder(n, dev.data(), out.data(), true);
     return out;
+  }
+  /// Ray n's intensities from MISO order into the solver's layout
+  void set_intensity(int n, const std::vector<double> &miso) {
+    if (miso.size() != n_) throw std::runtime_error("RT intensity size");
+    std::vector<double> dev(n_);
+    reorder(n, miso.data(), dev.data(), false);
+    MISO_HIP_RT_CHECK(hipMemcpy(rint.data() + n_ * std::size_t(n), dev.data(),
+                                sizeof(double) * n_, hipMemcpyHostToDevice));
+  }
+
+  /// RT::save (rt_host.hpp:173-199): int ray count, weights, mu_x, mu_y,
+  /// mu_z, then src_func, abs_coeff and rint, all in MISO order.
+  void save(const std::string &file_path) const {
+    std::FILE *f = std::fopen(file_path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot open " + file_path);
+    std::fwrite(&num_rays, sizeof(int), 1, f);
+    for (const auto *v : {&weights, &mu_x, &mu_y, &mu_z})
+      std::fwrite(v->data(), sizeof(double), v->size(), f);
+    for (const auto *a : {&src_func, &abs_coeff}) {
+      std::vector<double> h(n_);
+      MISO_HIP_CHECK(misohip_copy_to_host(ctx_->handle(), grid_->desc(), MISOHIP_F64, a->data(),
+                                          h.data()));
+      ctx_->synchronize();
+      std::fwrite(h.data(), sizeof(double), n_, f);
+    }
+    for (int n = 0; n < num_rays; ++n) {
+      const std::vector<double> h = intensity(n);
+      std::fwrite(h.data(), sizeof(double), n_, f);
+    }
+    std::fclose(f);
+  }
+  /// RT::load (rt_host.hpp:201-227): the ray count must match.
+  void load(const std::string &file_path) {
+    std::FILE *f = std::fopen(file_path.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open " + file_path);
+    auto rd = [&](void *p, std::size_t sz, std::size_t cnt) {
+      if (std::fread(p, sz, cnt, f) != cnt) {
+        std::fclose(f);
+        throw std::runtime_error("short RT file " + file_path);
+      }
+    };
+    int n = 0;
+    rd(&n, sizeof(int), 1);
+    if (n != num_rays) {
+      std::fclose(f);
+      throw std::runtime_error("RT file ray count does not match");
+    }
+    for (auto *v : {&weights, &mu_x, &mu_y, &mu_z}) rd(v->data(), sizeof(double), v->size());
+    std::vector<double> s(n_), k(n_), I(n_);
+    rd(s.data(), sizeof(double), n_);
+    rd(k.data(), sizeof(double), n_);
+    set_fields(k, s);
+    for (int r = 0; r < num_rays; ++r) {
+      rd(I.data(), sizeof(double), n_);
+      set_intensity(r, I);
+    }
+    std::fclose(f);
   }
 
   DeviceArray<double> rint, rint_old, src_func, abs_coeff;
   std::vector<double> weights, mu_x, mu_y, mu_z;
   int num_rays = 0;
   int ib0 = 0, ib1 = 0, jb0 = 0, jb1 = 0, kb0 = 0, kb1 = 0;
-  bool x_rays_yfast = true;
   int sweep_planes = 16;
   double last_diff = 0.0;
 
  private:
+  // ray n from the solver's layout to MISO order (to_miso), or back
+  void reorder(int n, const double *src, double *dst, bool to_miso) const {
+    const HostGrid<double> &h = grid_->host();
+    const std::size_t it = h.i_total(), jt = h.j_total(), kt = h.k_total();
+    const bool yf = (yfast_mask_ >> n) & 1u;
+    for (std::size_t i = 0; i < it; ++i)
+      for (std::size_t j = 0; j < jt; ++j)
+        for (std::size_t k = 0; k < kt; ++k) {
+          const std::size_t d = yf ? (k * it + i) * jt + j : (k * jt + j) * it + i;
+          const std::size_t m = (i * jt + j) * kt + k;
+          if (to_miso) dst[m] = src[d];
+          else dst[d] = src[m];
+        }
+  }
   misohip_rt_config config(double tolerance, int max_iters, const RtBoundary &bc) const {
     misohip_rt_config c{};
     c.num_rays = num_rays;
@@ -530,13 +596,14 @@ class RT {
     }
     c.tolerance = tolerance;
     c.max_iters = max_iters;
-    c.x_rays_yfast = x_rays_yfast ? 1 : 0;
+    c.x_rays_yfast = yfast_ ? 1 : 0;
     c.sweep_planes = sweep_planes;
     return c;
   }
   ExecContext<double> *ctx_;
   const Grid<double> *grid_;
   std::size_t n_;
+  bool yfast_;
   unsigned yfast_mask_ = 0u;
 };
 

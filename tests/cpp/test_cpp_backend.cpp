@@ -253,8 +253,7 @@ int cpp_rt_solve(int ni, int nj, int nk, const double *bounds, const double *kap
                              bounds[4], bounds[5]);
     hip::ExecContext<double> ctx(0);
     hip::Grid<double> grid(hg);
-    hip::RT rt(ctx, grid, 24);
-    rt.x_rays_yfast = yfast != 0;
+    hip::RT rt(ctx, grid, 24, yfast != 0);
     rt.sweep_planes = sweep_planes;
     const std::size_t N = hg.n_total();
     rt.set_fields(std::vector<double>(kappa, kappa + N), std::vector<double>(S, S + N));
@@ -276,6 +275,33 @@ int cpp_rt_solve(int ni, int nj, int nk, const double *bounds, const double *kap
     return 0;
   } catch (const std::exception &e) {
     std::fprintf(stderr, "cpp_rt_solve: %s\n", e.what());
+    return 2;
+  }
+}
+
+// RT::save then RT::load into a second RT: after a solve on the given
+// kappa, S (24 rays, x and y periodic, thermal bottom, vacuum top), the file
+// is written to `path` and read back; returns 0 when every ray's
+// intensities and the fields come back equal, 1 otherwise.
+int cpp_rt_save_load(int ni, int nj, int nk, const double *kappa, const double *S,
+                     const char *path) {
+  try {
+    hip::HostGrid<double> hg(ni, nj, nk, 2, -0.5, 0.5, -0.5, 0.5, -0.5, 0.5);
+    hip::ExecContext<double> ctx(0);
+    hip::Grid<double> grid(hg);
+    hip::RT a(ctx, grid, 24);
+    const std::size_t N = hg.n_total();
+    a.set_fields(std::vector<double>(kappa, kappa + N), std::vector<double>(S, S + N));
+    a.solve(1e-8, 100, hip::RtBoundary{});
+    a.save(path);
+    hip::RT b(ctx, grid, 24);
+    b.load(path);
+    for (int n = 0; n < 24; ++n)
+      if (a.intensity(n) != b.intensity(n)) return 1;
+    if (a.weights != b.weights || a.mu_x != b.mu_x) return 1;
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "cpp_rt_save_load: %s\n", e.what());
     return 2;
   }
 }

@@ -29,12 +29,15 @@ def _lib():
     L.cpp_rt_solve.argtypes = [C.c_int, C.c_int, C.c_int, D, D, D, I, I, D, C.c_double, C.c_int,
                                C.c_int, C.c_int, I, D, D]
     L.cpp_rt_solve.restype = C.c_int
+    L.cpp_rt_save_load.argtypes = [C.c_int, C.c_int, C.c_int, D, D, C.c_char_p]
+    L.cpp_rt_save_load.restype = C.c_int
     return L
 
 
 def test_cpp_test_library_exports():
     L = _lib()
     assert L.cpp_test_hd1d_shock_tube and L.cpp_run_steps and L.cpp_rt_solve
+    assert L.cpp_rt_save_load
 
 
 @pytest.mark.gpu
@@ -147,3 +150,39 @@ def test_cpp_rt_matches_oracle():
     assert np.abs(I1 - I_ref).max() <= 1e-12 * np.abs(I_ref).max()
     lat = (slice(1, -2), slice(1, -2), slice(1, -2))
     assert np.abs(q1[lat] - q_ref[lat]).max() <= 1e-11 * np.abs(q_ref[lat]).max()
+
+
+@pytest.mark.gpu
+def test_cpp_rt_save_load_round_trip(tmp_path):
+    """miso::hip::RT::save writes RT::save's layout (rt_host.hpp:173-199):
+    int32 ray count, weights and direction cosines (the reference's
+    quadrature vectors), then S, kappa and the intensities in MISO order;
+    load() into a new RT gives every ray back (checked in C++), with the
+    x-dominant rays kept y-fastest on the device."""
+    sizes = (10, 8, 6)
+    shape = tuple(s + 4 for s in sizes)
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(4)
+    kap = np.ascontiguousarray(2.0 + rng.random(shape))
+    S = np.ascontiguousarray(1.0 + rng.random(shape))
+    path = str(tmp_path / "rank_00000000.bin")
+    D = C.POINTER(C.c_double)
+    assert _lib().cpp_rt_save_load(*sizes, kap.ctypes.data_as(D), S.ctypes.data_as(D),
+                                   path.encode()) == 0
+    raw = open(path, "rb").read()
+    assert np.frombuffer(raw[:4], dtype="<i4")[0] == 24
+    body = np.frombuffer(raw[4:], dtype="<f8")
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_limiter_quadrature.npz"))
+    q = ref["quadrature_24"]
+    for c in range(4):
+        assert np.array_equal(body[24 * c:24 * (c + 1)], q[:, c])
+    off = 96
+    assert np.array_equal(body[off:off + n].reshape(shape), S)
+    assert np.array_equal(body[off + n:off + 2 * n].reshape(shape), kap)
+    rint = body[off + 2 * n:].reshape((24,) + shape)
+    assert rint.size == 24 * n and np.isfinite(rint).all()
+    # thermal bottom: the upward rays hold S on the bottom lattice layer
+    kb0 = 1
+    up = [r for r in range(24) if q[r, 3] > 0]
+    assert np.array_equal(rint[up][:, 1:-2, 1:-2, kb0], np.broadcast_to(S[1:-2, 1:-2, kb0],
+                                                                       (len(up),) + S[1:-2, 1:-2, kb0].shape))
