@@ -64,10 +64,10 @@ def parse():
                          "solved values (default: no flags)")
     ap.add_argument("--rmhd-no-correct", action="store_true",
                     help="do not add the correction the last plain solve leaves")
-    ap.add_argument("--rmhd-train-columns", type=int, default=200000)
-    ap.add_argument("--rmhd-train-seeds", type=int, default=4,
+    ap.add_argument("--rmhd-train-columns", type=int, default=400000)
+    ap.add_argument("--rmhd-train-seeds", type=int, default=6,
                     help="boxes (other seeds) the training columns come from")
-    ap.add_argument("--rmhd-epochs", type=int, default=200)
+    ap.add_argument("--rmhd-epochs", type=int, default=400)
     ap.add_argument("--rmhd-slab-tol", type=float, default=None,
                     help="tolerance of the bottom-slab solves (default: the RT tolerance)")
     ap.add_argument("--rmhd-slab-share", type=float, default=0.99,
