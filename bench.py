@@ -113,9 +113,13 @@ def rk_traffic(n, dtype, variant=0):
         return None
     name = "| `" + rk_kernel_name(variant, dtype)
     tot, cnt = 0.0, 0
+    in_traffic = False
     for line in open(path):
+        if line.startswith("| kernel |"):  # a table header: is it the traffic table?
+            in_traffic = "traffic MB" in line
+            continue
         cols = line.split("|")
-        if line.startswith(name) and len(cols) >= 10:  # the traffic table
+        if in_traffic and line.startswith(name) and len(cols) >= 11:
             # instances of one variant (the z-march first stage has its own)
             # are averaged over their dispatches
             tot += float(cols[7]) * 1e6 * int(cols[2])
