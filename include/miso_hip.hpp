@@ -34,6 +34,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -194,8 +196,12 @@ template <typename Real> struct HostGrid {
 template <typename Real> class Grid {
  public:
   explicit Grid(const HostGrid<Real> &h)
-      : host_(h), dx_(h.xg.total), dy_(h.yg.total), dz_(h.zg.total), dxi_(h.xg.total),
-        dyi_(h.yg.total), dzi_(h.zg.total) {
+      : host_(h), x_(h.xg.total), y_(h.yg.total), z_(h.zg.total), dx_(h.xg.total),
+        dy_(h.yg.total), dz_(h.zg.total), dxi_(h.xg.total), dyi_(h.yg.total),
+        dzi_(h.zg.total) {
+    x_.upload(h.xg.s.data(), h.xg.total);
+    y_.upload(h.yg.s.data(), h.yg.total);
+    z_.upload(h.zg.s.data(), h.zg.total);
     dx_.upload(h.xg.ds.data(), h.xg.total);
     dy_.upload(h.yg.ds.data(), h.yg.total);
     dz_.upload(h.zg.ds.data(), h.zg.total);
@@ -212,10 +218,14 @@ template <typename Real> class Grid {
   }
   const misohip_grid *desc() const { return &d_; }
   const HostGrid<Real> &host() const { return host_; }
+  /// cell-centre coordinates on the device, ghosts included
+  const Real *x() const { return x_.data(); }
+  const Real *y() const { return y_.data(); }
+  const Real *z() const { return z_.data(); }
 
  private:
   HostGrid<Real> host_;
-  DeviceArray<Real> dx_, dy_, dz_, dxi_, dyi_, dzi_;
+  DeviceArray<Real> x_, y_, z_, dx_, dy_, dz_, dxi_, dyi_, dzi_;
   misohip_grid d_{};
 };
 
@@ -269,12 +279,17 @@ template <typename Real> class Fields {
 };
 
 /// Boundary descriptor: per axis periodic (the halo self-exchange), mirror
-/// with a sign per field (bc::symmetric, boundary_condition.hpp:108-136) or
+/// with a sign per field (bc::symmetric, boundary_condition.hpp:108-136),
 /// fixed (every fill copies the axis' ghost cells from given fields: a user
-/// boundary functor writing set values, mhd_model_base.hpp:131-155).
+/// boundary functor writing set values, mhd_model_base.hpp:131-155) or user
+/// (a functor run before every fill, BC::apply: see user() and
+/// miso_hip_device.hpp for for_each / symmetric on the device).
 struct Boundary {
-  enum Kind { MIRROR = 0, PERIODIC = 1, FIXED = 2 };
+  enum Kind { MIRROR = 0, PERIODIC = 1, FIXED = 2, USER = 3 };
+  /// the user functor: called with the fields to fill and the stream
+  using UserFn = std::function<void(const misohip_fields &, hipStream_t)>;
   misohip_bc d{};
+  std::shared_ptr<UserFn> fn;
   Boundary(Kind x = PERIODIC, Kind y = PERIODIC, Kind z = PERIODIC) {
     d.periodic[0] = x; d.periodic[1] = y; d.periodic[2] = z;
     for (auto &a : d.sym_sign)
@@ -290,6 +305,37 @@ struct Boundary {
     for (int f = 0; f < 9; ++f) d.fixed[f] = fields.field(f);
     return *this;
   }
+  /// the functor of the USER axes (BC::apply): f(const misohip_fields &,
+  /// hipStream_t) enqueues its kernels on the stream it is given
+  template <typename F> Boundary &user(F f) {
+    fn = std::make_shared<UserFn>(std::move(f));
+    return *this;
+  }
+  /// C callback over fn (misohip_set_bc_callback); exceptions stop the step
+  static int call(void *user, const misohip_fields *q, void *stream) {
+    try {
+      (*static_cast<const UserFn *>(user))(*q, static_cast<hipStream_t>(stream));
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  }
+};
+
+/// Registers a boundary's user functor on the context for one call.
+class BoundaryScope {
+ public:
+  BoundaryScope(misohip_ctx *ctx, const Boundary &bc) : ctx_(bc.fn ? ctx : nullptr) {
+    if (ctx_) MISO_HIP_CHECK(misohip_set_bc_callback(ctx_, &Boundary::call, bc.fn.get()));
+  }
+  BoundaryScope(const BoundaryScope &) = delete;
+  BoundaryScope &operator=(const BoundaryScope &) = delete;
+  ~BoundaryScope() {
+    if (ctx_) misohip_set_bc_callback(ctx_, nullptr, nullptr);
+  }
+
+ private:
+  misohip_ctx *ctx_;
 };
 
 /// Source descriptors (the user Source functor's two forms here).
@@ -328,6 +374,7 @@ template <typename Real> class Integrator {
   }
 
   template <typename BC> void apply_boundary_condition(const BC &bc, Fields<Real> &qq) {
+    BoundaryScope scope(ctx_.handle(), bc);
     MISO_HIP_CHECK(misohip_ghost_fill(ctx_.handle(), grid_.desc(), dtype_of<Real>(), qq.desc(),
                                       &bc.d));
   }
@@ -342,6 +389,7 @@ template <typename Real> class Integrator {
   template <typename EOS, typename BC, typename Source>
   void update(Real dt, const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq) {
     misohip_params p = params(eos, src);
+    BoundaryScope scope(ctx_.handle(), bc);
     MISO_HIP_CHECK(misohip_update(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
                                   qq.desc(), &w_, double(dt)));
   }
@@ -352,6 +400,7 @@ template <typename Real> class Integrator {
   Real step(const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq) {
     misohip_params p = params(eos, src);
     double dt = 0.0;
+    BoundaryScope scope(ctx_.handle(), bc);
     MISO_HIP_CHECK(misohip_step(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
                                 qq.desc(), &w_, &dt));
     return Real(dt);
@@ -364,6 +413,7 @@ template <typename Real> class Integrator {
                           int nsteps) {
     misohip_params p = params(eos, src);
     std::vector<double> dts(nsteps > 0 ? nsteps : 0);
+    BoundaryScope scope(ctx_.handle(), bc);
     MISO_HIP_CHECK(misohip_run(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
                                qq.desc(), &w_, nsteps, dts.data()));
     return dts;

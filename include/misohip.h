@@ -54,6 +54,7 @@ extern "C" {
 #define MISOHIP_E_DTYPE (-3)    /* unknown dtype                        */
 #define MISOHIP_E_NOMEM (-4)    /* device allocation failed             */
 #define MISOHIP_E_RANGE (-5)    /* size beyond a documented limit       */
+#define MISOHIP_E_CALLBACK (-6) /* the user boundary callback failed    */
 
 typedef enum misohip_dtype { MISOHIP_F64 = 0, MISOHIP_F32 = 1 } misohip_dtype;
 
@@ -85,12 +86,29 @@ typedef struct misohip_fields {
  * ghost cells are read) -- the user boundary functor that writes given
  * values into the ghost zone (mhd_model_base.hpp:131-155 contract; the
  * reference applies the user functor before the halo exchange, so edges
- * shared with a periodic axis take the periodic image). */
+ * shared with a periodic axis take the periodic image); 3: user, the axis'
+ * ghost cells are left to the context's boundary callback
+ * (misohip_set_bc_callback), which every fill calls first, and the fill maps
+ * only the other axes. */
 typedef struct misohip_bc {
   int periodic[3];
   int sym_sign[3][9];
   const void *fixed[9];
 } misohip_bc;
+
+/* A user boundary condition (BC::apply, run before the halo exchange,
+ * mhd_integrator_host.hpp:250-255; its device form is a functor launched
+ * with for_each / bc::symmetric, demo/mhd3d_magnetosphere/src/bc.hpp:39-158).
+ * Every ghost fill of the context -- misohip_ghost_fill and the fills inside
+ * misohip_update, misohip_step and misohip_run -- calls fn(user, fields,
+ * stream) first; fn enqueues its device work on `stream` (the context's)
+ * and returns 0, or nonzero to stop (the call then returns
+ * MISOHIP_E_CALLBACK).  fn = NULL removes it.  A misohip_bc with an axis of
+ * kind 3 needs one.  With a callback set, the x and y AV sweeps run as two
+ * passes with a fill between them, as in the reference.  The reference has
+ * no such entry point: there BC is a template argument. */
+typedef int (*misohip_bc_fn)(void *user, const struct misohip_fields *q, void *stream);
+int misohip_set_bc_callback(misohip_ctx *ctx, misohip_bc_fn fn, void *user);
 
 /* Config values the step consumes (config_defaults.hpp, eos.hpp) and the
  * source term: src_kind 0 = none (EmptySourceTerm), 1 = constant gravity

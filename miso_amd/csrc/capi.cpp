@@ -86,10 +86,19 @@ bool bc_invalid(const misohip_bc *bc) {
   return false;
 }
 
+// an axis of kind 3 (user) needs the context's boundary callback
+bool bc_invalid(const misohip_ctx *ctx, const misohip_bc *bc) {
+  if (bc_invalid(bc)) return true;
+  for (int a = 0; a < 3; ++a)
+    if (bc->periodic[a] == 3 && !ctx->bc_fn) return true;
+  return false;
+}
+
 BcDev make_bc(const misohip_bc *bc) {
   BcDev b;
   for (int a = 0; a < 3; ++a) {
-    b.periodic[a] = bc->periodic[a] == 2 ? 2 : (bc->periodic[a] ? 1 : 0);
+    const int k = bc->periodic[a];
+    b.periodic[a] = k == 2 ? 2 : k == 3 ? 3 : (k ? 1 : 0);
     for (int f = 0; f < 9; ++f) b.sign[a][f] = bc->sym_sign[a][f] >= 0 ? 1.0f : -1.0f;
   }
   for (int f = 0; f < 9; ++f) b.fixed[f] = bc->fixed[f];
@@ -155,6 +164,9 @@ int cfl_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
 template <typename R>
 int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const BcDev &b) {
   Timed t_(ctx, MISOHIP_KIND_GHOST);
+  // the user boundary first, then the fill (BC::apply, then the halo
+  // exchange: mhd_integrator_host.hpp:250-255)
+  if (ctx->bc_fn && ctx->bc_fn(ctx->bc_user, q, ctx->stream) != 0) return MISOHIP_E_CALLBACK;
   MISOHIP_TRY(launch_ghost_fill<R>(ctx->stream, fld<R>(q), sh, b));
   return MISOHIP_OK;
 }
@@ -196,7 +208,10 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   // In 3D runs the x and y sweeps can run as one walk (tile choice
   // av_fuse_xy): the y walk x-sweeps every row it takes in.
   const bool fuse_z = g->is && g->js && !g->ks;
-  const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2;
+  // (not with a user boundary: the y sweep then needs the callback's ghost
+  // rows of the x-swept state)
+  const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2 &&
+                       !ctx->bc_fn;
   const misohip_fields *cur = argm;
   const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
   for (int dir = 0; dir < 3; ++dir) {
@@ -284,6 +299,7 @@ const char *misohip_strerror(int code) {
     case MISOHIP_E_DTYPE: return "unknown dtype";
     case MISOHIP_E_NOMEM: return "device allocation failed";
     case MISOHIP_E_RANGE: return "size beyond the supported maximum";
+    case MISOHIP_E_CALLBACK: return "the user boundary callback failed";
     default: return "unknown error";
   }
 }
@@ -372,6 +388,13 @@ int misohip_destroy(misohip_ctx *ctx) {
   return MISOHIP_OK;
 }
 
+int misohip_set_bc_callback(misohip_ctx *ctx, misohip_bc_fn fn, void *user) {
+  if (!ctx) return MISOHIP_E_ARG;
+  ctx->bc_fn = fn;
+  ctx->bc_user = fn ? user : nullptr;
+  return MISOHIP_OK;
+}
+
 int misohip_set_stream(misohip_ctx *ctx, void *stream) {
   if (!ctx) return MISOHIP_E_ARG;
   ctx->stream = static_cast<hipStream_t>(stream);
@@ -438,7 +461,7 @@ int misohip_copy_to_host(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype 
 int misohip_ghost_fill(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                        const misohip_fields *qq, const misohip_bc *bc) {
   MISOHIP_COMMON_CHECKS(ctx, g, dt)
-  if (check_fields(qq) || bc_invalid(bc)) return MISOHIP_E_ARG;
+  if (check_fields(qq) || bc_invalid(ctx, bc)) return MISOHIP_E_ARG;
   const BcDev b = make_bc(bc);
   if (dt == MISOHIP_F64) return ghost_t<double>(ctx, qq, make_shape(g), b);
   return ghost_t<float>(ctx, qq, make_shape(g), b);
@@ -491,7 +514,7 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                    const misohip_params *p, const misohip_bc *bc, const misohip_fields *qq,
                    const misohip_work *w, double dt_step) {
   MISOHIP_COMMON_CHECKS(ctx, g, dt)
-  if (!p || bc_invalid(bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
+  if (!p || bc_invalid(ctx, bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
       check_fields(&w->rslt) ||
       !w->cc)
     return MISOHIP_E_ARG;
@@ -506,7 +529,7 @@ int misohip_run(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                 const misohip_params *p, const misohip_bc *bc, const misohip_fields *qq,
                 const misohip_work *w, int nsteps, double *dts) {
   MISOHIP_COMMON_CHECKS(ctx, g, dt)
-  if (!p || bc_invalid(bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
+  if (!p || bc_invalid(ctx, bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
       check_fields(&w->rslt) || !w->cc || nsteps < 0)
     return MISOHIP_E_ARG;
   if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;

@@ -25,6 +25,8 @@ struct misohip_ctx {
   double *rt_kap_y = nullptr, *rt_src_y = nullptr;
   size_t rt_cells = 0;
   unsigned *rt_parts = nullptr;  // the RT change-reduction slots
+  misohip_bc_fn bc_fn = nullptr;  // the user boundary callback and its argument
+  void *bc_user = nullptr;
 };
 
 namespace misohip_detail {

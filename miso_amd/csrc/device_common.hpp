@@ -83,7 +83,7 @@ struct Pencils {
 // and takes its value from the fixed fields)
 __device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool &mirrored) {
   mirrored = false;
-  if (m == 0 || kind == 2) return x;
+  if (m == 0 || kind >= 2) return x;  // fixed (2) and user (3) axes keep their ghosts
   const int n = total - 2 * m;
   if (x < m) {
     if (kind == 1) return x + n;

@@ -88,6 +88,7 @@ SIGNATURES = {
                         C.c_void_p, C.c_void_p, C.c_long],
     "misohip_table2d": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                         C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long],
+    "misohip_set_bc_callback": [C.c_void_p, C.c_void_p, C.c_void_p],
     "misohip_rt_quadrature": [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                               _P(C.c_double)],
     "misohip_rt_solve": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,
