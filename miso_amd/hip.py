@@ -59,6 +59,10 @@ class MisohipWork(C.Structure):
     _fields_ = [("argm", MisohipFields), ("rslt", MisohipFields), ("cc", C.c_void_p)]
 
 
+# int (*)(void *user, const misohip_fields *q, void *stream)
+BC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(MisohipFields), C.c_void_p)
+
+
 # (name, argtypes) of every exported entry point of include/misohip.h
 _P = C.POINTER
 SIGNATURES = {
@@ -88,7 +92,7 @@ SIGNATURES = {
                         C.c_void_p, C.c_void_p, C.c_long],
     "misohip_table2d": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                         C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long],
-    "misohip_set_bc_callback": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "misohip_set_bc_callback": [C.c_void_p, BC_FN, C.c_void_p],
     "misohip_rt_quadrature": [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                               _P(C.c_double)],
     "misohip_rt_solve": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,

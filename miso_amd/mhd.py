@@ -14,9 +14,11 @@ libmisohip.so through the C ABI.
 """
 from __future__ import annotations
 
+import contextlib
 import copy
 import ctypes as C
 import math
+import traceback
 
 import numpy as np
 
@@ -217,19 +219,32 @@ class ExecContext:
 class BoundaryCondition:
     """Per-axis boundary: periodic self-exchange (True / 1), mirror with
     per-field signs (False / 0; boundary_condition.hpp:108-136, signs +1
-    unless given) or fixed ghost values ("fixed" / 2: every fill copies the
+    unless given), fixed ghost values ("fixed" / 2: every fill copies the
     axis' ghost cells from the `fixed` Fields -- a user boundary functor
-    writing given values, the mhd_model_base.hpp:131-155 contract)."""
+    writing given values, the mhd_model_base.hpp:131-155 contract) or user
+    ("user" / 3: `user(q)` runs before every fill -- BC::apply before the
+    halo exchange, mhd_integrator_host.hpp:250-255 -- with q the (9, Kt, Jt,
+    It) tensor being filled, and sets that axis' ghost cells with torch
+    operations on the current stream)."""
 
     FIXED = 2
+    USER = 3
 
-    def __init__(self, periodic=(True, True, True), sym_sign=None, fixed=None):
-        self.periodic = tuple(2 if (p == "fixed" or (not isinstance(p, bool) and p == 2))
-                              else int(bool(p)) for p in periodic)
+    def __init__(self, periodic=(True, True, True), sym_sign=None, fixed=None, user=None):
+        def kind(p):
+            if p == "fixed" or (not isinstance(p, bool) and p == 2):
+                return 2
+            if p == "user" or (not isinstance(p, bool) and p == 3):
+                return 3
+            return int(bool(p))
+        self.periodic = tuple(kind(p) for p in periodic)
         self.sym_sign = sym_sign or ((1,) * 9,) * 3
         self.fixed = fixed  # Fields holding the ghost values of fixed axes
+        self.user = user
         if 2 in self.periodic and fixed is None:
             raise ValueError("a fixed boundary needs the Fields that hold its ghost values")
+        if 3 in self.periodic and user is None:
+            raise ValueError("a user boundary needs its function")
 
     def c_struct(self) -> hip.MisohipBc:
         b = hip.MisohipBc()
@@ -272,10 +287,36 @@ class Integrator:
         w.cc = self.cc.data_ptr()
         self._work = w
 
+    # -- user boundaries -------------------------------------------------------
+    @contextlib.contextmanager
+    def _user_bc(self, bc: BoundaryCondition, qq: Fields):
+        """bc.user as the context's boundary callback for one call: the
+        fields a fill hands over are one of qq and the two workspaces."""
+        if bc.user is None:
+            yield
+            return
+        tensors = {f.data[0].data_ptr(): f.data for f in (qq, self.qq_argm, self.qq_rslt)}
+
+        def cb(_user, q, _stream):
+            try:
+                bc.user(tensors[q.contents.q[0]])
+                return 0
+            except Exception:  # noqa: BLE001 -- reported, then the call fails
+                traceback.print_exc()
+                return 1
+
+        fn = hip.BC_FN(cb)
+        hip.call("misohip_set_bc_callback", self.ctx.handle, fn, None)
+        try:
+            yield
+        finally:
+            hip.lib().misohip_set_bc_callback(self.ctx.handle, hip.BC_FN(), None)
+
     # -- per-function entry points ------------------------------------------
     def apply_boundary_condition(self, bc: BoundaryCondition, qq: Fields) -> None:
-        hip.call("misohip_ghost_fill", self.ctx.handle, C.byref(self.grid.c_struct()),
-                 self.dtype_code, C.byref(qq.c_struct()), C.byref(bc.c_struct()))
+        with self._user_bc(bc, qq):
+            hip.call("misohip_ghost_fill", self.ctx.handle, C.byref(self.grid.c_struct()),
+                     self.dtype_code, C.byref(qq.c_struct()), C.byref(bc.c_struct()))
 
     def divb_parameters(self, dt: float):
         """divb_parameters_set (mhd_integrator_host.hpp:337-341) in the run's Real."""
@@ -306,24 +347,27 @@ class Integrator:
         return out.value
 
     def update(self, dt: float, bc: BoundaryCondition, qq: Fields) -> None:
-        hip.call("misohip_update", self.ctx.handle, C.byref(self.grid.c_struct()),
-                 self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
-                 C.byref(qq.c_struct()), C.byref(self._work), float(dt))
+        with self._user_bc(bc, qq):
+            hip.call("misohip_update", self.ctx.handle, C.byref(self.grid.c_struct()),
+                     self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
+                     C.byref(qq.c_struct()), C.byref(self._work), float(dt))
 
     def run(self, bc: BoundaryCondition, qq: Fields, nsteps: int):
         """nsteps of ModelBase::update (cfl + update) on qq, the CFL of each
         later step reduced by the previous step's last sweep; returns the dts."""
         dts = (C.c_double * max(1, nsteps))()
-        hip.call("misohip_run", self.ctx.handle, C.byref(self.grid.c_struct()), self.dtype_code,
-                 C.byref(self.params), C.byref(bc.c_struct()), C.byref(qq.c_struct()),
-                 C.byref(self._work), int(nsteps), dts)
+        with self._user_bc(bc, qq):
+            hip.call("misohip_run", self.ctx.handle, C.byref(self.grid.c_struct()),
+                     self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
+                     C.byref(qq.c_struct()), C.byref(self._work), int(nsteps), dts)
         return list(dts)[:nsteps]
 
     def step(self, bc: BoundaryCondition, qq: Fields) -> float:
         out = C.c_double()
-        hip.call("misohip_step", self.ctx.handle, C.byref(self.grid.c_struct()), self.dtype_code,
-                 C.byref(self.params), C.byref(bc.c_struct()), C.byref(qq.c_struct()),
-                 C.byref(self._work), C.byref(out))
+        with self._user_bc(bc, qq):
+            hip.call("misohip_step", self.ctx.handle, C.byref(self.grid.c_struct()),
+                     self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
+                     C.byref(qq.c_struct()), C.byref(self._work), C.byref(out))
         return out.value
 
 

@@ -54,3 +54,40 @@ def test_user_functor_matches_descriptor(ctx, case):
     assert np.array_equal(da, db) and (da > 0).all()
     assert np.array_equal(a, b)
     assert not np.array_equal(a, q)  # the run moved the state
+
+
+@pytest.mark.gpu
+def test_python_user_boundary_matches_descriptor(ctx):
+    """The Python mirror's user boundary ("user" axis, a function of the
+    (9, Kt, Jt, It) tensor being filled, torch operations on the stream):
+    a z mirror with odd vz and bz written with torch gives the mirror
+    descriptor's bits over a 3-step run, every field and dt."""
+    import torch
+
+    from miso_amd import mhd, problems
+    g = mhd.Grid(20, 16, 12, 2, device="cuda:0")
+    q0 = problems.turbulence(g, device="cpu", seed=5)
+    odd = (0, 0, 0, 1, 0, 0, 1, 0, 0)
+    sign = torch.tensor([-1.0 if o else 1.0 for o in odd], dtype=torch.float64,
+                        device="cuda:0").view(9, 1, 1)
+    m_, kt = g.k_margin, g.k_total
+
+    def z_mirror(t):
+        for layer in range(m_):
+            t[:, layer] = sign * t[:, 2 * m_ - 1 - layer]
+            t[:, kt - m_ + layer] = sign * t[:, kt - m_ - 1 - layer]
+
+    signs = ((1,) * 9, (1,) * 9, tuple(-1 if o else 1 for o in odd))
+    runs = []
+    for bc in (mhd.BoundaryCondition((True, True, False), sym_sign=signs),
+               mhd.BoundaryCondition((True, True, "user"), user=z_mirror)):
+        m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
+        m.qq.load_ijk(q0)
+        m.integrator.apply_boundary_condition(bc, m.qq)
+        d = m.integrator.run(bc, m.qq, 3)
+        torch.cuda.synchronize()
+        runs.append((d, m.qq.data.clone()))
+    assert runs[0][0] == runs[1][0]
+    assert torch.equal(runs[0][1], runs[1][1])
+    with pytest.raises(ValueError):
+        mhd.BoundaryCondition((True, True, "user"))
