@@ -170,6 +170,19 @@ def cpu_baseline(target_s: float):
         el = time.perf_counter() - t0
         if el >= target_s or steps >= 50:
             break
+    # one more step on one thread (the same sample's per-thread rate)
+    O.set_threads(1)
+    t1 = time.perf_counter()
+    O.run(og, p, q, 1)
+    el1 = time.perf_counter() - t1
+    O.set_threads(threads)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                         "")
+    except OSError:
+        pass
     return {
         "value": n ** 3 * steps / el,
         "unit": "cell-updates/s",
@@ -177,6 +190,8 @@ def cpu_baseline(target_s: float):
         "kind": "port",
         "sample": f"3D turbulence {n}^3 FP64 (same IC family), {steps} steps in {el:.1f} s, "
                   f"{threads} OpenMP threads",
+        "single_thread_value": n ** 3 / el1,
+        "host": {"cpu": model, "nproc": os.cpu_count()},
     }
 
 
