@@ -311,6 +311,26 @@ def accuracy_leg(args):
                         "max_rel_linf": max(e["rel_linf"] for e in err.values()),
                         "div_b_rel_start": div0,
                         "div_b_rel_end": _div_b(q, 1.0 / n, 1.0 / n, 1.0)}
+        if dt == np.float64:
+            # the same 200 steps as a one-run batch replayed from a hipGraph
+            # (dt stays on the device: no host round trip per step)
+            from miso_amd.ensemble import Ensemble
+            q0t = torch.from_numpy(q0)
+            ens = Ensemble(ctx, [(n, n, 1)], lambda _m, _g: q0t, bc=bc, dtype=np.float64,
+                           cfl_number=0.4, gm=5.0 / 3.0,
+                           bounds=((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)))
+            ens.capture(50)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ens.step(nsteps)
+            torch.cuda.synchronize()
+            graph_s = time.perf_counter() - t0
+            qg = ens.runs[0].qq.to_host(ctx)
+            out[dt_name]["graph"] = {
+                "gpu_seconds": graph_s, "gpu_cell_updates_per_s": n * n * nsteps / graph_s,
+                "same_bits_as_step_by_step": bool(np.array_equal(qg, m.qq.to_host(ctx))),
+                "last_dt_equal": bool(float(ens.state()[0][0]) == float(dts[-1]))}
+            ens.close()
         del m
     print(f"[accuracy] fp64 max rel linf {out['f64']['max_rel_linf']:.2e}, "
           f"fp32 {out['f32']['max_rel_linf']:.2e}", file=sys.stderr, flush=True)

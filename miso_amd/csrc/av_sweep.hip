@@ -790,11 +790,11 @@ static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld
                                 const AvScalars &sc, const R *dz_arr = nullptr) {
   const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
   const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
-  const int nchunks = (n_dir + (sc.chunk < 4 ? 4 : sc.chunk) - 1) / (sc.chunk < 4 ? 4 : sc.chunk);
-  const int chunk = (n_dir + nchunks - 1) / nchunks;  // even walk lengths
   const int wx = walk_waves_x(n_oth);
   const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
+  int chunk, nchunks;  // even walk lengths
+  walk_split(n_dir, 256L * nbx * nbo, sc.chunk, chunk, nchunks);
   const long nbl = (long)nchunks * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   hipLaunchKernelGGL((av_walk_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
@@ -852,11 +852,11 @@ static hipError_t launch_walk_cfl_t(hipStream_t st, const Fld<const R> &q, const
                                     const AvScalars &sc, const R *dz_arr, const CflFuse<R> &cf) {
   const int n_dir = DIR == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
   const int n_oth = DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
-  const int nchunks = (n_dir + (sc.chunk < 4 ? 4 : sc.chunk) - 1) / (sc.chunk < 4 ? 4 : sc.chunk);
-  const int chunk = (n_dir + nchunks - 1) / nchunks;  // even walk lengths
   const int wx = walk_waves_x(n_oth);
   const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
+  int chunk, nchunks;  // even walk lengths
+  walk_split(n_dir, 256L * nbx * nbo, sc.chunk, chunk, nchunks);
   const long nbl = (long)nchunks * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   hipLaunchKernelGGL((av_walk_cfl_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st,
@@ -883,9 +883,9 @@ hipError_t launch_av_sweep_xy(hipStream_t st, const Fld<const R> &q, const Fld<R
                               const BcDev &bc, const AvScalars &sc) {
   const int nj = sh.Jt - 2 * sh.mj;
   if (!sh.is || !sh.js || !sh.ks || nj < 2 || sh.mj < 2) return hipErrorInvalidValue;
-  const int nchunks = (nj + (sc.chunk < 4 ? 4 : sc.chunk) - 1) / (sc.chunk < 4 ? 4 : sc.chunk);
-  const int chunk = (nj + nchunks - 1) / nchunks;  // even walk lengths
   const int nseg = (sh.It - 2 * sh.mi + 59) / 60;
+  int chunk, nchunks;  // even walk lengths
+  walk_split(nj, 64L * nseg * (sh.Kt - 2 * sh.mk + 2), sc.chunk, chunk, nchunks);
   const long nw = (long)nseg * nchunks * (sh.Kt - 2 * sh.mk + 2);
   const long nbl = (nw + 3) / 4;
   if (nbl > 2147483647L) return hipErrorInvalidValue;

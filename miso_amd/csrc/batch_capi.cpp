@@ -137,7 +137,7 @@ int create_t(misohip_ctx *ctx, int n, const misohip_grid *grids, const misohip_f
   const int chunk = ctx->tile.sweep_chunk < 4 ? 4 : ctx->tile.sweep_chunk;
   BcDev bd{};
   for (int a = 0; a < 3; ++a) {
-    if (bc->periodic[a] == 2) {  // fixed-value boundaries are not batched
+    if (bc->periodic[a] == 2 || bc->periodic[a] == 3) {  // fixed and user boundaries are not batched
       delete b;
       return MISOHIP_E_ARG;
     }
@@ -175,7 +175,10 @@ int create_t(misohip_ctx *ctx, int n, const misohip_grid *grids, const misohip_f
     I.dyi = static_cast<const R *>(g->dyi); I.dzi = static_cast<const R *>(g->dzi);
     I.sh = shape_of(g);
     I.min_dxyz = g->min_dxyz;
-    I.chunk = chunk;
+    // this run's y walk alone decides (the tuner's per-run choice replaces it)
+    int nch = 0;
+    walk_split(g->j_size, 64L * ((g->i_size + 63) / 64) * (g->ks ? g->k_size : 1), chunk, I.chunk,
+               nch);
   }
   fill_table(b->tab);
   hipError_t e = hipMalloc(reinterpret_cast<void **>(&b->d_inst), sizeof(BatchInst<R>) * n);
