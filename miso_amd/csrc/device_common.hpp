@@ -101,11 +101,17 @@ __device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool
 // Blocks of the z-march kernel: (nbx, nby) columns of 64 x 4 cells, each cut
 // into nzc chunks of kz planes; x fastest, then y, then the chunk, so that
 // with xcd_virtual_block every XCD walks compact x-y regions of one slab.
+// The requested chunk is halved (not below 4 planes) while there would be
+// fewer blocks than the 256 CUs (a small grid in long chunks would leave
+// most of the chip idle); every chunk loads its own first planes, so the
+// results do not depend on the split.
 struct March {
   int nbx, nby, nzc, kz, nb;
   __host__ bool init(int nbx_, int nby_, int nk, int kz_) {
     if (kz_ < 1 || nk < 1) return false;
     nbx = nbx_; nby = nby_;
+    while (kz_ > 4 && (long)nbx * nby * ((nk + kz_ - 1) / kz_) < 256)
+      kz_ = (kz_ + 1) / 2 < 4 ? 4 : (kz_ + 1) / 2;
     nzc = (nk + kz_ - 1) / kz_;
     kz = (nk + nzc - 1) / nzc;  // the chunks as even as the count allows
     const long n = (long)nbx * nby * nzc;

@@ -257,6 +257,9 @@ def _run_both(ctx, problem, sizes, nsteps, dtype=np.float64, periodic=(1, 1, 1),
 STEP_CASES = {
     "orszag_tang_64": (problems.orszag_tang, (64, 64, 1), 20, {}),
     "turbulence_24": (problems.turbulence, (24, 24, 24), 5, {}),
+    # 30 planes: the z-march's chunks on a small grid (4 planes) end in a
+    # short one, and the walks' chunks do not divide the axes
+    "turbulence_20x16x30": (problems.turbulence, (20, 16, 30), 3, {}),
     "kh_48x32_mirror_y": (
         functools.partial(problems.kelvin_helmholtz, seed=3), (48, 32, 1), 10,
         dict(periodic=(1, 0, 1), gm=1.4, bounds=((-0.5, 0.5), (-0.5, 0.5), (-0.5, 0.5)),
@@ -292,8 +295,8 @@ def test_run_fp64(ctx, case):
     assert_bitwise(out, ref, f"run {case}")
 
 
-@pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
-                                  "stratified_gravity_16"])
+@pytest.mark.parametrize("case", ["turbulence_24", "turbulence_20x16x30",
+                                  "stratified_box_fixed_z_16x16x64", "stratified_gravity_16"])
 @pytest.mark.parametrize("variant,zchunk", [(4, 5), (4, 32), (5, 5), (5, 32)])
 def test_step_fp64_z_march(ctx, case, variant, zchunk):
     """The z-march RK kernels (variants 4 and 5) against the oracle, bit for bit."""
