@@ -100,14 +100,14 @@ hipError_t launch_cfl_init(hipStream_t, unsigned *);
 __host__ __device__ inline int walk_waves_x(int n_other) { return n_other < 4 ? 4 : 1; }
 
 // Chunks of a walk over n_dir cells: the requested length (at least 4),
-// halved while the launch would have fewer than 2^16 threads (a small grid
+// halved while the launch would have fewer than 2^18 threads (a small grid
 // walked in long chunks leaves the chip idle), then evened out.  Every
 // chunk recomputes its first slopes and flux from the fields, so the
 // results do not depend on the split.
 inline void walk_split(int n_dir, long threads_per_chunk, int chunk_req, int &chunk,
                        int &nchunks) {
   int c = chunk_req < 4 ? 4 : chunk_req;
-  while (c > 4 && (long)((n_dir + c - 1) / c) * threads_per_chunk < (1L << 16)) c = (c + 1) / 2;
+  while (c > 4 && (long)((n_dir + c - 1) / c) * threads_per_chunk < (1L << 18)) c = (c + 1) / 2;
   if (c < 4) c = 4;
   nchunks = (n_dir + c - 1) / c;
   chunk = (n_dir + nchunks - 1) / nchunks;
