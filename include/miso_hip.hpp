@@ -2,8 +2,8 @@
 //
 // The classes below carry MISO's names, constructor arguments and method
 // signatures for the MHD time step so that a MISO model can run on the HIP
-// backend by changing its Backend type (see INTEGRATION.md for the
-// specialisations MISO itself would add):
+// backend by changing its Backend type; MISO's own backend::HIP
+// specialisations would forward to them:
 //
 //   miso::hip::ExecContext<Real>     ExecContext<Real, backend::CUDA>
 //                                    (miso/include/miso/mhd_core.hpp:15-33)
