@@ -183,6 +183,20 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                    const misohip_fields *qq, const misohip_work *w,
                    double dt_step);
 
+/* Integrator::update, then the operator-split radiative heating
+ * ei += dt * qrad / ro on the interior (misohip_rt_heat) and the ghost fill,
+ * and, when next_dt is not NULL, Integrator::cfl of the heated state into
+ * *next_dt (host-synchronising).  With x and y active the last AV sweep
+ * heats each cell as it stores it and reduces the CFL candidates from the
+ * heated values, so the heating and CFL passes disappear; the results are
+ * the same bits as misohip_update, misohip_rt_heat, misohip_ghost_fill and
+ * misohip_cfl in sequence.  FP64 only.  No counterpart in the reference,
+ * which has no radiative heating. */
+int misohip_update_heated(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                          const misohip_params *p, const misohip_bc *bc,
+                          const misohip_fields *qq, const misohip_work *w, double dt_step,
+                          const void *qrad, double *next_dt);
+
 /* nsteps of ModelBase::update on the same fields (the loop of
  * ModelBase::run, mhd_model_base.hpp:89-117): from the second step on, the
  * CFL reduction is carried out by the previous step's last AV sweep on the

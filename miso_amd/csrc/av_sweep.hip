@@ -396,6 +396,11 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
     if (CFLF) {
       R st[9];
       av_cell_update<R, DEGEN>(a0, fdw, fup, dd, dt, rslt, c, cc_0, ep, fh, dz0, st);
+      if (cf->heat_q) {  // the heating kernel's expression on the stored values
+        const double e = double(st[7]) + cf->heat_dt * cf->heat_q[c] / double(st[0]);
+        st[7] = R(e);
+        rslt.q[7][c] = st[7];
+      }
       const int jc = DIR == 1 ? m_dir + p0 + t : j, kc = DIR == 2 ? m_dir + p0 + t : k;
       const R hmin = nmin3(cf->dx[i], cf->dy[jc], cf->dz[kc]);
       const float cand = cfl_candidate_v(st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7],

@@ -178,7 +178,7 @@ int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const Bc
 template <typename R>
 int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
              const misohip_bc *bc, const misohip_fields *qq, const misohip_work *w, R dt,
-             bool fuse_cfl = false) {
+             bool fuse_cfl = false, const double *heat_q = nullptr) {
   const R cfl_number = (R)p->cfl_number;
   const R min_dxyz = (R)g->min_dxyz;
   // divb_parameters_set (337-341)
@@ -235,6 +235,8 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
       cf.gm = (double)(R)p->gm;
       cf.cfl = (double)(R)p->cfl_number;
       cf.slot = ctx->d_slot;
+      cf.heat_q = heat_q;
+      cf.heat_dt = (double)dt;
       MISOHIP_TRY(launch_cfl_init(ctx->stream, ctx->d_slot));
       MISOHIP_TRY(launch_av_last_sweep_cfl<R>(
           ctx->stream, cfld<R>(cur), fld<R>(dst), static_cast<const R *>(w->cc),
@@ -522,6 +524,45 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
   if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
   if (dt == MISOHIP_F64) return update_t<double>(ctx, g, p, bc, qq, w, dt_step);
   return update_t<float>(ctx, g, p, bc, qq, w, (float)dt_step);
+}
+
+int misohip_update_heated(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                          const misohip_params *p, const misohip_bc *bc,
+                          const misohip_fields *qq, const misohip_work *w, double dt_step,
+                          const void *qrad, double *next_dt) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || bc_invalid(ctx, bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
+      check_fields(&w->rslt) || !w->cc || !qrad || dt != MISOHIP_F64)
+    return MISOHIP_E_ARG;
+  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
+  const double *q = static_cast<const double *>(qrad);
+  if (g->is && g->js) {  // the last AV sweep heats each cell it stores and reduces the CFL
+    int rc = update_t<double>(ctx, g, p, bc, qq, w, dt_step, true, q);
+    if (rc) return rc;
+    if (next_dt) {
+      MISOHIP_TRY(hipMemcpyAsync(ctx->h_slot, ctx->d_slot, sizeof(unsigned),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+      MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+      float f;
+      std::memcpy(&f, ctx->h_slot, sizeof(float));
+      *next_dt = (double)f;
+    }
+    return MISOHIP_OK;
+  }
+  // otherwise the same operations one after another
+  int rc = update_t<double>(ctx, g, p, bc, qq, w, dt_step);
+  if (rc) return rc;
+  const Shape sh = make_shape(g);
+  {
+    Timed t_(ctx, MISOHIP_KIND_RT);
+    MISOHIP_TRY(launch_rt_heat(ctx->stream, static_cast<const double *>(qq->q[0]),
+                               static_cast<double *>(qq->q[7]), q, sh, dt_step));
+  }
+  rc = ghost_t<double>(ctx, qq, sh, make_bc(bc));
+  if (rc) return rc;
+  if (next_dt) return cfl_t<double>(ctx, g, p, qq, next_dt);
+  return MISOHIP_OK;
 }
 
 

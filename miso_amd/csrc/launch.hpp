@@ -91,6 +91,10 @@ template <typename R> struct CflFuse {
   const R *dx, *dy, *dz;
   double gm, cfl;
   unsigned *slot;
+  // optional radiative heating applied to each stored cell first (FP64
+  // only): ei = ei + heat_dt * heat_q / ro, the operator-split heating
+  const double *heat_q = nullptr;
+  double heat_dt = 0.0;
 };
 hipError_t launch_cfl_init(hipStream_t, unsigned *);
 

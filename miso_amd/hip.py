@@ -92,6 +92,9 @@ SIGNATURES = {
                         C.c_void_p, C.c_void_p, C.c_long],
     "misohip_table2d": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
                         C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long],
+    "misohip_update_heated": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
+                              _P(MisohipFields), _P(MisohipWork), C.c_double, C.c_void_p,
+                              _P(C.c_double)],
     "misohip_set_bc_callback": [C.c_void_p, BC_FN, C.c_void_p],
     "misohip_rt_quadrature": [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                               _P(C.c_double)],

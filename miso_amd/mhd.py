@@ -352,6 +352,20 @@ class Integrator:
                      self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
                      C.byref(qq.c_struct()), C.byref(self._work), float(dt))
 
+    def update_heated(self, dt: float, bc: BoundaryCondition, qq: Fields, qrad) -> float:
+        """update(dt), the radiative heating ei += dt * qrad / ro on the
+        interior and the ghost fill (FP64); returns the next step's dt, the
+        CFL of the heated state.  With x and y active the last AV sweep does
+        the heating and the CFL reduction as it stores each cell: the same
+        bits as update + RT.heat + apply_boundary_condition + cfl."""
+        nxt = C.c_double()
+        with self._user_bc(bc, qq):
+            hip.call("misohip_update_heated", self.ctx.handle, C.byref(self.grid.c_struct()),
+                     self.dtype_code, C.byref(self.params), C.byref(bc.c_struct()),
+                     C.byref(qq.c_struct()), C.byref(self._work), float(dt),
+                     C.c_void_p(qrad.data_ptr()), C.byref(nxt))
+        return nxt.value
+
     def run(self, bc: BoundaryCondition, qq: Fields, nsteps: int):
         """nsteps of ModelBase::update (cfl + update) on qq, the CFL of each
         later step reduced by the previous step's last sweep; returns the dts."""

@@ -196,6 +196,10 @@ class RadiativeMHD:
         self.q_rt = None                     # last solved Q_rad (learned mode)
         self.slab = None                     # bottom slab solved every step
         self.steps_done = 0
+        # heating and the next CFL inside the step's last AV sweep (same bits
+        # as the separate heating, ghost-fill and CFL passes)
+        self.fused_heating = True
+        self._next_dt = None
 
     # -- set-up -------------------------------------------------------------
     def initialize(self, seed: int) -> None:
@@ -208,6 +212,7 @@ class RadiativeMHD:
         self.reservoir.data.copy_(self.model.qq.data)
         self.bc = BoundaryCondition(periodic=(True, True, "fixed"), fixed=self.reservoir)
         self.model.integrator.apply_boundary_condition(self.bc, self.model.qq)
+        self._next_dt = None
 
     # -- radiative transfer -------------------------------------------------
     def solve_rt(self):
@@ -274,9 +279,13 @@ class RadiativeMHD:
     # -- stepping -----------------------------------------------------------
     def _advance(self, qrad) -> float:
         integ = self.model.integrator
-        dt = integ.step(self.bc, self.model.qq)
-        self.rt.heat(self.ctx, self.model.qq, dt, qrad)
-        integ.apply_boundary_condition(self.bc, self.model.qq)
+        if self.fused_heating:
+            dt = self._next_dt if self._next_dt is not None else integ.cfl(self.model.qq)
+            self._next_dt = integ.update_heated(dt, self.bc, self.model.qq, qrad)
+        else:
+            dt = integ.step(self.bc, self.model.qq)
+            self.rt.heat(self.ctx, self.model.qq, dt, qrad)
+            integ.apply_boundary_condition(self.bc, self.model.qq)
         self.steps_done += 1
         return dt
 

@@ -99,6 +99,26 @@ def test_rt_source_and_heat_match_numpy(ctx):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("sizes", [(8, 64), (12, 40)])
+def test_fused_heating_step_matches_the_separate_passes(ctx, sizes):
+    """misohip_update_heated (the heating and the next CFL in the last AV
+    sweep) against update + RT.heat + ghost fill + cfl on the plain-RT
+    stepping of the stratified box: the same dts and fields, bit for bit,
+    over six steps."""
+    import torch
+    n_xy, nz = sizes
+    states = []
+    for fused in (False, True):
+        box = _box(ctx, n_xy, nz, seed=9)
+        box.fused_heating = fused
+        dts = [box.step_plain() for _ in range(6)]
+        torch.cuda.synchronize()
+        states.append((dts, box.model.qq.data.clone()))
+    assert states[0][0] == states[1][0]
+    assert torch.equal(states[0][1], states[1][1])
+
+
+@pytest.mark.gpu
 def test_column_features_outputs_and_flags(ctx):
     import torch
     from miso_amd.surrogate import ColumnSurrogate, interior_columns
