@@ -394,6 +394,22 @@ template <typename Real> class Integrator {
                                   qq.desc(), &w_, double(dt)));
   }
 
+  /// update, the radiative heating ei += dt * qrad / ro (qrad: a device
+  /// field of the grid) and the ghost fill; returns the next step's dt (the
+  /// CFL of the heated state).  FP64; with x and y active the heating and
+  /// the CFL reduction run inside the last AV sweep.
+  template <typename EOS, typename BC, typename Source>
+  double update_heated(Real dt, const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq,
+                       const void *qrad) {
+    static_assert(std::is_same<Real, double>::value, "the radiative heating is FP64");
+    misohip_params p = params(eos, src);
+    double next = 0.0;
+    BoundaryScope scope(ctx_.handle(), bc);
+    MISO_HIP_CHECK(misohip_update_heated(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p,
+                                         &bc.d, qq.desc(), &w_, double(dt), qrad, &next));
+    return next;
+  }
+
   /// cfl + update in one call with one host synchronisation
   /// (ModelBase::update, mhd_model_base.hpp:57-62); returns dt.
   template <typename EOS, typename BC, typename Source>
