@@ -154,6 +154,11 @@ SodRun run_sod(int axis) {
 
 }  // namespace
 
+// the radiative step's C++ entry compiles for the types a MISO model uses
+template double hip::Integrator<double>::update_heated<hip::IdealEOS, hip::Boundary, hip::Gravity>(
+    double, const hip::IdealEOS &, const hip::Boundary &, const hip::Gravity &,
+    hip::Fields<double> &, const void *);
+
 extern "C" {
 
 // Returns 0 when every check passes; msg receives a one-line summary.
