@@ -429,6 +429,15 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
 
   for (int k = kb; k < ke; ++k) {
     const bool more = k + 1 < ke;  // uniform over the block
+    // plane k's orgn values, issued first so that their latency hides
+    // behind the x, y and z derivatives (not in the FP64 4-row tile, which
+    // would spill more registers)
+    constexpr bool EARLY = !SAME && (TY >= 8 || sizeof(R) < 8);
+    R og[9];
+    if (EARLY && valid) {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) og[m] = orgn.q[m][pc + k * plane];
+    }
     R S[17];
     // ---- x (LDS tile)
     dir_derivs<R, 0, 1>(TileAcc<R, TS>{lds}, tc, 1, ddx, gm1, S);
@@ -505,7 +514,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     if (valid) {
       R o[9];
 #pragma unroll
-      for (int m = 0; m < 9; ++m) o[m] = SAME ? ctr[m] : orgn.q[m][pc + k * plane];
+      for (int m = 0; m < 9; ++m) o[m] = SAME ? ctr[m] : EARLY ? og[m] : orgn.q[m][pc + k * plane];
       rk_update<R>(o, rslt, pc + k * plane, ctr[0], sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x,
                    t_vx, a_vy, d4y, d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph,
                    a_ei, t_ei);
