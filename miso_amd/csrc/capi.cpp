@@ -386,6 +386,12 @@ int misohip_destroy(misohip_ctx *ctx) {
   if (ctx->rt_kap_y) (void)hipFree(ctx->rt_kap_y);
   if (ctx->rt_src_y) (void)hipFree(ctx->rt_src_y);
   if (ctx->rt_parts) (void)hipFree(ctx->rt_parts);
+  if (ctx->rt_side) {
+    (void)hipStreamSynchronize(ctx->rt_side);
+    (void)hipStreamDestroy(ctx->rt_side);
+  }
+  if (ctx->rt_fork) (void)hipEventDestroy(ctx->rt_fork);
+  if (ctx->rt_join) (void)hipEventDestroy(ctx->rt_join);
   delete ctx;
   return MISOHIP_OK;
 }

@@ -25,6 +25,10 @@ struct misohip_ctx {
   double *rt_kap_y = nullptr, *rt_src_y = nullptr;
   size_t rt_cells = 0;
   unsigned *rt_parts = nullptr;  // the RT change-reduction slots
+  // a second stream (default priority) and two events: an RT solve sweeps
+  // its multi-plane rays there while the plane rays sweep on `stream`
+  hipStream_t rt_side = nullptr;
+  hipEvent_t rt_fork = nullptr, rt_join = nullptr;
   misohip_bc_fn bc_fn = nullptr;  // the user boundary callback and its argument
   void *bc_user = nullptr;
 };

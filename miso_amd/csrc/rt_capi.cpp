@@ -290,6 +290,16 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
   uf.It = It;
   uf.Jt = Jt;
   if (!ctx->rt_parts) MISOHIP_TRY(hipMalloc(&ctx->rt_parts, sizeof(unsigned) * 32 * RT_PARTS));
+  // The multi-plane rays and the plane rays touch disjoint intensities and
+  // meet only in the change maximum (atomic), so with both present the
+  // multi-plane sweeps run on a second stream beside the plane sweeps.
+  const bool split = nm > 0 && !phases.empty();
+  if (split && !ctx->rt_side) {
+    MISOHIP_TRY(hipStreamCreateWithFlags(&ctx->rt_side, hipStreamNonBlocking));
+    MISOHIP_TRY(hipEventCreateWithFlags(&ctx->rt_fork, hipEventDisableTiming));
+    MISOHIP_TRY(hipEventCreateWithFlags(&ctx->rt_join, hipEventDisableTiming));
+  }
+  hipStream_t mst = split ? ctx->rt_side : ctx->stream;
   unsigned *parts = ctx->rt_parts;
   MISOHIP_TRY(launch_rt_parts_reduce(ctx->stream, parts, nullptr, true));
   int it = 0;
@@ -330,13 +340,21 @@ int misohip_rt_solve(misohip_ctx *ctx, const misohip_grid *g, const misohip_rt_c
                                        su.nrays));
           }
         }
+      if (split) {
+        MISOHIP_TRY(hipEventRecord(ctx->rt_fork, ctx->stream));
+        MISOHIP_TRY(hipStreamWaitEvent(ctx->rt_side, ctx->rt_fork, 0));
+      }
+      for (int p0 = 0; p0 < m_planes; p0 += P)
+        MISOHIP_TRY(launch_rt_sweep_multi(mst, I, kap, src, kap_y, src_y, mrays, nm, p0, P,
+                                          su.ray_stride, m_nb, parts));
       for (const Phase &ph : phases)
         for (int p = ph.p0; p < ph.p1; ++p)
           MISOHIP_TRY(launch_rt_sweep_plane(ctx->stream, I, kap, src, kap_y, src_y, ph.rays,
                                             ph.n, p, su.ray_stride, ph.max_nodes, parts));
-      for (int p0 = 0; p0 < m_planes; p0 += P)
-        MISOHIP_TRY(launch_rt_sweep_multi(ctx->stream, I, kap, src, kap_y, src_y, mrays, nm, p0,
-                                          P, su.ray_stride, m_nb, parts));
+      if (split) {
+        MISOHIP_TRY(hipEventRecord(ctx->rt_join, ctx->rt_side));
+        MISOHIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->rt_join, 0));
+      }
       MISOHIP_TRY(launch_rt_faces(ctx->stream, I, old, false, uf, su.ray_stride, su.nrays, parts));
       MISOHIP_TRY(launch_rt_parts_reduce(ctx->stream, parts, ctx->d_slot + 1, false));
     }
