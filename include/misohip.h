@@ -131,7 +131,11 @@ typedef struct misohip_work {
 } misohip_work;
 
 /* Context: device ordinal and the HIP stream every call is ordered on
- * (NULL = the null stream).  Creates no stream of its own. */
+ * (NULL = the null stream).  Creates no stream at creation; the first
+ * misohip_rt_solve with both multi-plane and plane rays creates one more
+ * (default priority) for the multi-plane sweeps, forked from and joined back
+ * to the context's stream with events inside each iteration, and
+ * misohip_destroy frees it. */
 int misohip_create(int device, void *stream, misohip_ctx **out);
 int misohip_destroy(misohip_ctx *ctx);
 int misohip_set_stream(misohip_ctx *ctx, void *stream);
