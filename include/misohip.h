@@ -18,6 +18,8 @@
  *                                 (miso/include/miso/mhd_core.hpp:15-33)
  *   misohip_copy_to_device/host   Fields::copy_from H<->D
  *                                 (miso/include/miso/mhd_fields.hpp:83-107,148-172)
+ *   misohip_alloc / free / memcpy Array3D<T, backend::CUDA> storage and copies
+ *                                 (miso/include/miso/array3d.hpp:156-167,206-211,254-263)
  *   misohip_ghost_fill            Integrator::apply_boundary_condition =
  *                                 bc.apply + HaloExchanger::apply
  *                                 (miso/include/miso/mhd_integrator_host.hpp:250-255,
@@ -42,6 +44,8 @@
  */
 #ifndef MISOHIP_H
 #define MISOHIP_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -141,6 +145,18 @@ int misohip_destroy(misohip_ctx *ctx);
 int misohip_set_stream(misohip_ctx *ctx, void *stream);
 int misohip_synchronize(misohip_ctx *ctx);
 const char *misohip_strerror(int code);
+
+/* Raw device memory, for callers with no HIP binding of their own.
+ * misohip_alloc: `bytes` of uninitialised device memory on the context's
+ * device (NULL for 0 bytes).  misohip_free: waits for the context's stream,
+ * then releases memory from misohip_alloc (NULL is a no-op).
+ * misohip_memcpy: `bytes` copied as-is (no layout change), ordered on the
+ * context's stream; a host buffer must stay valid until the stream has
+ * reached the copy (misohip_synchronize). */
+enum { MISOHIP_H2D = 1, MISOHIP_D2H = 2, MISOHIP_D2D = 3 };
+int misohip_alloc(misohip_ctx *ctx, size_t bytes, void **out);
+int misohip_free(misohip_ctx *ctx, void *p);
+int misohip_memcpy(misohip_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 
 /* MISO-order (z-fastest) host array <-> x-fastest device field.  `host`
  * holds i_total*j_total*k_total elements; synchronous. */

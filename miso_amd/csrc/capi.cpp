@@ -415,6 +415,40 @@ int misohip_synchronize(misohip_ctx *ctx) {
   return MISOHIP_OK;
 }
 
+int misohip_alloc(misohip_ctx *ctx, size_t bytes, void **out) {
+  if (!ctx || !out) return MISOHIP_E_ARG;
+  *out = nullptr;
+  if (bytes == 0) return MISOHIP_OK;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  MISOHIP_TRY(hipMalloc(out, bytes));
+  return MISOHIP_OK;
+}
+
+int misohip_free(misohip_ctx *ctx, void *p) {
+  if (!ctx) return MISOHIP_E_ARG;
+  if (!p) return MISOHIP_OK;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  MISOHIP_TRY(hipStreamSynchronize(ctx->stream));
+  MISOHIP_TRY(hipFree(p));
+  return MISOHIP_OK;
+}
+
+int misohip_memcpy(misohip_ctx *ctx, void *dst, const void *src, size_t bytes, int kind) {
+  if (!ctx) return MISOHIP_E_ARG;
+  hipMemcpyKind k;
+  switch (kind) {
+    case MISOHIP_H2D: k = hipMemcpyHostToDevice; break;
+    case MISOHIP_D2H: k = hipMemcpyDeviceToHost; break;
+    case MISOHIP_D2D: k = hipMemcpyDeviceToDevice; break;
+    default: return MISOHIP_E_ARG;
+  }
+  if (bytes == 0) return MISOHIP_OK;
+  if (!dst || !src) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  MISOHIP_TRY(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+  return MISOHIP_OK;
+}
+
 int misohip_copy_to_device(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                            const void *host, void *dev) {
   if (!ctx || !host || !dev) return MISOHIP_E_ARG;

@@ -62,6 +62,10 @@ class MisohipWork(C.Structure):
 # int (*)(void *user, const misohip_fields *q, void *stream)
 BC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(MisohipFields), C.c_void_p)
 
+# misohip_memcpy directions
+MEMCPY_H2D, MEMCPY_D2H, MEMCPY_D2D = 1, 2, 3
+E_ARG = -2
+
 
 # (name, argtypes) of every exported entry point of include/misohip.h
 _P = C.POINTER
@@ -70,6 +74,9 @@ SIGNATURES = {
     "misohip_destroy": [C.c_void_p],
     "misohip_set_stream": [C.c_void_p, C.c_void_p],
     "misohip_synchronize": [C.c_void_p],
+    "misohip_alloc": [C.c_void_p, C.c_size_t, _P(C.c_void_p)],
+    "misohip_free": [C.c_void_p, C.c_void_p],
+    "misohip_memcpy": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int],
     "misohip_strerror": [C.c_int],
     "misohip_copy_to_device": [C.c_void_p, _P(MisohipGrid), C.c_int, C.c_void_p, C.c_void_p],
     "misohip_copy_to_host": [C.c_void_p, _P(MisohipGrid), C.c_int, C.c_void_p, C.c_void_p],
