@@ -25,6 +25,12 @@
 
 #include "mhd_oracle.h"
 
+/* OG_PLANT (default 0) selects a deliberately wrong variant for the tests
+ * that show the pins catch it (tests/test_oracle_pins.py). */
+#ifndef OG_PLANT
+#define OG_PLANT 0
+#endif
+
 static inline double max2n(double a, double b) { return (double)fmaxf((float)a, (float)b); }
 static inline double min3n(double a, double b, double c) {
   return (double)fminf(fminf((float)a, (float)b), (float)c);
@@ -128,6 +134,7 @@ static void rt_set_face(const rtg *r, int nr, const double *mu, int axis, int si
 /* rt_host.hpp:680-689 */
 static inline double bilinear(double cx, double cy, double u00, double u10, double u01,
                               double u11) {
+  if (OG_PLANT == 15) { const double t = u10; u10 = u01; u01 = t; }
   return (1.0 - cx) * (1.0 - cy) * u00 + cx * (1.0 - cy) * u10 + (1.0 - cx) * cy * u01 +
          cx * cy * u11;
 }
@@ -142,6 +149,7 @@ static inline double integrate_intensity(double Iu, double Su, double Sd, double
     return Iu * exp_dtau + (u0 - u1) * Su + u1 * Sd;
   }
   const double u1 = 1.0 - (u0 / dtau);
+  if (OG_PLANT == 14) return Iu * exp_dtau + u1 * Su + (u0 - u1) * Sd;
   return Iu * exp_dtau + (u0 - u1) * Su + u1 * Sd;
 }
 
@@ -278,6 +286,6 @@ void og_rt_qrad(const og_grid *g, int num_rays, const double *kap, const double 
         double J = 0.0;
         for (int q = 0; q < nr; ++q) J += w[q] * I[NODE(r, q, i, j, k)];
         const size_t c = CELL(r, i, j, k);
-        qrad[c] = 4.0 * M_PI * kap[c] * (J - S[c]);
+        qrad[c] = (OG_PLANT == 16 ? 2.0 : 4.0) * M_PI * kap[c] * (J - S[c]);
       }
 }

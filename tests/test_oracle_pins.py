@@ -595,6 +595,120 @@ def test_grid_matches_grid_hpp():
     assert g.min_dxyz == float(np.float32(min(1 / 8, 1e30, 2 / 5)))
 
 
+# ------------------------------------- AV characteristic speed (A7)
+CC_FACS = (1.3, 0.7, 2.1)  # cs_fac, vv_fac, ca_fac: distinct and non-zero
+
+
+def char_vel_err(mod=O):
+    """Largest relative miss of the oracle's frozen AV speed against the
+    textbook speeds it combines (mhd_artificial_viscosity_host.hpp:57-64):
+    the adiabatic sound speed sqrt(gamma p / rho) with p = (gamma-1) rho e,
+    the flow speed |v| and the Gaussian-unit Alfven speed |B| / sqrt(4 pi rho).
+    Each factor is switched on alone first, so a term paired with the wrong
+    factor, a dropped term or a wrong constant all show."""
+    from tests.parity_util import random_state
+    g = mod.make_grid((6, 5, 4), 2)
+    q = random_state(g.shape, seed=11, amp=0.8)
+    gm = 5.0 / 3.0
+    ro, v, b, e = q[0], q[1:4], q[4:7], q[7]
+    pres = (gm - 1.0) * ro * e
+    speeds = (np.sqrt(gm * pres / ro), np.sqrt((v * v).sum(0)),
+              np.sqrt((b * b).sum(0)) / np.sqrt(4.0 * math.pi * ro))
+    worst = 0.0
+    cases = [tuple(f if n == m else 0.0 for n, f in enumerate(CC_FACS)) for m in range(3)]
+    cases.append(CC_FACS)
+    for cs_f, vv_f, ca_f in cases:
+        p = mod.Params(gm=gm, cs_fac=cs_f, vv_fac=vv_f, ca_fac=ca_f)
+        got = mod.char_vel(g, p, q)
+        want = cs_f * speeds[0] + vv_f * speeds[1] + ca_f * speeds[2]
+        worst = max(worst, float(np.max(np.abs(got - want) / want)))
+    return worst
+
+
+def test_char_vel_is_the_sum_of_textbook_speeds():
+    assert char_vel_err() < 1e-14
+
+
+# ------------------------------------- radiative transfer closed forms (A15)
+def rt_linear_source_err(mod=O, kappa=3.0, a=0.8, b=1.7, i0=0.25, n=32):
+    """One +z ray (w = 1) through a uniform absorber whose source function is
+    linear in optical depth, S = a + b tau, entering with I0 at the lower
+    face.  The transfer equation dI/dtau = S - I then has the exact solution
+    I = I0 e^-tau + a (1 - e^-tau) + b (tau - 1 + e^-tau), and the linear-
+    source short-characteristics integral (rt_host.hpp:667-678) is exact for
+    S linear in tau, so the solver must reproduce it to rounding.  Q_rad is
+    minus the divergence of the radiative flux F = 4 pi I z (the one ray
+    stands for the full solid angle, 4 pi): Q = -4 pi kappa dI/dtau, with the
+    derivative of the exact solution.  Returns (I error, Q error), relative
+    to the largest value."""
+    g = mod.make_grid((1, 1, n), 2, bounds=((-0.5, 0.5),) * 3)
+    dz = g.axis("dz")[0]
+    kap = np.full(g.shape, kappa)
+    k = np.arange(g.shape[2]) - 1.0          # the lattice's lower face is cell 1
+    tau_cell = kappa * dz * k
+    S = np.broadcast_to(a + b * tau_cell, g.shape).copy()
+    I = np.zeros((1,) + g.shape)
+    mod.rt_solve(g, kap, S, I, periodic=(0, 0, 0), face_kind=(0, 0, 0, 0, 3, 0),
+                 face_value=(0, 0, 0, 0, i0, 0), tolerance=1e-15, max_iters=3, num_rays=1)
+    tau = tau_cell[1:n + 2]
+    ex = np.exp(-tau)
+    want_i = i0 * ex + a * (1.0 - ex) + b * (tau - 1.0 + ex)
+    got_i = I[0, 0, 0, 1:n + 2]
+    err_i = float(np.max(np.abs(got_i - want_i)) / np.max(np.abs(want_i)))
+    q = mod.rt_qrad(g, kap, S, I, num_rays=1)[0, 0, 1:n + 2]
+    dI = -i0 * ex + a * ex + b * (1.0 - ex)
+    want_q = -4.0 * math.pi * kappa * dI
+    err_q = float(np.max(np.abs(q - want_q)) / np.max(np.abs(want_q)))
+    return err_i, err_q
+
+
+def rt_linear_field_err(mod=O):
+    """All 24 Carlson A4 rays through a uniform absorber many optical depths
+    thick whose source function is linear in position, S = S0 + g.x, with
+    thermal inflow (I = S) on every face.  The transfer equation
+    mu.grad I = kappa (S - I) has the particular solution
+    I_r = S - (mu_r . g) / kappa, and the inflow's departure from it decays as
+    exp(-tau) along the ray; bilinear interpolation on the upwind face is
+    exact for linear fields and the segment integral is exact for S linear in
+    tau, so nodes more than ~40 optical depths from every face must hold the
+    particular solution.  The solver takes the direction cosines through
+    util::max2, which rounds them to float (rt_host.hpp:462-466), so the
+    segment it integrates over points along mu rounded to float and the
+    offset (mu_r . g) / kappa is reproduced to float precision: the miss is
+    measured relative to that offset.  The cells are anisotropic
+    (dx != dy != dz), so the two transverse interpolation weights of every
+    ray differ and a transposed bilinear corner shows.  Returns the largest
+    miss over the offset."""
+    sizes = (48, 44, 40)
+    g = mod.make_grid(sizes, 2, bounds=((0.0, 1.2), (0.0, 1.32), (0.0, 1.4)))
+    kappa = 100.0                          # at least 2.5 optical depths per segment
+    grad = np.array([0.3, -0.2, 0.45])
+    x, y, z = g.axis("x"), g.axis("y"), g.axis("z")
+    S = 5.0 + grad[0] * x[:, None, None] + grad[1] * y[None, :, None] + grad[2] * z[None, None, :]
+    kap = np.full(g.shape, kappa)
+    I = np.zeros((24,) + g.shape)
+    mod.rt_solve(g, kap, S, I, periodic=(0, 0, 0), face_kind=(1,) * 6, tolerance=1e-15,
+                 max_iters=3)
+    w, mx, my, mz = mod.rt_quadrature(24)
+    # lattice nodes 1 .. n+1 (cells); keep those >= 17 cells from every face
+    sl = tuple(slice(1 + 17, 1 + s - 17) for s in sizes)
+    worst = 0.0
+    for r in range(24):
+        off = (mx[r] * grad[0] + my[r] * grad[1] + mz[r] * grad[2]) / kappa
+        worst = max(worst, float(np.max(np.abs(I[r][sl] - (S[sl] - off))) / abs(off)))
+    return worst
+
+
+def test_rt_linear_source_in_tau_is_exact():
+    err_i, err_q = rt_linear_source_err()
+    assert err_i < 1e-13, err_i
+    assert err_q < 1e-11, err_q
+
+
+def test_rt_linear_field_gives_the_particular_solution():
+    assert rt_linear_field_err() < 1e-6
+
+
 # --------------------------------------------------- planted mistakes
 PLANTS = {
     1: "pressure-gradient sign in the x momentum equation",
@@ -609,6 +723,10 @@ PLANTS = {
     10: "magnetic energy dropped from the AV total energy",
     11: "Alfven speed dropped from the CFL speed",
     12: "min/max without float narrowing",
+    13: "Alfven speed dropped from the frozen AV speed",
+    14: "RT upwind and downwind source weights swapped",
+    15: "RT bilinear corners u10 and u01 transposed",
+    16: "Q_rad prefactor 2 pi instead of 4 pi",
 }
 
 
@@ -647,6 +765,15 @@ def _pins_fail(mod):
     q = random_state(g.shape, seed=5)
     if mod.cfl(g, mod.Params(cfl_number=0.4), q) != cfl_brute(g, q, 5.0 / 3.0, 0.4):
         fails.append("cfl")
+    if char_vel_err(mod) >= 1e-14:
+        fails.append("char_vel")
+    err_i, err_q = rt_linear_source_err(mod)
+    if err_i >= 1e-13:
+        fails.append("rt_linear_source")
+    if err_q >= 1e-11:
+        fails.append("rt_qrad")
+    if rt_linear_field_err(mod) >= 1e-6:
+        fails.append("rt_linear_field")
     return fails
 
 
