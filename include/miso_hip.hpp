@@ -325,17 +325,20 @@ struct Boundary {
 /// Registers a boundary's user functor on the context for one call.
 class BoundaryScope {
  public:
-  BoundaryScope(misohip_ctx *ctx, const Boundary &bc) : ctx_(bc.fn ? ctx : nullptr) {
-    if (ctx_) MISO_HIP_CHECK(misohip_set_bc_callback(ctx_, &Boundary::call, bc.fn.get()));
+  BoundaryScope(misohip_ctx *ctx, const Boundary &bc) : ctx_(ctx) {
+    MISO_HIP_CHECK(misohip_get_bc_callback(ctx_, &prev_fn_, &prev_user_));
+    // exactly this call's boundary: the functor, or none when bc has none
+    MISO_HIP_CHECK(misohip_set_bc_callback(ctx_, bc.fn ? &Boundary::call : nullptr,
+                                           bc.fn ? bc.fn.get() : nullptr));
   }
   BoundaryScope(const BoundaryScope &) = delete;
   BoundaryScope &operator=(const BoundaryScope &) = delete;
-  ~BoundaryScope() {
-    if (ctx_) misohip_set_bc_callback(ctx_, nullptr, nullptr);
-  }
+  ~BoundaryScope() { misohip_set_bc_callback(ctx_, prev_fn_, prev_user_); }
 
  private:
   misohip_ctx *ctx_;
+  misohip_bc_fn prev_fn_ = nullptr;
+  void *prev_user_ = nullptr;
 };
 
 /// Source descriptors (the user Source functor's two forms here).

@@ -108,15 +108,44 @@ typedef struct misohip_bc {
  * stream) first; fn enqueues its device work on `stream` (the context's)
  * and returns 0, or nonzero to stop (the call then returns
  * MISOHIP_E_CALLBACK).  fn = NULL removes it.  A misohip_bc with an axis of
- * kind 3 needs one.  With a callback set, the x and y AV sweeps run as two
- * passes with a fill between them, as in the reference.  The reference has
- * no such entry point: there BC is a template argument. */
+ * kind 3 needs one.  With a callback set, every sweep runs as its own pass
+ * followed by a fill, and the CFL is a pass of its own over the state the
+ * last fill leaves, in the reference's order (mhd_integrator_host.hpp:
+ * 280-300, mhd_model_base.hpp:57-62): no x-y walk, no z sweep folded into
+ * the 2D y sweep, no CFL or radiative heating folded into the last sweep, so
+ * a callback may write interior cells (floors, blending near an obstacle).
+ * The reference has no such entry point: there BC is a template argument. */
 typedef int (*misohip_bc_fn)(void *user, const struct misohip_fields *q, void *stream);
 int misohip_set_bc_callback(misohip_ctx *ctx, misohip_bc_fn fn, void *user);
+/* The context's current callback and its user pointer (NULL, NULL when none
+ * is set), so that a caller installing its own for one call can put the
+ * previous one back.  No counterpart in the reference (see above). */
+int misohip_get_bc_callback(misohip_ctx *ctx, misohip_bc_fn *fn, void **user);
+
+/* A user source term (Source::vx, vy, vz, ei; mhd_model_base.hpp:131-155,
+ * e.g. demo/mhd3d_magnetosphere/src/src.hpp:4-41).  With
+ * misohip_params.src_kind = 2 every RK stage -- misohip_rk_stage and the
+ * stages inside misohip_update, misohip_step, misohip_run and
+ * misohip_update_heated -- first calls fn(user, argm, arrays, stream): fn
+ * enqueues on `stream` (the context's) the device work that writes
+ * src.vx, src.vy, src.vz and src.ei of that stage's argm state into
+ * arrays[0..3] (device fields of the grid's shape and layout) at every
+ * interior cell, and returns 0, or nonzero to stop (the call then returns
+ * MISOHIP_E_CALLBACK).  The stage kernel adds arrays[0..2] to the momentum
+ * equations inside dt and arrays[3] to the internal energy outside it, where
+ * the reference evaluates the functor (mhd_integrator_host.hpp:134, 151,
+ * 168, 241).  The arrays belong to the caller and must stay valid while the
+ * callback is set; fn = NULL removes it.  The reference has no such entry
+ * point: there Source is a template argument inlined into the stage. */
+typedef int (*misohip_src_fn)(void *user, const struct misohip_fields *argm,
+                              void *const arrays[4], void *stream);
+int misohip_set_source_callback(misohip_ctx *ctx, misohip_src_fn fn, void *user,
+                                void *const arrays[4]);
 
 /* Config values the step consumes (config_defaults.hpp, eos.hpp) and the
  * source term: src_kind 0 = none (EmptySourceTerm), 1 = constant gravity
- * -grav*ro along grav_axis (0/1/2). */
+ * -grav*ro along grav_axis (0/1/2), 2 = the context's source callback
+ * (misohip_set_source_callback). */
 typedef struct misohip_params {
   double gm;
   double cfl_number;

@@ -94,6 +94,19 @@ bool bc_invalid(const misohip_ctx *ctx, const misohip_bc *bc) {
   return false;
 }
 
+// a source description the context can carry out: gravity on a real axis,
+// or the user callback with its four arrays
+bool src_invalid(const misohip_ctx *ctx, const misohip_params *p) {
+  if (p->src_kind < 0 || p->src_kind > 2) return true;
+  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return true;
+  if (p->src_kind == 2) {
+    if (!ctx->src_fn) return true;
+    for (int d = 0; d < 4; ++d)
+      if (!ctx->src_arr[d]) return true;
+  }
+  return false;
+}
+
 BcDev make_bc(const misohip_bc *bc) {
   BcDev b;
   for (int a = 0; a < 3; ++a) {
@@ -109,8 +122,14 @@ template <typename R>
 int rk_stage_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
                const misohip_fields *o, const misohip_fields *a, const misohip_fields *r,
                R dt_s, R ch2, R damp) {
+  // a user source first writes src.vx, src.vy, src.vz, src.ei of this
+  // stage's argm (the reference evaluates the functor on argm inside the
+  // stage, mhd_integrator_host.hpp:134,151,168,241)
+  if (p->src_kind == 2 && ctx->src_fn(ctx->src_user, a, ctx->src_arr, ctx->stream) != 0)
+    return MISOHIP_E_CALLBACK;
   Timed t_(ctx, MISOHIP_KIND_RK_STAGE);
   RkScalars sc;
+  for (int d = 0; d < 4; ++d) sc.src_arr[d] = p->src_kind == 2 ? ctx->src_arr[d] : nullptr;
   sc.dt = dt_s; sc.ch2 = ch2; sc.damp = damp; sc.gm = (R)p->gm;
   sc.grav = (R)p->grav; sc.src_kind = p->src_kind; sc.grav_axis = p->grav_axis;
   sc.pencil_bx = ctx->tile.rk_pencil_bx;
@@ -207,9 +226,11 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   // each of its results before they are stored.
   // In 3D runs the x and y sweeps can run as one walk (tile choice
   // av_fuse_xy): the y walk x-sweeps every row it takes in.
-  const bool fuse_z = g->is && g->js && !g->ks;
-  // (not with a user boundary: the y sweep then needs the callback's ghost
-  // rows of the x-swept state)
+  // Neither fusion runs with a user boundary: the reference applies the
+  // boundary after every sweep (mhd_integrator_host.hpp:286-299), so the y
+  // sweep reads the callback's ghost rows of the x-swept state and the z
+  // sweep the callback's output of the y sweep.
+  const bool fuse_z = g->is && g->js && !g->ks && !ctx->bc_fn;
   const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2 &&
                        !ctx->bc_fn;
   const misohip_fields *cur = argm;
@@ -264,11 +285,14 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
 }
 
 // ModelBase::run's loop body nsteps times; from the second step on, dt comes
-// from the CFL reduction the previous step's last AV sweep carried out.
+// from the CFL reduction the previous step's last AV sweep carried out --
+// except with a user boundary, whose callback runs after that sweep and may
+// write interior cells, so the CFL pass then reads the state it leaves, as
+// ModelBase::update does (mhd_model_base.hpp:57-62).
 template <typename R>
 int run_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p, const misohip_bc *bc,
           const misohip_fields *qq, const misohip_work *w, int nsteps, double *dts) {
-  const bool can_fuse = g->is && g->js;  // 3D, or 2D in x-y
+  const bool can_fuse = g->is && g->js && !ctx->bc_fn;  // 3D, or 2D in x-y
   double d = 0.0;
   for (int s = 0; s < nsteps; ++s) {
     if (s == 0 || !can_fuse) {
@@ -403,6 +427,27 @@ int misohip_set_bc_callback(misohip_ctx *ctx, misohip_bc_fn fn, void *user) {
   return MISOHIP_OK;
 }
 
+int misohip_set_source_callback(misohip_ctx *ctx, misohip_src_fn fn, void *user,
+                                void *const arrays[4]) {
+  if (!ctx) return MISOHIP_E_ARG;
+  if (fn) {
+    if (!arrays) return MISOHIP_E_ARG;
+    for (int d = 0; d < 4; ++d)
+      if (!arrays[d]) return MISOHIP_E_ARG;
+  }
+  ctx->src_fn = fn;
+  ctx->src_user = fn ? user : nullptr;
+  for (int d = 0; d < 4; ++d) ctx->src_arr[d] = fn ? arrays[d] : nullptr;
+  return MISOHIP_OK;
+}
+
+int misohip_get_bc_callback(misohip_ctx *ctx, misohip_bc_fn *fn, void **user) {
+  if (!ctx || !fn || !user) return MISOHIP_E_ARG;
+  *fn = ctx->bc_fn;
+  *user = ctx->bc_user;
+  return MISOHIP_OK;
+}
+
 int misohip_set_stream(misohip_ctx *ctx, void *stream) {
   if (!ctx) return MISOHIP_E_ARG;
   ctx->stream = static_cast<hipStream_t>(stream);
@@ -515,7 +560,7 @@ int misohip_rk_stage(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                      double ch2, double damp) {
   MISOHIP_COMMON_CHECKS(ctx, g, dt)
   if (!p || check_fields(orgn) || check_fields(argm) || check_fields(rslt)) return MISOHIP_E_ARG;
-  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (src_invalid(ctx, p)) return MISOHIP_E_ARG;
   if (dt == MISOHIP_F64) return rk_stage_t<double>(ctx, g, p, orgn, argm, rslt, dt_s, ch2, damp);
   return rk_stage_t<float>(ctx, g, p, orgn, argm, rslt, (float)dt_s, (float)ch2, (float)damp);
 }
@@ -560,7 +605,7 @@ int misohip_update(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
       check_fields(&w->rslt) ||
       !w->cc)
     return MISOHIP_E_ARG;
-  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (src_invalid(ctx, p)) return MISOHIP_E_ARG;
   if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
   if (dt == MISOHIP_F64) return update_t<double>(ctx, g, p, bc, qq, w, dt_step);
   return update_t<float>(ctx, g, p, bc, qq, w, (float)dt_step);
@@ -574,10 +619,13 @@ int misohip_update_heated(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype
   if (!p || bc_invalid(ctx, bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
       check_fields(&w->rslt) || !w->cc || !qrad || dt != MISOHIP_F64)
     return MISOHIP_E_ARG;
-  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (src_invalid(ctx, p)) return MISOHIP_E_ARG;
   if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
   const double *q = static_cast<const double *>(qrad);
-  if (g->is && g->js) {  // the last AV sweep heats each cell it stores and reduces the CFL
+  // the last AV sweep heats each cell it stores and reduces the CFL; not with
+  // a user boundary, whose callback must see the unheated state after the
+  // update and the heated one before the CFL
+  if (g->is && g->js && !ctx->bc_fn) {
     int rc = update_t<double>(ctx, g, p, bc, qq, w, dt_step, true, q);
     if (rc) return rc;
     if (next_dt) {
@@ -613,7 +661,7 @@ int misohip_run(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
   if (!p || bc_invalid(ctx, bc) || !w || check_fields(qq) || check_fields(&w->argm) ||
       check_fields(&w->rslt) || !w->cc || nsteps < 0)
     return MISOHIP_E_ARG;
-  if (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)) return MISOHIP_E_ARG;
+  if (src_invalid(ctx, p)) return MISOHIP_E_ARG;
   if (dt == MISOHIP_F64) return run_t<double>(ctx, g, p, bc, qq, w, nsteps, dts);
   return run_t<float>(ctx, g, p, bc, qq, w, nsteps, dts);
 }

@@ -61,6 +61,8 @@ class MisohipWork(C.Structure):
 
 # int (*)(void *user, const misohip_fields *q, void *stream)
 BC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(MisohipFields), C.c_void_p)
+SRC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(MisohipFields), C.POINTER(C.c_void_p),
+                     C.c_void_p)
 
 # misohip_memcpy directions
 MEMCPY_H2D, MEMCPY_D2H, MEMCPY_D2D = 1, 2, 3
@@ -103,6 +105,8 @@ SIGNATURES = {
                               _P(MisohipFields), _P(MisohipWork), C.c_double, C.c_void_p,
                               _P(C.c_double)],
     "misohip_set_bc_callback": [C.c_void_p, BC_FN, C.c_void_p],
+    "misohip_get_bc_callback": [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)],
+    "misohip_set_source_callback": [C.c_void_p, SRC_FN, C.c_void_p, _P(C.c_void_p)],
     "misohip_rt_quadrature": [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                               _P(C.c_double)],
     "misohip_rt_solve": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,

@@ -11,6 +11,9 @@
 //                              exact Riemann solution, x/y/z agreement 1e-5
 //   cpp_run_steps              N steps of a given state through the C++
 //                              classes, for bitwise comparison with the oracle
+//   cpp_boundary_scope         a C++ call runs exactly its own Boundary's
+//                              functor (none when it has none) and leaves a
+//                              callback set through the C ABI in place
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -243,6 +246,68 @@ int cpp_run_steps(int fp64, int ni, int nj, int nk, const double *bounds, const 
   } catch (const std::exception &e) {
     std::fprintf(stderr, "cpp_run_steps: %s\n", e.what());
     return 2;
+  }
+}
+
+namespace {
+int count_calls(void *user, const misohip_fields *, void *) {
+  ++*static_cast<int *>(user);
+  return 0;
+}
+}  // namespace
+
+// A callback installed through the C ABI; C++ fills and steps with a plain
+// Boundary must not run it, with a user Boundary must run the functor
+// instead, and both must leave it installed.  Returns 0 when all hold; msg
+// names the first check that failed.
+int cpp_boundary_scope(char *msg, int len) {
+  auto fail = [&](const char *what) {
+    std::snprintf(msg, len, "%s", what);
+    return 1;
+  };
+  try {
+    hip::HostGrid<double> hg(8, 8, 8, 2, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0);
+    hip::ExecContext<double> ctx(0);
+    hip::MHDConfig cfg;
+    hip::MHD<double> mhd(cfg, hg, ctx);
+    hip::IdealEOS eos{5.0 / 3.0};
+    auto ic = [](hip::HostFields<double> &q, const hip::HostGrid<double> &) {
+      for (std::size_t n = 0; n < q.q[0].size(); ++n) {
+        q.q[0][n] = 1.0 + 0.01 * double(n % 7);
+        q.q[7][n] = 1.5;
+      }
+    };
+    int c_calls = 0;
+    if (misohip_set_bc_callback(ctx.handle(), &count_calls, &c_calls)) return fail("set");
+    hip::Boundary plain{hip::Boundary::PERIODIC, hip::Boundary::PERIODIC,
+                        hip::Boundary::PERIODIC};
+    mhd.apply_initial_condition(ic, plain);
+    const double dt = mhd.cfl(eos);
+    mhd.update(dt, eos, plain, hip::NoSource{});
+    ctx.synchronize();
+    if (c_calls != 0) return fail("a plain Boundary ran the C-level callback");
+    int user_calls = 0;
+    hip::Boundary user = plain;
+    user.user([&](const misohip_fields &, hipStream_t) { ++user_calls; });
+    mhd.update(dt, eos, user, hip::NoSource{});
+    ctx.synchronize();
+    if (c_calls != 0) return fail("a user Boundary ran the C-level callback");
+    if (user_calls != 7) return fail("the user functor did not run before each of 7 fills");
+    misohip_bc_fn fn = nullptr;
+    void *u = nullptr;
+    if (misohip_get_bc_callback(ctx.handle(), &fn, &u) || fn != &count_calls || u != &c_calls)
+      return fail("the C-level callback was not restored");
+    hip::Fields<double> &qq = mhd.qq;
+    misohip_bc d = plain.d;
+    if (misohip_ghost_fill(ctx.handle(), mhd.grid.desc(), MISOHIP_F64, qq.desc(), &d))
+      return fail("ghost fill");
+    ctx.synchronize();
+    if (c_calls != 1) return fail("the restored callback did not run on a C-level fill");
+    misohip_set_bc_callback(ctx.handle(), nullptr, nullptr);
+    std::snprintf(msg, len, "ok");
+    return 0;
+  } catch (const std::exception &e) {
+    return fail(e.what());
   }
 }
 

@@ -31,13 +31,24 @@ def _lib():
     L.cpp_rt_solve.restype = C.c_int
     L.cpp_rt_save_load.argtypes = [C.c_int, C.c_int, C.c_int, D, D, C.c_char_p]
     L.cpp_rt_save_load.restype = C.c_int
+    L.cpp_boundary_scope.argtypes = [C.c_char_p, C.c_int]
+    L.cpp_boundary_scope.restype = C.c_int
     return L
 
 
 def test_cpp_test_library_exports():
     L = _lib()
     assert L.cpp_test_hd1d_shock_tube and L.cpp_run_steps and L.cpp_rt_solve
-    assert L.cpp_rt_save_load
+    assert L.cpp_rt_save_load and L.cpp_boundary_scope
+
+
+@pytest.mark.gpu
+def test_cpp_boundary_scope_restores_the_c_callback():
+    """A C++ call runs its own Boundary's functor, or none, and puts back a
+    callback that was set through the C ABI before it."""
+    buf = C.create_string_buffer(256)
+    rc = _lib().cpp_boundary_scope(buf, 256)
+    assert rc == 0, buf.value.decode()
 
 
 @pytest.mark.gpu
