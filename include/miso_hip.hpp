@@ -20,9 +20,13 @@
 //   miso::hip::RT                    rt::RT<double> (miso/include/miso/rt_host.hpp:104-689),
 //                                    CPU-only in the reference
 //
-// User boundary and source functors cannot cross a C ABI; they are given as
-// descriptors: Boundary (per axis periodic / mirror with per-field signs /
-// fixed ghosts) and Gravity / NoSource.  Errors follow the reference's
+// User boundary and source functors cannot cross a C ABI as template
+// arguments; they are descriptors -- Boundary (per axis periodic / mirror
+// with per-field signs / fixed ghosts) and Gravity / NoSource -- or device
+// functors run through the context's callbacks: Boundary::user (a functor
+// run before every ghost fill) and UserSource (the four source terms of each
+// RK stage's argm written into arrays the stage adds; make_source in
+// miso_hip_device.hpp builds one from a MISO Source struct).  Errors follow the reference's
 // convention (miso/include/miso/cuda_util.cuh:17-33): a failed backend call
 // prints file, line and message and aborts; invalid configuration throws
 // std::runtime_error (miso/include/miso/grid.hpp:244-276).
@@ -352,6 +356,62 @@ struct Gravity {
   void fill(misohip_params &p) const { p.src_kind = 1; p.grav_axis = axis; p.grav = g; }
 };
 
+/// A user source term (Source::vx, vy, vz, ei; mhd_model_base.hpp:131-155):
+/// a functor that writes the four terms of a stage's argm state into four
+/// device arrays of the grid's shape, run before every RK stage through the
+/// context's source callback (misohip_set_source_callback); the stage adds
+/// them where the reference inlines the functor.  make_source
+/// (miso_hip_device.hpp) builds one from a MISO-style Source struct.
+template <typename Real> struct UserSource {
+  using Fn = std::function<void(const misohip_fields &argm, Real *const out[4], hipStream_t)>;
+  std::shared_ptr<Fn> fn;
+  std::shared_ptr<std::array<DeviceArray<Real>, 4>> arrays;
+  UserSource(const Grid<Real> &grid, Fn f)
+      : fn(std::make_shared<Fn>(std::move(f))),
+        arrays(std::make_shared<std::array<DeviceArray<Real>, 4>>()) {
+    for (auto &a : *arrays) a = DeviceArray<Real>(grid.host().n_total());
+  }
+  void fill(misohip_params &p) const { p.src_kind = 2; p.grav_axis = 2; p.grav = 0.0; }
+  /// C callback over fn; exceptions stop the step
+  static int call(void *user, const misohip_fields *argm, void *const out[4], void *stream) {
+    try {
+      const UserSource *self = static_cast<const UserSource *>(user);
+      Real *o[4] = {static_cast<Real *>(out[0]), static_cast<Real *>(out[1]),
+                    static_cast<Real *>(out[2]), static_cast<Real *>(out[3])};
+      (*self->fn)(*argm, o, static_cast<hipStream_t>(stream));
+      return 0;
+    } catch (...) {
+      return 1;
+    }
+  }
+};
+
+/// Registers a user source on the context for one call and puts back the
+/// callback that was there; the descriptor sources need nothing.
+class SourceScope {
+ public:
+  SourceScope(misohip_ctx *, const NoSource &) {}
+  SourceScope(misohip_ctx *, const Gravity &) {}
+  template <typename Real> SourceScope(misohip_ctx *ctx, const UserSource<Real> &src) : ctx_(ctx) {
+    MISO_HIP_CHECK(misohip_get_source_callback(ctx_, &prev_fn_, &prev_user_, prev_arr_));
+    void *arr[4];
+    for (int d = 0; d < 4; ++d) arr[d] = (*src.arrays)[d].data();
+    MISO_HIP_CHECK(misohip_set_source_callback(ctx_, &UserSource<Real>::call,
+                                               const_cast<UserSource<Real> *>(&src), arr));
+  }
+  SourceScope(const SourceScope &) = delete;
+  SourceScope &operator=(const SourceScope &) = delete;
+  ~SourceScope() {
+    if (ctx_) misohip_set_source_callback(ctx_, prev_fn_, prev_user_, prev_arr_);
+  }
+
+ private:
+  misohip_ctx *ctx_ = nullptr;
+  misohip_src_fn prev_fn_ = nullptr;
+  void *prev_user_ = nullptr;
+  void *prev_arr_[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
 /// The ideal-gas EOS the step uses (eos::IdealEOS, eos.hpp:11-43).
 struct IdealEOS {
   double gm;
@@ -393,6 +453,7 @@ template <typename Real> class Integrator {
   void update(Real dt, const EOS &eos, const BC &bc, const Source &src, Fields<Real> &qq) {
     misohip_params p = params(eos, src);
     BoundaryScope scope(ctx_.handle(), bc);
+    SourceScope sscope(ctx_.handle(), src);
     MISO_HIP_CHECK(misohip_update(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
                                   qq.desc(), &w_, double(dt)));
   }
@@ -408,6 +469,7 @@ template <typename Real> class Integrator {
     misohip_params p = params(eos, src);
     double next = 0.0;
     BoundaryScope scope(ctx_.handle(), bc);
+    SourceScope sscope(ctx_.handle(), src);
     MISO_HIP_CHECK(misohip_update_heated(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p,
                                          &bc.d, qq.desc(), &w_, double(dt), qrad, &next));
     return next;
@@ -420,6 +482,7 @@ template <typename Real> class Integrator {
     misohip_params p = params(eos, src);
     double dt = 0.0;
     BoundaryScope scope(ctx_.handle(), bc);
+    SourceScope sscope(ctx_.handle(), src);
     MISO_HIP_CHECK(misohip_step(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
                                 qq.desc(), &w_, &dt));
     return Real(dt);
@@ -433,6 +496,7 @@ template <typename Real> class Integrator {
     misohip_params p = params(eos, src);
     std::vector<double> dts(nsteps > 0 ? nsteps : 0);
     BoundaryScope scope(ctx_.handle(), bc);
+    SourceScope sscope(ctx_.handle(), src);
     MISO_HIP_CHECK(misohip_run(ctx_.handle(), grid_.desc(), dtype_of<Real>(), &p, &bc.d,
                                qq.desc(), &w_, nsteps, dts.data()));
     return dts;

@@ -2,7 +2,8 @@
 // backend: the backend::HIP forms of MISO's for_each (execution.hpp:56-100)
 // and bc::symmetric (boundary_condition.hpp:54-182), with views of the
 // device fields and grid, for boundary functors of kind USER
-// (miso::hip::Boundary::user in miso_hip.hpp).  A magnetosphere-style
+// (miso::hip::Boundary::user in miso_hip.hpp), and make_source, which turns a
+// MISO-style Source struct into a UserSource.  A magnetosphere-style
 // boundary (demo/mhd3d_magnetosphere/src/bc.hpp:39-158) is a functor that
 // launches these on the stream it is given.  HIP-only: compile the user's
 // translation unit with hipcc for gfx950.
@@ -192,6 +193,40 @@ void symmetric(backend::HIP, Array3DView<Real> arr, const GridView<Real> &grid, 
   hipLaunchKernelGGL(symmetric_kernel<Real>, dim3(for_each_blocks(n)), dim3(256), 0, stream, arr,
                      margin, sign == Sign::Pos ? Real(1) : Real(-1), dir, side);
   MISO_HIP_RT_CHECK(hipGetLastError());
+}
+
+/// A MISO Source struct -- vx, vy, vz, ei(FieldsView<const Real>, i, j, k)
+/// as __host__ __device__ members (mhd_model_base.hpp:131-155,
+/// demo/mhd3d_magnetosphere/src/src.hpp:4-41) -- as a UserSource: before
+/// every RK stage a kernel evaluates the four members on the stage's argm at
+/// every interior cell and stores them for the stage to add, where the
+/// reference inlines the calls (mhd_integrator_host.hpp:134,151,168,241).
+///
+///   struct PointMass { GridView<double> g; double gg; __host__ __device__
+///     double vx(FieldsView<const double> q, int i, int j, int k) const {...} ... };
+///   auto src = miso::hip::make_source(mhd.grid, PointMass{GridView<double>(mhd.grid), 0.3});
+///   mhd.run(eos, bc, src, nsteps);
+template <typename Real, class Src> UserSource<Real> make_source(const Grid<Real> &grid, Src src) {
+  const misohip_grid g = *grid.desc();
+  return UserSource<Real>(grid, [g, src](const misohip_fields &argm, Real *const out[4],
+                                         hipStream_t stream) {
+    const FieldsView<const Real> q(argm, g);
+    const Array3DView<Real> sx{out[0], g.i_total, g.j_total, g.k_total};
+    const Array3DView<Real> sy{out[1], g.i_total, g.j_total, g.k_total};
+    const Array3DView<Real> sz{out[2], g.i_total, g.j_total, g.k_total};
+    const Array3DView<Real> se{out[3], g.i_total, g.j_total, g.k_total};
+    const Range3D inner{{g.i_margin, g.i_total - g.i_margin},
+                        {g.j_margin, g.j_total - g.j_margin},
+                        {g.k_margin, g.k_total - g.k_margin}};
+    for_each(backend::HIP{}, inner,
+             [=] __device__(int i, int j, int k) {
+               sx(i, j, k) = src.vx(q, i, j, k);
+               sy(i, j, k) = src.vy(q, i, j, k);
+               sz(i, j, k) = src.vz(q, i, j, k);
+               se(i, j, k) = src.ei(q, i, j, k);
+             },
+             stream);
+  });
 }
 
 }  // namespace hip

@@ -141,6 +141,11 @@ typedef int (*misohip_src_fn)(void *user, const struct misohip_fields *argm,
                               void *const arrays[4], void *stream);
 int misohip_set_source_callback(misohip_ctx *ctx, misohip_src_fn fn, void *user,
                                 void *const arrays[4]);
+/* The context's current source callback, its user pointer and arrays (all
+ * NULL when none is set), so that a caller installing its own for one call
+ * can put the previous one back.  No counterpart in the reference. */
+int misohip_get_source_callback(misohip_ctx *ctx, misohip_src_fn *fn, void **user,
+                                void *arrays[4]);
 
 /* Config values the step consumes (config_defaults.hpp, eos.hpp) and the
  * source term: src_kind 0 = none (EmptySourceTerm), 1 = constant gravity

@@ -270,6 +270,9 @@ int misohip_batch_create(misohip_ctx *ctx, misohip_dtype dt, int n, const misohi
                          const misohip_params *p, const misohip_bc *bc, misohip_batch **out) {
   if (!ctx || !grids || !qq || !work || !p || !bc || !out || n < 1) return MISOHIP_E_ARG;
   if (dt != MISOHIP_F64 && dt != MISOHIP_F32) return MISOHIP_E_DTYPE;
+  // sources: none or gravity (a user source callback runs per run, not per batch)
+  if (p->src_kind < 0 || p->src_kind > 1 || (p->src_kind == 1 && (p->grav_axis < 0 || p->grav_axis > 2)))
+    return MISOHIP_E_ARG;
   *out = nullptr;
   MISOHIP_TRY(hipSetDevice(ctx->device));
   misohip_batch *b = new (std::nothrow) misohip_batch;

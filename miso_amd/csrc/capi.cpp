@@ -441,6 +441,15 @@ int misohip_set_source_callback(misohip_ctx *ctx, misohip_src_fn fn, void *user,
   return MISOHIP_OK;
 }
 
+int misohip_get_source_callback(misohip_ctx *ctx, misohip_src_fn *fn, void **user,
+                                void *arrays[4]) {
+  if (!ctx || !fn || !user || !arrays) return MISOHIP_E_ARG;
+  *fn = ctx->src_fn;
+  *user = ctx->src_user;
+  for (int d = 0; d < 4; ++d) arrays[d] = ctx->src_arr[d];
+  return MISOHIP_OK;
+}
+
 int misohip_get_bc_callback(misohip_ctx *ctx, misohip_bc_fn *fn, void **user) {
   if (!ctx || !fn || !user) return MISOHIP_E_ARG;
   *fn = ctx->bc_fn;

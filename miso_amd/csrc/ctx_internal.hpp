@@ -31,6 +31,9 @@ struct misohip_ctx {
   hipEvent_t rt_fork = nullptr, rt_join = nullptr;
   misohip_bc_fn bc_fn = nullptr;  // the user boundary callback and its argument
   void *bc_user = nullptr;
+  misohip_src_fn src_fn = nullptr;  // the user source callback, its argument and arrays
+  void *src_user = nullptr;
+  void *src_arr[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace misohip_detail {

@@ -15,6 +15,9 @@ struct RkScalars {
   int variant;  // 3D kernel: 0 three load phases, 1 one per point, 2 = 1 at 4 waves/SIMD, 3 LDS tile,
                 // 4 z-march over 64 x 4 columns, 5 over 64 x 8
   int zchunk;   // planes walked per block by the z-march kernel
+  // src_kind 2: src.vx, src.vy, src.vz, src.ei of the stage's argm (fields
+  // of the grid's shape, the run's Real)
+  const void *src_arr[4];
 };
 
 // Scalars of one AV sweep (ArtificialViscosity members plus dt).

@@ -139,9 +139,16 @@ __device__ __forceinline__ void rk_update(const R o[9], const Fld<R> &rslt, int 
   const R o_bx = o[4], o_by = o[5], o_bz = o[6];
   const R o_ei = o[7], o_ph = o[8];
 
-  // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134)
+  // source functor (gravity reads argm's density, mhd_integrator_host.hpp:134;
+  // a user source arrives as the four arrays its callback wrote)
   R src[3] = {R(0), R(0), R(0)};
+  R src_ei = R(0);
   if (sc.src_kind == 1) src[sc.grav_axis] = -R(sc.grav) * gro;
+  if (sc.src_kind == 2) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) src[d] = static_cast<const R *>(sc.src_arr[d])[c];
+    src_ei = static_cast<const R *>(sc.src_arr[3])[c];
+  }
 
   // continuity
   const R ro = o_ro + dt * a_ro;
@@ -163,7 +170,7 @@ __device__ __forceinline__ void rk_update(const R o[9], const Fld<R> &rslt, int 
   const R eflux = dt * (a_ei + pii4 * t_ei);
   const double dro = ro, dvx = vx, dvy = vy, dvz = vz, dbx = bx, dby = by, dbz = bz;
   const double ei = (Et + double(eflux) - 0.5 * dro * (+dvx * dvx + dvy * dvy + dvz * dvz) -
-                     double(pii8) * (+dbx * dbx + dby * dby + dbz * dbz) + 0.0) /
+                     double(pii8) * (+dbx * dbx + dby * dby + dbz * dbz) + double(src_ei)) /
                     dro;
 
   rslt.q[0][c] = ro;

@@ -107,6 +107,8 @@ SIGNATURES = {
     "misohip_set_bc_callback": [C.c_void_p, BC_FN, C.c_void_p],
     "misohip_get_bc_callback": [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p)],
     "misohip_set_source_callback": [C.c_void_p, SRC_FN, C.c_void_p, _P(C.c_void_p)],
+    "misohip_get_source_callback": [C.c_void_p, _P(C.c_void_p), _P(C.c_void_p),
+                                    _P(C.c_void_p)],
     "misohip_rt_quadrature": [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                               _P(C.c_double)],
     "misohip_rt_solve": [C.c_void_p, _P(MisohipGrid), _P(MisohipRtConfig), C.c_void_p, C.c_void_p,

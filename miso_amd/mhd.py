@@ -258,11 +258,27 @@ class BoundaryCondition:
         return b
 
 
+class UserSource:
+    """A user source term, Source::vx, vy, vz and ei (mhd_model_base.hpp:
+    131-155, e.g. demo/mhd3d_magnetosphere/src/src.hpp:4-41).  The reference
+    evaluates it on argm at every interior cell of every RK stage
+    (mhd_integrator_host.hpp:134,151,168,241); here fn(argm, out) runs before
+    every stage with argm the (9, Kt, Jt, It) tensor of that stage and out a
+    (4, Kt, Jt, It) tensor, and writes src.vx, src.vy, src.vz, src.ei of
+    every interior cell into out[0..3] with torch operations on the current
+    stream.  The stage adds out[0..2] to the momenta inside dt and out[3] to
+    the internal energy outside it."""
+
+    def __init__(self, fn):
+        self.fn = fn
+
+
 class Integrator:
     """Integrator<Real, backend::HIP> (mhd_integrator_host.hpp:13-357)."""
 
     def __init__(self, config: Config, grid: Grid, exec_ctx: ExecContext, gm: float,
-                 src_kind: int = 0, grav_axis: int = 2, grav: float = 0.0):
+                 src_kind: int = 0, grav_axis: int = 2, grav: float = 0.0,
+                 source: UserSource | None = None):
         av = config["mhd"]["artificial_viscosity"]
         self.grid, self.ctx = grid, exec_ctx
         self.cfl_number = float(config["mhd"]["cfl_number"])
@@ -272,6 +288,9 @@ class Integrator:
         p.ep, p.fh = float(av["ep"]), float(av["fh"])
         p.cs_fac, p.ca_fac, p.vv_fac = float(av["cs_fac"]), float(av["ca_fac"]), float(av["vv_fac"])
         p.src_kind, p.grav_axis, p.grav = int(src_kind), int(grav_axis), float(grav)
+        self.source = source
+        if source is not None:
+            p.src_kind = 2
         if p.ep < 0 or p.fh < 0:
             raise ValueError("artificial viscosity ep and fh must be >= 0")
         self.params = p
@@ -286,14 +305,47 @@ class Integrator:
         w.rslt = self.qq_rslt.c_struct()
         w.cc = self.cc.data_ptr()
         self._work = w
+        self.src_out = None
+        if source is not None:
+            self.src_out = torch.zeros((4,) + tuple(self.qq_argm.data.shape[1:]),
+                                       dtype=self.qq_argm.data.dtype, device=grid.device)
+
+    # -- user sources ----------------------------------------------------------
+    @contextlib.contextmanager
+    def _user_src(self, *fields: Fields):
+        """self.source as the context's source callback for one call: the
+        argm a stage hands over is one of `fields` and the two workspaces."""
+        if self.source is None:
+            yield
+            return
+        tensors = {f.data[0].data_ptr(): f.data for f in fields + (self.qq_argm, self.qq_rslt)}
+        out = self.src_out
+
+        def cb(_user, q, _arrays, _stream):
+            try:
+                self.source.fn(tensors[q.contents.q[0]], out)
+                return 0
+            except Exception:  # noqa: BLE001 -- reported, then the call fails
+                traceback.print_exc()
+                return 1
+
+        fn = hip.SRC_FN(cb)
+        arrays = (C.c_void_p * 4)(*[out[d].data_ptr() for d in range(4)])
+        hip.call("misohip_set_source_callback", self.ctx.handle, fn, None, arrays)
+        try:
+            yield
+        finally:
+            hip.lib().misohip_set_source_callback(self.ctx.handle, hip.SRC_FN(), None, None)
 
     # -- user boundaries -------------------------------------------------------
     @contextlib.contextmanager
     def _user_bc(self, bc: BoundaryCondition, qq: Fields):
         """bc.user as the context's boundary callback for one call: the
-        fields a fill hands over are one of qq and the two workspaces."""
+        fields a fill hands over are one of qq and the two workspaces; the
+        user source, if any, is installed for the call as well."""
         if bc.user is None:
-            yield
+            with self._user_src(qq):
+                yield
             return
         tensors = {f.data[0].data_ptr(): f.data for f in (qq, self.qq_argm, self.qq_rslt)}
 
@@ -308,7 +360,8 @@ class Integrator:
         fn = hip.BC_FN(cb)
         hip.call("misohip_set_bc_callback", self.ctx.handle, fn, None)
         try:
-            yield
+            with self._user_src(qq):
+                yield
         finally:
             hip.lib().misohip_set_bc_callback(self.ctx.handle, hip.BC_FN(), None)
 
@@ -329,10 +382,11 @@ class Integrator:
                    qq_argm: Fields, qq_rslt: Fields) -> None:
         R = self.grid.dtype.type
         damp = float(R(math.exp(float(R(-R(dt_s) / R(tau))))))
-        hip.call("misohip_rk_stage", self.ctx.handle, C.byref(self.grid.c_struct()),
-                 self.dtype_code, C.byref(self.params), C.byref(qq_orgn.c_struct()),
-                 C.byref(qq_argm.c_struct()), C.byref(qq_rslt.c_struct()), float(dt_s),
-                 float(ch2), damp)
+        with self._user_src(qq_orgn, qq_argm):
+            hip.call("misohip_rk_stage", self.ctx.handle, C.byref(self.grid.c_struct()),
+                     self.dtype_code, C.byref(self.params), C.byref(qq_orgn.c_struct()),
+                     C.byref(qq_argm.c_struct()), C.byref(qq_rslt.c_struct()), float(dt_s),
+                     float(ch2), damp)
 
     def av_sweep(self, direction: int, dt: float, qq: Fields, qq_rslt: Fields, cc) -> None:
         hip.call("misohip_av_sweep", self.ctx.handle, C.byref(self.grid.c_struct()),

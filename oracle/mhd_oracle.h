@@ -40,7 +40,9 @@ typedef struct og_grid {
  * `sym_sign[a][f]` (+1 / -1); 2 sets the axis' ghost cells from `fixed`
  * (a user boundary functor writing given ghost values; it runs first, then
  * the mirrors, then the halo exchange).  `src_kind` 1 = constant gravity along
- * `grav_axis` (the force -g*ro of demo/hd2d_rayleigh_taylor/src/main.cpp:109-112).
+ * `grav_axis` (the force -g*ro of demo/hd2d_rayleigh_taylor/src/main.cpp:109-112);
+ * 2 = the point-mass gravity of demo/mhd3d_magnetosphere/src/src.hpp:18-34
+ * (g = `grav`) with a heating src.ei = src_heat * ro.
  */
 typedef struct og_params {
   double gm, cfl_number, ep, fh, cs_fac, ca_fac, vv_fac;
@@ -49,6 +51,7 @@ typedef struct og_params {
   int src_kind;
   int grav_axis;
   double grav;
+  double src_heat;
   const void *fixed; /* periodic[a] == 2: 9 fields (the q layout) whose ghost
                         cells the fill copies in */
 } og_params;

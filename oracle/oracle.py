@@ -34,7 +34,8 @@ class OgParams(C.Structure):
         ("fh", C.c_double), ("cs_fac", C.c_double), ("ca_fac", C.c_double),
         ("vv_fac", C.c_double), ("periodic", C.c_int * 3),
         ("sym_sign", (C.c_int * 9) * 3), ("src_kind", C.c_int),
-        ("grav_axis", C.c_int), ("grav", C.c_double), ("fixed", C.c_void_p)]
+        ("grav_axis", C.c_int), ("grav", C.c_double), ("src_heat", C.c_double),
+        ("fixed", C.c_void_p)]
 
 
 def build(force: bool = False) -> str:
@@ -178,11 +179,13 @@ class Params:
     src_kind: int = 0
     grav_axis: int = 2
     grav: float = 0.0
+    src_heat: float = 0.0  # src_kind 2: src.ei = src_heat * ro
     fixed: object = None  # (9, It, Jt, Kt) array, MISO order, for axes with periodic == 2
 
     def c(self) -> OgParams:
         p = OgParams()
-        for n in ("gm", "cfl_number", "ep", "fh", "cs_fac", "ca_fac", "vv_fac", "grav"):
+        for n in ("gm", "cfl_number", "ep", "fh", "cs_fac", "ca_fac", "vv_fac", "grav",
+                  "src_heat"):
             setattr(p, n, float(getattr(self, n)))
         for a in range(3):
             p.periodic[a] = int(self.periodic[a])

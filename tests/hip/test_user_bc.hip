@@ -129,3 +129,74 @@ int hip_user_bc_fixed_x(int ni, int nj, int nk, const double *q_in, int nsteps, 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- sources
+namespace {
+// the point-mass gravity of demo/mhd3d_magnetosphere/src/src.hpp:18-34 with
+// a heating src.ei = h ro, written as a MISO Source struct
+struct PointMass {
+  mh::GridView<double> grid;
+  double g_grav, heat;
+  __host__ __device__ double r3(int i, int j, int k) const {
+    const double r = sqrt(grid.x[i] * grid.x[i] + grid.y[j] * grid.y[j] + grid.z[k] * grid.z[k]);
+    return r * r * r;
+  }
+  __host__ __device__ double vx(mh::FieldsView<const double> qq, int i, int j, int k) const {
+    return -qq.ro(i, j, k) * g_grav * grid.x[i] / r3(i, j, k);
+  }
+  __host__ __device__ double vy(mh::FieldsView<const double> qq, int i, int j, int k) const {
+    return -qq.ro(i, j, k) * g_grav * grid.y[j] / r3(i, j, k);
+  }
+  __host__ __device__ double vz(mh::FieldsView<const double> qq, int i, int j, int k) const {
+    return -qq.ro(i, j, k) * g_grav * grid.z[k] / r3(i, j, k);
+  }
+  __host__ __device__ double ei(mh::FieldsView<const double> qq, int i, int j, int k) const {
+    return heat * qq.ro(i, j, k);
+  }
+};
+}  // namespace
+
+extern "C" {
+
+// nsteps of MHD::run (use_run) or of cfl + update on a state in MISO order,
+// periodic, with the point-mass Source through make_source; bounds[6] are the
+// box's extents.
+int hip_user_source_point_mass(int ni, int nj, int nk, const double *bounds, const double *q_in,
+                               int nsteps, int use_run, double g_grav, double heat,
+                               double *q_out, double *dts) {
+  try {
+    mh::HostGrid<double> hg(ni, nj, nk, 2, bounds[0], bounds[1], bounds[2], bounds[3], bounds[4],
+                            bounds[5]);
+    mh::ExecContext<double> ctx(0);
+    mh::MHDConfig cfg;
+    cfg.cfl_number = 0.4;
+    mh::MHD<double> mhd(cfg, hg, ctx);
+    const std::size_t N = hg.n_total();
+    auto ic = [&](mh::HostFields<double> &q, const mh::HostGrid<double> &) {
+      for (int f = 0; f < 9; ++f) std::memcpy(q.q[f].data(), q_in + f * N, sizeof(double) * N);
+    };
+    mh::Boundary bc;
+    mhd.apply_initial_condition(ic, bc);
+    auto src = mh::make_source(mhd.grid, PointMass{mh::GridView<double>(mhd.grid), g_grav, heat});
+    mh::IdealEOS eos{5.0 / 3.0};
+    if (use_run) {
+      const std::vector<double> d = mhd.run(eos, bc, src, nsteps);
+      for (int s = 0; s < nsteps; ++s) dts[s] = d[s];
+    } else {
+      for (int s = 0; s < nsteps; ++s) {
+        const double dt = mhd.cfl(eos);
+        mhd.update(dt, eos, bc, src);
+        dts[s] = dt;
+      }
+    }
+    mh::HostFields<double> out(hg.i_total(), hg.j_total(), hg.k_total());
+    mhd.qq.copy_to(ctx, out);
+    for (int f = 0; f < 9; ++f) std::memcpy(q_out + f * N, out.q[f].data(), sizeof(double) * N);
+    return 0;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "hip_user_source_point_mass: %s\n", e.what());
+    return 2;
+  }
+}
+
+}  // extern "C"

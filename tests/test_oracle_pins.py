@@ -709,6 +709,39 @@ def test_rt_linear_field_gives_the_particular_solution():
     assert rt_linear_field_err() < 1e-6
 
 
+# ------------------------------------------------ source terms (A4, f2)
+def source_stage_err(mod=O):
+    """One RK stage from a uniform fluid at rest with the point-mass source
+    (src_kind 2: src.v = -ro g x_a / r^3, src.ei = h ro; the functor of
+    demo/mhd3d_magnetosphere/src/src.hpp:18-34 plus a heating).  Every flux
+    divergence of a uniform state is exactly zero, so update_sc4
+    (mhd_integrator_host.hpp:121-243) reduces to ro' = ro, v' = dt src.v / ro
+    and ei' = (ro ei - ro v'^2 / 2 + src.ei) / ro: the momentum source inside
+    dt, the heating outside it.  Returns the largest relative miss."""
+    g = mod.make_grid((6, 5, 4), 2, bounds=((1.0, 2.0), (-0.5, 0.5), (0.5, 1.5)))
+    gg, heat, dt, ro0, ei0 = 0.3, 0.05, 1e-3, 1.3, 1.5
+    q = uniform(g, ro=ro0, ei=ei0)
+    out = np.zeros_like(q)
+    p = mod.Params(src_kind=2, grav=gg, src_heat=heat)
+    mod.rk_stage(g, p, dt, 1.0, 2.0 * dt, q, q, out)
+    inner = _interior(g)
+    x, y, z = g.axis("x"), g.axis("y"), g.axis("z")
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    r3 = np.sqrt(X * X + Y * Y + Z * Z) ** 3
+    v = [dt * (-ro0 * gg * c / r3) / ro0 for c in (X, Y, Z)]
+    ke = 0.5 * ro0 * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2)
+    ei = (ro0 * ei0 - ke + heat * ro0) / ro0
+    worst = float(np.max(np.abs(out[0][inner] - ro0)) / ro0)
+    for m, want in ((1, v[0]), (2, v[1]), (3, v[2]), (7, ei)):
+        worst = max(worst, float(np.max(np.abs(out[m][inner] - want[inner])) /
+                                 np.max(np.abs(want[inner]))))
+    return worst
+
+
+def test_source_terms_enter_the_stage_where_the_reference_puts_them():
+    assert source_stage_err() < 1e-13
+
+
 # --------------------------------------------------- planted mistakes
 PLANTS = {
     1: "pressure-gradient sign in the x momentum equation",
@@ -727,6 +760,7 @@ PLANTS = {
     14: "RT upwind and downwind source weights swapped",
     15: "RT bilinear corners u10 and u01 transposed",
     16: "Q_rad prefactor 2 pi instead of 4 pi",
+    17: "source heating src.ei multiplied by dt",
 }
 
 
@@ -774,6 +808,8 @@ def _pins_fail(mod):
         fails.append("rt_qrad")
     if rt_linear_field_err(mod) >= 1e-6:
         fails.append("rt_linear_field")
+    if source_stage_err(mod) >= 1e-13:
+        fails.append("source_stage")
     return fails
 
 
