@@ -1,0 +1,49 @@
+"""FP64 parity at the benchmark's own geometry and launch shapes.
+
+The 3D decaying-turbulence workload is benchmarked at 512^3 with tile shapes
+the autotuner picks there; the parity cases in test_gpu_parity.py use small
+grids, where the z-march shortens its chunks and the walks their lengths.
+Here the same tiles run on a grid large enough that none of them shrinks
+(256^3: 4 x 32 march columns of 64 x 8 cells, two 128-plane chunks each, the
+256-cell z walk and the 64-row x-y walk unsplit), five steps of one
+misohip_run call against the oracle, fields and dts bit for bit; and 64^3
+over 50 steps, where rounding differences would have time to grow.
+"""
+import numpy as np
+import pytest
+
+from miso_amd import problems
+from tests.parity_util import assert_bitwise, rel_linf
+from tests.test_gpu_parity import _run_both
+
+pytestmark = pytest.mark.gpu
+
+# (pencil_bx, pencil_by, sweep_chunk, rk_variant, rk_zchunk, av_fuse_xy, av_xy_chunk):
+# the shapes the autotuner chose for the 512^3 FP64 step
+HEADLINE_TILES = (1, 1, 256, 5, 128, 1, 64)
+
+
+def _with_tiles(ctx, tiles, fn):
+    from miso_amd.autotune import get_tiles, set_tiles
+    t0 = get_tiles(ctx)
+    saved = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+             t0.av_fuse_xy, t0.av_xy_chunk)
+    set_tiles(ctx, *tiles)
+    try:
+        return fn()
+    finally:
+        set_tiles(ctx, *saved)
+
+
+@pytest.mark.parametrize("sizes,nsteps", [((256, 256, 256), 5), ((64, 64, 64), 50)])
+def test_turbulence_fp64_headline_tiles(ctx, sizes, nsteps):
+    from oracle import oracle as O
+    O.set_threads(min(16, O.get_threads()))
+    print(f"turbulence {sizes} FP64, {nsteps} steps, tiles {HEADLINE_TILES}", flush=True)
+    out, ref, dts, ref_dts = _with_tiles(
+        ctx, HEADLINE_TILES,
+        lambda: _run_both(ctx, problems.turbulence, sizes, nsteps, use_run=True))
+    assert np.array_equal(dts, ref_dts), (dts, ref_dts)
+    err = rel_linf(out, ref)
+    assert max(err.values()) <= 1e-12, err
+    assert_bitwise(out, ref, f"turbulence {sizes} x {nsteps} steps")
