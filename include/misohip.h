@@ -269,6 +269,27 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                  const misohip_fields *qq, const misohip_work *w,
                  double *dt_out);
 
+/* Bytes of the per-device workspace of misohip_update_noalloc. */
+#define MISOHIP_NOALLOC_SCRATCH_BYTES ((size_t)48 << 20)
+
+/* misohip_update without the workspace argument and without any allocation:
+ * argm, rslt and cc (19 arrays, each rounded up to 256 bytes) live in a
+ * device array of MISOHIP_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on until the process ends, so one array (all cells,
+ * ghosts included) may take at most MISOHIP_NOALLOC_SCRATCH_BYTES / 19 bytes
+ * after rounding: 331,104 cells in FP64 (a 64^3 interior, 68^3 with
+ * ghosts), 662,208 in FP32 (an 83^3 interior); larger grids return
+ * MISOHIP_E_RANGE.  No hipMalloc, hipFree or
+ * event creation happens inside (launch timing is off for the call).  Calls
+ * on one GPU share its array: calls that overlap on one GPU (two streams,
+ * two threads) race.  Results equal misohip_update's bit for bit.  No
+ * counterpart in the reference: it is for callers that cannot allocate --
+ * code under HIP stream capture, tight serving loops, harnesses that forbid
+ * allocation. */
+int misohip_update_noalloc(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                           const misohip_params *p, const misohip_bc *bc,
+                           const misohip_fields *qq, double dt_step);
+
 /* ---------------------------------------------------------------------
  * Radiative transfer (FP64).  Replaces rt::RT::solve and its helpers
  * (miso/include/miso/rt_host.hpp:104-689, CPU-only in the reference) and

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "../../include/misohip.h"
@@ -315,6 +316,57 @@ int run_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p, cons
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Workspace of misohip_update_noalloc: the 19 arrays of misohip_work (argm,
+// rslt, cc) carved out of one file-scope device array, so that a caller that
+// cannot allocate (stream capture, serving loops) can still step a grid.
+// Every GPU holds its own copy of the array, at its own address, for the life
+// of the process; misohip_create looks that address up once per device with
+// the device current.  On the host the name `scratch` is not a device
+// address: only the looked-up pointer is ever passed to a kernel.
+static __device__ __align__(256) unsigned char scratch[MISOHIP_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+void *g_scratch_ptr[kMaxScratchDevices];
+std::mutex g_scratch_mu;
+
+// the current device's copy of `scratch` (called with ctx->device current)
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) return MISOHIP_OK;  // noalloc refuses it
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return MISOHIP_OK;
+  void *p = nullptr;
+  MISOHIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return MISOHIP_E_HIP;
+  g_scratch_ptr[device] = p;
+  return MISOHIP_OK;
+}
+
+// argm, rslt and cc of a grid in the device's scratch array, each array on a
+// 256-byte boundary; MISOHIP_E_RANGE when the 19 arrays do not fit
+int noalloc_work(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt, misohip_work *w) {
+  if (ctx->device < 0 || ctx->device >= kMaxScratchDevices) return MISOHIP_E_RANGE;
+  void *base;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    base = g_scratch_ptr[ctx->device];
+  }
+  if (!base) return MISOHIP_E_ARG;  // a context not made by misohip_create
+  const size_t elt = dt == MISOHIP_F64 ? sizeof(double) : sizeof(float);
+  const size_t n = (size_t)g->i_total * g->j_total * g->k_total;
+  const size_t arr = (n * elt + 255) / 256 * 256;
+  if (arr > MISOHIP_NOALLOC_SCRATCH_BYTES / 19) return MISOHIP_E_RANGE;
+  unsigned char *b = static_cast<unsigned char *>(base);
+  for (int m = 0; m < 9; ++m) {
+    w->argm.q[m] = b + (size_t)m * arr;
+    w->rslt.q[m] = b + (size_t)(9 + m) * arr;
+  }
+  w->cc = b + (size_t)18 * arr;
+  return MISOHIP_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char *misohip_strerror(int code) {
@@ -346,6 +398,13 @@ int misohip_create(int device, void *stream, misohip_ctx **out) {
     if (c->d_slot) (void)hipFree(c->d_slot);
     delete c;
     return hip_fail(e);
+  }
+  // the device's address of the no-allocation workspace, looked up once
+  if (int rc = scratch_setup(device)) {
+    (void)hipFree(c->d_slot);
+    (void)hipHostFree(c->h_slot);
+    delete c;
+    return rc;
   }
   *out = c;
   return MISOHIP_OK;
@@ -683,6 +742,25 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
   if (rc) return rc;
   if (dt_out) *dt_out = d;
   return misohip_update(ctx, g, dt, p, bc, qq, w, d);
+}
+
+int misohip_update_noalloc(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
+                           const misohip_params *p, const misohip_bc *bc,
+                           const misohip_fields *qq, double dt_step) {
+  MISOHIP_COMMON_CHECKS(ctx, g, dt)
+  if (!p || bc_invalid(ctx, bc) || check_fields(qq)) return MISOHIP_E_ARG;
+  if (src_invalid(ctx, p)) return MISOHIP_E_ARG;
+  if (!(dt_step > 0.0)) return MISOHIP_E_ARG;
+  misohip_work w;
+  if (int rc = noalloc_work(ctx, g, dt, &w)) return rc;
+  // profiling records events from a pool that may have to grow; this entry
+  // point must not create any, so it runs untimed
+  const bool prof = ctx->profiling;
+  ctx->profiling = false;
+  const int rc = dt == MISOHIP_F64 ? update_t<double>(ctx, g, p, bc, qq, &w, dt_step)
+                                   : update_t<float>(ctx, g, p, bc, qq, &w, (float)dt_step);
+  ctx->profiling = prof;
+  return rc;
 }
 
 }  // extern "C"
