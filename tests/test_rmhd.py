@@ -262,29 +262,43 @@ def test_fourth_feature_is_the_horizontal_laplacian_of_ln_t(ctx):
 
 @pytest.mark.gpu
 def test_learned_rt_meets_the_accuracy_bar_at_full_column_height(ctx):
-    """The radiative box's learned RT at the full column height of 384 cells
-    (32 x 32 columns, cubic cells): the bench's network (4 features x 384,
-    three hidden layers of 256, BF16 weights, FP32 accumulation), trained on
-    the GPU on columns the plain GPU RT solver produces for boxes with other
-    seeds, a plain solve every 16 steps and the bottom slab on.  At every
-    learned step of three cycles its Q_rad is within relative L2 1e-2 of a
-    converged FP64 RT solve of the same state, and it beats both the network
-    without the solve's correction and the stale last solve."""
+    """The radiative box's learned RT at the full column height of 384 cells:
+    the benchmark's physical box ([0,1]^3, the same stratification, opacity,
+    seeded perturbations and time step) on 128 x 128 columns instead of 384 x
+    384, with the benchmark's recipe -- the network of 4 features x 384,
+    three hidden layers of 256, BF16 weights, FP32 accumulation, trained on
+    the GPU for 400 epochs on 400,000 columns the plain GPU RT solver
+    produces for six boxes with other seeds; a plain solve every 16 steps,
+    the solve's correction and the bottom slab on.  At every learned step of
+    three cycles (and the untimed one before them) its Q_rad is within
+    relative L2 1e-2 of a converged FP64 RT solve of the same state, and it
+    beats both the network without the solve's correction and the stale
+    last solve.  (A box as narrow as it is tall in cells, 32 x 32 x 384 with
+    cubic cells, is not this workload: its perturbations are 12 times
+    shorter horizontally and Q_rad changes by ~90 % between solves.)"""
     import torch
-    from miso_amd.rmhd import rel_l2
+    from miso_amd.mhd import Grid
+    from miso_amd.rmhd import RadiativeMHD, rel_l2
     from miso_amd.surrogate import ColumnSurrogate, collect_columns
-    nxy, nz, every, cycles = 32, 384, 16, 3
+    nxy, nz, every, cycles = 128, 384, 16, 3
     nsteps = every * cycles
-    data_box = _box(ctx, nxy, nz, seed=101)
+
+    def box_(seed):
+        g = Grid(nxy, nxy, nz, bounds=((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), device="cuda:0")
+        b = RadiativeMHD(g, ctx)
+        b.initialize(seed)
+        return b
+
+    data_box = box_(101)
     F, Tq = collect_columns(data_box, seeds=tuple(range(101, 107)), steps=nsteps + every,
-                            sample_every=4, features=4)
+                            sample_every=4, max_columns=400000, features=4)
     del data_box
     sur = ColumnSurrogate(ctx, nz, hidden=256, hidden_layers=3, max_batch=nxy * nxy, seed=0,
                           features=4)
-    mse, val = sur.fit(F, Tq, epochs=150, batch=1024, lr=1e-3)
+    mse, val = sur.fit(F, Tq, epochs=400, batch=1024, lr=1e-3)
     print(f"{F.shape[0]} columns: train mse {mse:.3e}, validation rel L2 {val:.3e}", flush=True)
     del F, Tq
-    box = _box(ctx, nxy, nz, seed=7)
+    box = box_(7)
     box.attach_surrogate(sur, rt_every=every, correct=True, slab_share=0.99)
     errs = {"learned": [], "network": [], "stale": []}
     tmp = torch.empty_like(box.q_rt)
@@ -307,6 +321,7 @@ def test_learned_rt_meets_the_accuracy_bar_at_full_column_height(ctx):
     print(f"bottom slab: {None if box.slab is None else (box.slab.layers, box.slab.buffer)}",
           flush=True)
     assert len(e["learned"]) == nsteps + every - (cycles + 1)
+    assert box.slab is not None
     assert e["learned"].max() <= 1e-2
     assert e["learned"].mean() < e["network"].mean()
     assert e["learned"].mean() < e["stale"].mean()
