@@ -128,8 +128,17 @@ def rk_traffic(n, dtype, variant=0):
 
 
 def calibrate_copy(nbytes=2 << 30, reps=10):
-    """A plain copy kernel of known size: reads and writes `nbytes` per launch."""
+    """Plain streaming kernels of known size: torch's device copy, and the
+    library's hand-written copy, read and write kernels (misohip_stream_bench:
+    16-byte accesses, nontemporal stores) on 4 GiB buffers -- the HBM rate the
+    MHD kernels are compared against.  The torch copy of `nbytes` is also the
+    kernel the counter calibration profiles."""
+    import ctypes as C
+
     import torch
+
+    from miso_amd import hip
+    from miso_amd.mhd import ExecContext
     n = nbytes // 8
     a = torch.ones(n, dtype=torch.float64, device="cuda:0")
     b = torch.empty_like(a)
@@ -142,38 +151,77 @@ def calibrate_copy(nbytes=2 << 30, reps=10):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
-                      "avg_ms": ms, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}), flush=True)
+    out = {"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps,
+           "avg_ms": ms, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+    del a, b
+    big = 4 << 30
+    x = torch.ones(big // 8, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    ctx = ExecContext(0)
+    kinds = {"copy": (0, 2), "read": (1, 1), "write": (2, 1)}
+    for name, (kind, passes) in kinds.items():
+        args = (ctx.handle, kind, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr()), big)
+        hip.call("misohip_stream_bench", *args)
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            e0.record()
+            hip.call("misohip_stream_bench", *args)
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        out[f"hand_{name}_GBps"] = passes * big / (best * 1e-3) / 1e9
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(target_s: float):
-    """The oracle on a bounded sample of the same workload: 3D turbulence,
-    FP64, 128^3, as many steps as fit in ~target_s."""
+def cpu_baseline(target_s: float, q_full=None, n_full: int = 0):
+    """The oracle on the benchmark's own workload: with q_full (the GPU run's
+    initial state, MISO order, ghosts filled) one step -- two when the first
+    took less than target_s / 2 -- of the n_full^3 FP64 turbulence on every
+    host thread; without it a 128^3 sample of the same initial-condition
+    family for ~target_s.  A 128^3 step on one thread gives the per-thread
+    rate."""
     import numpy as np
 
     from miso_amd import problems
     from miso_amd.mhd import Grid
     from oracle import oracle as O
 
-    n = 128
-    og = O.make_grid((n, n, n), 2)
-    g = Grid(n, n, n, 2, device="cpu")
-    q = problems.to_numpy(problems.turbulence(g, device="cpu"))
     p = O.Params(cfl_number=0.4)
-    O.ghost_fill(og, p, q)
     threads = O.get_threads()
-    t0 = time.perf_counter()
-    steps = 0
-    while True:
-        O.run(og, p, q, 1)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= target_s or steps >= 50:
-            break
-    # one more step on one thread (the same sample's per-thread rate)
+    if q_full is not None:
+        n = n_full
+        og = O.make_grid((n, n, n), 2)
+        q = q_full
+        t0 = time.perf_counter()
+        steps = 0
+        while True:
+            O.run(og, p, q, 1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if steps >= 2 or el >= 0.5 * target_s:
+                break
+        del og
+    n1 = 128
+    og1 = O.make_grid((n1, n1, n1), 2)
+    g1 = Grid(n1, n1, n1, 2, device="cpu")
+    q1 = problems.to_numpy(problems.turbulence(g1, device="cpu"))
+    O.ghost_fill(og1, p, q1)
+    if q_full is None:
+        n, og, q = n1, og1, q1
+        t0 = time.perf_counter()
+        steps = 0
+        while True:
+            O.run(og, p, q, 1)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el >= target_s or steps >= 50:
+                break
+    # one more 128^3 step on one thread (the per-thread rate)
     O.set_threads(1)
     t1 = time.perf_counter()
-    O.run(og, p, q, 1)
+    O.run(og1, p, q1, 1)
     el1 = time.perf_counter() - t1
     O.set_threads(threads)
     model = ""
@@ -188,9 +236,12 @@ def cpu_baseline(target_s: float):
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"3D turbulence {n}^3 FP64 (same IC family), {steps} steps in {el:.1f} s, "
-                  f"{threads} OpenMP threads",
-        "single_thread_value": n ** 3 / el1,
+        "sample": (f"3D turbulence {n}^3 FP64, the benchmark's initial state, {steps} "
+                   f"step(s) in {el:.1f} s, {threads} OpenMP threads" if q_full is not None else
+                   f"3D turbulence {n}^3 FP64 (same IC family), {steps} steps in {el:.1f} s, "
+                   f"{threads} OpenMP threads"),
+        "single_thread_value": n1 ** 3 / el1,
+        "single_thread_sample": f"one {n1}^3 step on one thread",
         "host": {"cpu": model, "nproc": os.cpu_count()},
     }
 
@@ -221,7 +272,10 @@ def _div_b(q, dx, dy, dz):
         sl = [slice(2, -2) if (a != ax and f.shape[a] > 1) else slice(None) for a in range(3)]
         parts.append(g[tuple(sl)])
     div = parts[0] + parts[1] + parts[2]
-    bmag = np.sqrt(bx ** 2 + by ** 2 + bz ** 2).max()
+    # |B| over the interior (edge and corner ghost cells are not part of
+    # the state: the oracle leaves them unset)
+    inner = tuple(slice(2, -2) if bx.shape[a] > 1 else slice(None) for a in range(3))
+    bmag = np.sqrt(bx[inner] ** 2 + by[inner] ** 2 + bz[inner] ** 2).max()
     return float(np.abs(div).max() * min(dx, dy, dz) / bmag)
 
 
@@ -334,7 +388,64 @@ def accuracy_leg(args):
         del m
     print(f"[accuracy] fp64 max rel linf {out['f64']['max_rel_linf']:.2e}, "
           f"fp32 {out['f32']['max_rel_linf']:.2e}", file=sys.stderr, flush=True)
+    out["turbulence_3d"] = accuracy_3d(ctx, args)
     ctx.close()
+    return out
+
+
+def accuracy_3d(ctx, args, n=64, nsteps=50):
+    """3D decaying turbulence n^3 (the benchmark's initial-condition family),
+    nsteps of misohip_run at CFL 0.4 in FP64 and in FP32 against the FP64
+    oracle: parity and dts of the FP64 run, per-field error of the FP32 run,
+    div B of both at the start and the end."""
+    import numpy as np
+    import torch
+
+    from miso_amd import mhd, problems
+    from oracle import oracle as O
+
+    cfg = mhd.Config({"mhd": {"cfl_number": 0.4}})
+    bc = mhd.BoundaryCondition((True, True, True))
+    og = O.make_grid((n, n, n), 2)
+    g64 = mhd.Grid(n, n, n, 2, device="cuda:0")
+    q0 = problems.to_numpy(problems.turbulence(g64))
+    ref = q0.copy()
+    p = O.Params(cfl_number=0.4)
+    O.ghost_fill(og, p, ref)
+    t0 = time.perf_counter()
+    ref_dts = O.run(og, p, ref, nsteps)
+    cpu_s = time.perf_counter() - t0
+    h = 1.0 / n
+    out = {"workload": f"3D turbulence {n}^3, gamma 5/3, periodic, CFL 0.4, {nsteps} steps "
+                       "(misohip_run)",
+           "cpu_oracle_seconds": cpu_s, "div_b_rel_oracle_end": _div_b(ref, h, h, h)}
+    sl = (slice(None),) + g64.interior()
+    names = ("ro", "vx", "vy", "vz", "bx", "by", "bz", "ei", "ph")
+    for dt_name, dt in (("f64", np.float64), ("f32", np.float32)):
+        g = g64 if dt == np.float64 else mhd.Grid(n, n, n, 2, dtype=np.float32, device="cuda:0")
+        m = mhd.MHD(cfg, g, ctx, 5.0 / 3.0)
+        m.apply_initial_condition(q0.astype(dt), bc)
+        div0 = _div_b(m.qq.to_host(ctx), h, h, h)
+        dts = m.integrator.run(bc, m.qq, nsteps)
+        torch.cuda.synchronize()
+        q = m.qq.to_host(ctx).astype(np.float64)
+        err = {}
+        for f, nm in enumerate(names):
+            a, b = q[sl][f], ref[sl][f]
+            scale = max(np.abs(b).max(), 1e-300)
+            err[nm] = {"rel_linf": float(np.abs(a - b).max() / scale),
+                       "rel_l2": float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))}
+        r = {"error_vs_fp64_oracle": err,
+             "max_rel_linf": max(e["rel_linf"] for e in err.values()),
+             "dt_equal_to_oracle_every_step": bool(np.array_equal(np.array(dts), ref_dts)),
+             "dt_last_rel_diff": float(abs(dts[-1] - ref_dts[-1]) / ref_dts[-1]),
+             "div_b_rel_start": div0, "div_b_rel_end": _div_b(q, h, h, h)}
+        if dt == np.float64:
+            r["bitwise_equal_to_oracle"] = bool(np.array_equal(q[sl], ref[sl]))
+        out[dt_name] = r
+        del m
+    print(f"[accuracy 3d] fp64 max rel linf {out['f64']['max_rel_linf']:.2e}, fp32 "
+          f"{out['f32']['max_rel_linf']:.2e}", file=sys.stderr, flush=True)
     return out
 
 
@@ -618,6 +729,8 @@ def main():
     del q0
     torch.cuda.synchronize()
     m.integrator.apply_boundary_condition(bc, m.qq)
+    # the initial state in MISO order for the CPU baseline (FP64 runs)
+    q_host = m.qq.to_host(ctx) if (not args.no_cpu and args.dtype == "f64") else None
     tiles = None
     if args.tiles:
         from miso_amd.autotune import set_tiles
@@ -720,7 +833,8 @@ def main():
     if not args.no_rmhd:
         out["rmhd"] = rmhd_leg(args)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, q_host, n)
+        del q_host
     print(json.dumps(out), flush=True)
 
 

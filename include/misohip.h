@@ -269,6 +269,13 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
                  const misohip_fields *qq, const misohip_work *w,
                  double *dt_out);
 
+/* Plain streaming kernels that calibrate the HBM rate the MHD kernels are
+ * compared against: kind 0 copies `bytes` from src to dst, 1 reads src, 2
+ * writes dst; 16-byte accesses, four in flight per lane, nontemporal stores,
+ * 16384 blocks of 256 threads; bytes a multiple of 16.  Stream-ordered.  No
+ * counterpart in the reference: measurement only. */
+int misohip_stream_bench(misohip_ctx *ctx, int kind, const void *src, void *dst, size_t bytes);
+
 /* Bytes of the per-device workspace of misohip_update_noalloc. */
 #define MISOHIP_NOALLOC_SCRATCH_BYTES ((size_t)48 << 20)
 
@@ -421,7 +428,10 @@ typedef struct misohip_tiles {
   int rk_zchunk;  /* planes per block of the z-march kernels, 1..4096 */
   int av_fuse_xy; /* 3D runs: 1 = the x and y AV sweeps as one walk that
                      x-sweeps every row it takes in (the x sweep's ghost fill
-                     drops out), 0 = separate x and y sweeps.  Same results. */
+                     drops out), FP64 code for three waves per SIMD; 2 = the
+                     same walk compiled for two waves per SIMD (no registers
+                     parked in scratch); 0 = separate x and y sweeps.  Same
+                     results. */
   int av_xy_chunk; /* rows walked per wave by that walk, 4..4096 */
 } misohip_tiles;
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t);

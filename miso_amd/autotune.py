@@ -26,6 +26,8 @@ import json
 import os
 import time
 
+import numpy as np
+
 from . import hip
 
 RK_PENCILS = ((1, 4), (2, 4), (2, 8), (4, 2), (4, 4), (4, 8), (4, 16), (8, 4), (8, 8))
@@ -144,10 +146,11 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
         z_times[c] = t.get("av_z", 0.0)
     if min(g.i_size, g.j_size, g.k_size) > 1:
         cz = min(z_times, key=z_times.get)
-        for xc in chunks:
-            set_tiles(ctx, bx, by, cz, var, zc, 1, xc)
-            t = _profile(ctx, run, totals=True)
-            sweep_times[(1, cz, xc)] = sum(t.get(k, 0.0) for k in kinds)
+        for fz in (1, 2) if g.dtype == np.float64 else (1,):
+            for xc in chunks:
+                set_tiles(ctx, bx, by, cz, var, zc, fz, xc)
+                t = _profile(ctx, run, totals=True)
+                sweep_times[(fz, cz, xc)] = sum(t.get(k, 0.0) for k in kinds)
     fz, best_c, xc = min(sweep_times, key=sweep_times.get)
     xc = xc or 64
     set_tiles(ctx, bx, by, best_c, var, zc, fz, xc)
@@ -157,7 +160,7 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
            "times_ms": {"rk_stage": {(f"z{z}v{v}" if v in (4, 5) else f"{a}x{b}v{v}"): t
                                      for (a, b, v, z), t in rk_times.items()},
                         "av_and_ghost_per_run": {
-                            (f"xy{x}z{c}" if f else f"sep{c}"): v
+                            (f"xy{x}z{c}" + ("w2" if f == 2 else "") if f else f"sep{c}"): v
                             for (f, c, x), v in sweep_times.items()}}}
     cache.put(key, out)
     return dict(out, cached=False, seconds=time.perf_counter() - t0)

@@ -349,13 +349,20 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
   const int s = sh.stride(DIR);
   const R ep = R(sc.ep), fh = R(sc.fh);
   int c = sh.idx(i, j, k);
+  // the fields of the cell q steps along DIR from c: a ghost cell of a
+  // wrapped axis is read at its periodic source (cc is not: the x-y walk
+  // stores it on the one-cell ring the walk reads)
+  const bool wd = DIR == 2 && (sc.wrap & 4);
+  int pd = DIR == 1 ? j : k;  // c's coordinate along DIR, advanced with c
+  const int td = DIR == 1 ? sh.Jt : sh.Kt;
+  auto fc = [&](int q) { return wd ? c + (wrap_coord(pd + q, td, m_dir, true) - pd) * s : c + q * s; };
 
   // window: a0 = p, a1 = p+1 (and p+2 loaded per step); slopes at p; flux p-1/2
   AvCons<R> am, a0, a1, a2;
-  load_cons(qq, c - 2 * s, am);
-  load_cons(qq, c - s, a0);
+  load_cons(qq, fc(-2), am);
+  load_cons(qq, fc(-1), a0);
   load_cons(qq, c, a1);
-  load_cons(qq, c + s, a2);
+  load_cons(qq, fc(1), a2);
   R d0[9], fdw[9];
   R cc_m = cc[c - s], cc_0 = cc[c], cc_1 = cc[c + s];
   {
@@ -370,19 +377,34 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
   // shift so that a0 = p0, a1 = p0+1
   a0 = a1;
   a1 = a2;
-  // the fields and speed of row p+2 arrive one step ahead of their use
-  R nf[9], ncc;
+  // the fields and speed of row p+2 arrive two steps ahead of their use
+  // (row p+3 is in flight meanwhile): two waves per SIMD leave the registers
+  // for it, and one step of arithmetic does not cover a load from HBM
+  R nf[9], ncc, nf2[9] = {}, ncc2 = R(0);
+  {
+    const int c2 = fc(2);
 #pragma unroll
-  for (int m = 0; m < 9; ++m) nf[m] = qq.q[m][c + 2 * s];
+    for (int m = 0; m < 9; ++m) nf[m] = qq.q[m][c2];
+  }
   ncc = cc[c + 2 * s];
+  if (len > 1) {  // row p0+3 is inside the array when p0+1 is an output cell
+    const int c3 = fc(3);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) nf2[m] = qq.q[m][c3];
+    ncc2 = cc[c + 3 * s];
+  }
   for (int t = 0; t < len; ++t) {
     AvCons<R> an;
     cons_of(nf, an);
     const R cc_2 = ncc;
-    if (t + 1 < len) {  // row p+3 is inside the array when p+1 is an output cell
 #pragma unroll
-      for (int m = 0; m < 9; ++m) nf[m] = qq.q[m][c + 3 * s];
-      ncc = cc[c + 3 * s];
+    for (int m = 0; m < 9; ++m) nf[m] = nf2[m];
+    ncc = ncc2;
+    if (t + 2 < len) {  // row p+4 is inside the array when p+2 is an output cell
+      const int c4 = fc(4);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) nf2[m] = qq.q[m][c4];
+      ncc2 = cc[c + 4 * s];
     }
     const R cup = R(0.5) * (cc_0 + cc_1);
     R fup[9];
@@ -416,6 +438,7 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
     cc_0 = cc_1;
     cc_1 = cc_2;
     c += s;
+    ++pd;
   }
   }
   if (CFLF) {
@@ -667,8 +690,13 @@ __device__ __forceinline__ void ystep_field(const R w0[9], const R w1[9], const 
   cell_field<R, M>(v0, df, o, efl);
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void av_xy_walk_kernel(Fld<const R> qq, Fld<R> rslt,
+// W: waves per SIMD the FP64 walk is compiled for -- 3 (168 registers; the
+// compiler parks about 20 of them in scratch, and a scratch reload waits
+// for every earlier global load and store) or 2 (190 registers, nothing
+// parked).  FP32 needs about 100 registers and runs 4.
+template <typename R, int W>
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(sizeof(R) == 8 ? W : 4, sizeof(R) == 8 ? W : 4))) void av_xy_walk_kernel(Fld<const R> qq, Fld<R> rslt,
                                                         R *__restrict__ cc_store,
                                                         const R *__restrict__ dxi,
                                                         const R *__restrict__ dyi, Shape sh,
@@ -679,15 +707,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   const int seg = w % nseg, rr = w / nseg, ch = rr % nchunks, k = sh.mk - 1 + rr / nchunks;
   const int lane = threadIdx.x & 63;
   const int i = sh.mi + seg * 60 - 2 + lane;
-  const int ic = min(i, sh.It - 1);
+  // a lane on a ghost column of a wrapped x axis reads its periodic source
+  const int ic = wrap_coord(min(i, sh.It - 1), sh.It, sh.mi, sc.wrap & 1);
   const bool out_lane = lane >= 2 && lane < 62 && i < sh.It - sh.mi;
   const int nj = sh.Jt - 2 * sh.mj;
   const int p0 = ch * chunk, len = min(chunk, nj - p0);
   const R gm = R(sc.gm), csf = R(sc.cs_fac), caf = R(sc.ca_fac), vvf = R(sc.vv_fac);
   if (k < sh.mk || k >= sh.Kt - sh.mk) {  // z ring plane: the frozen speed only
+    // (of the source plane when z is wrapped: cc depends on the cell alone)
+    const int ks = wrap_coord(k, sh.Kt, sh.mk, sc.wrap & 4);
     for (int t = 0; t < len; ++t) {
       const int c = sh.idx(ic, sh.mj + p0 + t, k);
-      const R v = char_speed(qq, c, gm, csf, caf, vvf);
+      const R v = char_speed(qq, sh.idx(ic, sh.mj + p0 + t, ks), gm, csf, caf, vvf);
       if (out_lane) cc_store[c] = v;
     }
     return;
@@ -733,15 +764,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
     w0[m] = w1[m];
     w1[m] = w2[m];
   }
+  R dd_n = dyi[jb];  // the next output row's 1/dy, loaded a row ahead
   for (int t = 0; t < len; ++t) {
     const int j = jb + t;
     R wn[9];
     R cc_2;
     av_xrow_sweep(nf, sh, bc, j + 2, ddx, sc, dt, wn, cc_2);
-    // row j+3 exists (at most the last ghost row) when j+1 is an output row
-    if (t + 1 < len) av_xrow_load(qq, sh, bc, ic, j + 3, k, nf);
+    // row j+3 exists (at most the last ghost row) when j+1 is an output row;
+    // 1/dy of row j+1 comes after it, so waiting for it waits for the row
+    const R dd = dd_n;
+    if (t + 1 < len) {
+      av_xrow_load(qq, sh, bc, ic, j + 3, k, nf);
+      dd_n = dyi[j + 1];
+    }
     const R cup = R(0.5) * (cc_0 + cc_1);
-    const R dd = dyi[j];
     const double Et = cell_et(w0);
     R o[9], efl;
     ystep_field<R, 0>(w0, w1, wn, cup, ep, fh, dd, dt, d0, fdw, o, efl);
@@ -894,8 +930,12 @@ hipError_t launch_av_sweep_xy(hipStream_t st, const Fld<const R> &q, const Fld<R
   const long nw = (long)nseg * nchunks * (sh.Kt - 2 * sh.mk + 2);
   const long nbl = (nw + 3) / 4;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((av_xy_walk_kernel<R>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
-                     cc_store, dxi, dyi, sh, bc, chunk, nchunks, nseg, (int)nw, sc);
+  if (sc.xy_waves == 2)
+    hipLaunchKernelGGL((av_xy_walk_kernel<R, 2>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
+                       cc_store, dxi, dyi, sh, bc, chunk, nchunks, nseg, (int)nw, sc);
+  else
+    hipLaunchKernelGGL((av_xy_walk_kernel<R, 3>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
+                       cc_store, dxi, dyi, sh, bc, chunk, nchunks, nseg, (int)nw, sc);
   return hipGetLastError();
 }
 

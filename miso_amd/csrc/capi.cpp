@@ -122,7 +122,7 @@ BcDev make_bc(const misohip_bc *bc) {
 template <typename R>
 int rk_stage_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
                const misohip_fields *o, const misohip_fields *a, const misohip_fields *r,
-               R dt_s, R ch2, R damp) {
+               R dt_s, R ch2, R damp, int wrap = 0) {
   // a user source first writes src.vx, src.vy, src.vz, src.ei of this
   // stage's argm (the reference evaluates the functor on argm inside the
   // stage, mhd_integrator_host.hpp:134,151,168,241)
@@ -137,6 +137,7 @@ int rk_stage_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   sc.pencil_by = ctx->tile.rk_pencil_by;
   sc.variant = ctx->tile.rk_variant;
   sc.zchunk = ctx->tile.rk_zchunk;
+  sc.wrap = wrap;
   MISOHIP_TRY(launch_rk_stage<R>(ctx->stream, cfld<R>(o), cfld<R>(a), fld<R>(r),
                                  static_cast<const R *>(g->dxi), static_cast<const R *>(g->dyi),
                                  static_cast<const R *>(g->dzi), make_shape(g), sc));
@@ -153,12 +154,13 @@ template <typename R> AvScalars av_scalars(const misohip_params *p, R dt, int ch
 
 template <typename R>
 int av_sweep_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p, int dir, R dt,
-               const misohip_fields *q, const misohip_fields *r, void *cc) {
+               const misohip_fields *q, const misohip_fields *r, void *cc, int wrap = 0) {
   const R *dd = static_cast<const R *>(dir == 0 ? g->dxi : dir == 1 ? g->dyi : g->dzi);
   Timed t_(ctx, MISOHIP_KIND_AV_X + dir);
+  AvScalars sc = av_scalars<R>(p, dt, ctx->tile.sweep_chunk);
+  sc.wrap = wrap;
   MISOHIP_TRY(launch_av_sweep<R>(ctx->stream, dir, cfld<R>(q), fld<R>(r), static_cast<R *>(cc),
-                                 static_cast<const R *>(cc), dd, make_shape(g),
-                                 av_scalars<R>(p, dt, ctx->tile.sweep_chunk)));
+                                 static_cast<const R *>(cc), dd, make_shape(g), sc));
   return MISOHIP_OK;
 }
 
@@ -181,13 +183,17 @@ int cfl_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   return MISOHIP_OK;
 }
 
+// axes: the faces to fill (bit 0 x, 1 y, 2 z); none: nothing to do (only
+// passes without a user boundary leave faces out)
 template <typename R>
-int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const BcDev &b) {
+int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const BcDev &b,
+            int axes = 7) {
+  if (!axes && !ctx->bc_fn) return MISOHIP_OK;
   Timed t_(ctx, MISOHIP_KIND_GHOST);
   // the user boundary first, then the fill (BC::apply, then the halo
   // exchange: mhd_integrator_host.hpp:250-255)
   if (ctx->bc_fn && ctx->bc_fn(ctx->bc_user, q, ctx->stream) != 0) return MISOHIP_E_CALLBACK;
-  MISOHIP_TRY(launch_ghost_fill<R>(ctx->stream, fld<R>(q), sh, b));
+  if (axes) MISOHIP_TRY(launch_ghost_fill<R>(ctx->stream, fld<R>(q), sh, b, axes));
   return MISOHIP_OK;
 }
 
@@ -208,6 +214,18 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   const R dts[4] = {(R)(dt / 4.0), (R)(dt / 3.0), (R)(dt / 2.0), dt};
   const BcDev b = make_bc(bc);
   const Shape sh = make_shape(g);
+  // In 3D, with the x and y AV sweeps as one walk, a z-march RK variant and
+  // no user boundary, the RK stages, the x-y walk and the z walk read each
+  // ghost cell of a periodic axis at its periodic source (the value the fill
+  // would copy there): the fills between those passes leave the periodic
+  // faces out (none at all in a periodic box), and the step's last fill
+  // writes every face as before.
+  const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2 &&
+                       !ctx->bc_fn;
+  int wrap = 0;
+  if (fuse_xy && (ctx->tile.rk_variant == 4 || ctx->tile.rk_variant == 5))
+    for (int a = 0; a < 3; ++a)
+      if (bc->periodic[a] == 1) wrap |= 1 << a;
   // stage 1 reads qq as both orgn and argm; later stages ping-pong between
   // the two workspace field sets.
   const misohip_fields *argm = qq;
@@ -216,9 +234,9 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
     const misohip_fields *out = s == 3 ? &w->rslt : bufs[s & 1];
     if (s == 3 && argm == &w->rslt) out = &w->argm;
     const R damp = (R)std::exp(-dts[s] / tau);
-    int rc = rk_stage_t<R>(ctx, g, p, qq, argm, out, dts[s], ch2, damp);
+    int rc = rk_stage_t<R>(ctx, g, p, qq, argm, out, dts[s], ch2, damp, wrap);
     if (rc) return rc;
-    rc = ghost_t<R>(ctx, out, sh, b);
+    rc = ghost_t<R>(ctx, out, sh, b, 7 & ~wrap);
     if (rc) return rc;
     argm = out;
   }
@@ -232,8 +250,6 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   // sweep reads the callback's ghost rows of the x-swept state and the z
   // sweep the callback's output of the y sweep.
   const bool fuse_z = g->is && g->js && !g->ks && !ctx->bc_fn;
-  const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2 &&
-                       !ctx->bc_fn;
   const misohip_fields *cur = argm;
   const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
   for (int dir = 0; dir < 3; ++dir) {
@@ -243,10 +259,12 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
     const bool last = dir == 2 || (dir == 1 && fuse_z);
     if (dir == 0 && fuse_xy) {
       Timed t_(ctx, MISOHIP_KIND_AV_X);
+      AvScalars sc = av_scalars<R>(p, dt, ctx->tile.av_xy_chunk);
+      sc.wrap = wrap;
+      sc.xy_waves = ctx->tile.av_fuse_xy == 2 ? 2 : 3;
       MISOHIP_TRY(launch_av_sweep_xy<R>(ctx->stream, cfld<R>(cur), fld<R>(dst),
                                         static_cast<R *>(w->cc), static_cast<const R *>(g->dxi),
-                                        static_cast<const R *>(g->dyi), sh, b,
-                                        av_scalars<R>(p, dt, ctx->tile.av_xy_chunk)));
+                                        static_cast<const R *>(g->dyi), sh, b, sc));
       rc = MISOHIP_OK;
     } else if (last && fuse_cfl) {
       Timed t_(ctx, fuse_z ? MISOHIP_KIND_AV_Y : MISOHIP_KIND_AV_Z);
@@ -260,10 +278,11 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
       cf.heat_q = heat_q;
       cf.heat_dt = (double)dt;
       MISOHIP_TRY(launch_cfl_init(ctx->stream, ctx->d_slot));
+      AvScalars sc = av_scalars<R>(p, dt, ctx->tile.sweep_chunk);
+      sc.wrap = fuse_xy ? wrap : 0;
       MISOHIP_TRY(launch_av_last_sweep_cfl<R>(
           ctx->stream, cfld<R>(cur), fld<R>(dst), static_cast<const R *>(w->cc),
-          static_cast<const R *>(g->dyi), static_cast<const R *>(g->dzi), sh,
-          av_scalars<R>(p, dt, ctx->tile.sweep_chunk), cf));
+          static_cast<const R *>(g->dyi), static_cast<const R *>(g->dzi), sh, sc, cf));
       rc = MISOHIP_OK;
     } else if (dir == 1 && fuse_z) {
       Timed t_(ctx, MISOHIP_KIND_AV_Y);
@@ -273,10 +292,12 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
           av_scalars<R>(p, dt, ctx->tile.sweep_chunk)));
       rc = MISOHIP_OK;
     } else {
-      rc = av_sweep_t<R>(ctx, g, p, dir, dt, cur, dst, w->cc);
+      rc = av_sweep_t<R>(ctx, g, p, dir, dt, cur, dst, w->cc, fuse_xy ? wrap : 0);
     }
     if (rc) return rc;
-    rc = ghost_t<R>(ctx, dst, sh, b);
+    // after the x-y walk only the z walk reads the state, along z alone
+    const int axes = last ? 7 : (dir == 0 && fuse_xy && wrap) ? 4 & ~wrap : 7;
+    rc = ghost_t<R>(ctx, dst, sh, b, axes);
     if (rc) return rc;
     if (dir == 1 && fuse_z) break;
     other = cur;
@@ -414,7 +435,7 @@ int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t) {
   if (!ctx || !t) return MISOHIP_E_ARG;
   if (t->rk_pencil_bx < 1 || t->rk_pencil_by < 1 || t->sweep_chunk < 4 || t->sweep_chunk > 4096 ||
       t->rk_variant < 0 || t->rk_variant > 5 || t->rk_zchunk < 1 || t->rk_zchunk > 4096 ||
-      t->av_fuse_xy < 0 || t->av_fuse_xy > 1 || t->av_xy_chunk < 4 || t->av_xy_chunk > 4096)
+      t->av_fuse_xy < 0 || t->av_fuse_xy > 2 || t->av_xy_chunk < 4 || t->av_xy_chunk > 4096)
     return MISOHIP_E_ARG;
   ctx->tile = *t;
   return MISOHIP_OK;
@@ -742,6 +763,15 @@ int misohip_step(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,
   if (rc) return rc;
   if (dt_out) *dt_out = d;
   return misohip_update(ctx, g, dt, p, bc, qq, w, d);
+}
+
+int misohip_stream_bench(misohip_ctx *ctx, int kind, const void *src, void *dst, size_t bytes) {
+  if (!ctx || kind < 0 || kind > 2 || bytes % 16) return MISOHIP_E_ARG;
+  if ((kind != 2 && !src) || (kind != 1 && !dst)) return MISOHIP_E_ARG;
+  MISOHIP_TRY(hipSetDevice(ctx->device));
+  MISOHIP_TRY(launch_stream(ctx->stream, src, dst, bytes, kind,
+                            reinterpret_cast<double *>(ctx->d_slot)));
+  return MISOHIP_OK;
 }
 
 int misohip_update_noalloc(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,

@@ -98,6 +98,15 @@ __device__ __forceinline__ int map_coord(int x, int total, int m, int kind, bool
   return x;
 }
 
+// Periodic source coordinate of x on an axis of `total` cells with margin m
+// when `on` (a ghost coordinate maps to the interior cell that the periodic
+// ghost fill copies into it); x itself otherwise.
+__device__ __forceinline__ int wrap_coord(int x, int total, int m, bool on) {
+  if (!on) return x;
+  const int n = total - 2 * m;
+  return x < m ? x + n : (x >= total - m ? x - n : x);
+}
+
 // Blocks of the z-march kernel: (nbx, nby) columns of 64 x 4 cells, each cut
 // into nzc chunks of kz planes; x fastest, then y, then the chunk, so that
 // with xcd_virtual_block every XCD walks compact x-y regions of one slab.

@@ -87,11 +87,14 @@ __host__ __device__ inline void ghost_counts(const Shape &sh, long &n_x, long &n
 }
 
 template <typename R>
-hipError_t launch_ghost_fill(hipStream_t st, const Fld<R> &q, const Shape &sh, const BcDev &bc) {
+hipError_t launch_ghost_fill(hipStream_t st, const Fld<R> &q, const Shape &sh, const BcDev &bc,
+                             int axes) {
   const long ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
-  const long n_x = 2L * sh.mi * sh.Jt * sh.Kt;
-  const long n_y = 2L * sh.mj * ni * sh.Kt;
-  const long n_z = 2L * sh.mk * ni * nj;
+  // an axis left out contributes no slab; the others keep their extents, so
+  // every cell a filled face holds is still written exactly once
+  const long n_x = (axes & 1) ? 2L * sh.mi * sh.Jt * sh.Kt : 0;
+  const long n_y = (axes & 2) ? 2L * sh.mj * ni * sh.Kt : 0;
+  const long n_z = (axes & 4) ? 2L * sh.mk * ni * nj : 0;
   const long n_tot = n_x + n_y + n_z;
   if (n_tot == 0) return hipSuccess;
   const long blocks = (n_tot + 255) / 256;
@@ -357,7 +360,7 @@ hipError_t launch_transpose(hipStream_t st, const R *src, R *dst, int It, int Jt
 
 #define MISOHIP_INST(R)                                                                           \
   template hipError_t launch_ghost_fill<R>(hipStream_t, const Fld<R> &, const Shape &,           \
-                                           const BcDev &);                                        \
+                                           const BcDev &, int);                                   \
   template hipError_t launch_cfl<R>(hipStream_t, const Fld<const R> &, const R *, const R *,     \
                                     const R *, const Shape &, double, double, unsigned *);        \
   template hipError_t launch_transpose<R>(hipStream_t, const R *, R *, int, int, int, bool);

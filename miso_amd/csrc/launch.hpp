@@ -15,6 +15,10 @@ struct RkScalars {
   int variant;  // 3D kernel: 0 three load phases, 1 one per point, 2 = 1 at 4 waves/SIMD, 3 LDS tile,
                 // 4 z-march over 64 x 4 columns, 5 over 64 x 8
   int zchunk;   // planes walked per block by the z-march kernel
+  // z-march kernel: axes (bit 0 x, 1 y, 2 z) whose ghost cells it does not
+  // read but maps to their periodic source cells, so that no ghost fill
+  // needs to run before it on those axes
+  int wrap = 0;
   // src_kind 2: src.vx, src.vy, src.vz, src.ei of the stage's argm (fields
   // of the grid's shape, the run's Real)
   const void *src_arr[4];
@@ -24,6 +28,10 @@ struct RkScalars {
 struct AvScalars {
   double dt, ep, fh, gm, cs_fac, ca_fac, vv_fac;
   int chunk;  // cells walked per thread by the y/z sweeps
+  // x-y walk and z walk: axes (bit 0 x, 1 y, 2 z) whose ghost cells they
+  // map to periodic source cells instead of reading them
+  int wrap = 0;
+  int xy_waves = 3;  // FP64 x-y walk: waves per SIMD it is compiled for (3 or 2)
 };
 
 // Boundary description on the device: axis kind (0 mirror, 1 periodic,
@@ -100,6 +108,8 @@ template <typename R> struct CflFuse {
   double heat_dt = 0.0;
 };
 hipError_t launch_cfl_init(hipStream_t, unsigned *);
+// plain streaming kernels (calib.hip): kind 0 copy, 1 read, 2 write
+hipError_t launch_stream(hipStream_t, const void *, void *, size_t, int, double *);
 
 // Waves of a walking-sweep block laid along x: all four when the other
 // transverse axis has fewer than four cells (2D runs), else one (the four
@@ -209,7 +219,10 @@ template <typename R>
 hipError_t launch_table2d(hipStream_t, const R *, int, int, R, R, R, R, const R *, const R *, R *,
                           long);
 template <typename R>
-hipError_t launch_ghost_fill(hipStream_t, const Fld<R> &, const Shape &, const BcDev &);
+// axes: bit 0 x, 1 y, 2 z -- the faces to fill (edges of an axis left out
+// keep their values)
+hipError_t launch_ghost_fill(hipStream_t, const Fld<R> &, const Shape &, const BcDev &,
+                             int axes = 7);
 template <typename R>
 hipError_t launch_cfl(hipStream_t, const Fld<const R> &, const R *, const R *, const R *,
                       const Shape &, double, double, unsigned *);

@@ -391,6 +391,12 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   constexpr bool TWO = NH > NT;               // some threads own two halo cells
   __shared__ R lds[9 * TS];
   __shared__ R slot[2 * 9 * NT];  // [plane parity][field][thread]
+  // each thread parks the halo values of the next plane it loaded here
+  // (thread-private cells) until the tile is rebuilt: held in registers
+  // across the update they would be spilled to scratch, and a scratch
+  // access waits for every global load before it, prefetches included
+  constexpr bool PARK = TY >= 8;  // the 4-row tile keeps its halo in registers
+  __shared__ R hbuf[PARK ? 9 * NH : 1];
   const int V = xcd_virtual_block(blockIdx.x, mc.nb);
   const int bx = V % mc.nbx, rr = V / mc.nbx, by = rr % mc.nby, zc = rr / mc.nby;
   const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
@@ -400,14 +406,24 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   const bool valid = i < sh.It - sh.mi && j < sh.Jt - sh.mj;
   const int plane = sh.It * sh.Jt;
   // in-plane offsets; a position outside the array reads cell (0, 0) of the
-  // plane instead, whose value no valid cell uses
-  const int pc = (i < sh.It && j < sh.Jt) ? i + sh.It * j : 0;
+  // plane instead, whose value no valid cell uses.  A thread on a ghost
+  // column or row of a wrapped axis (the last tile of an axis whose size is
+  // not a multiple of the tile's) loads its periodic source: its values are
+  // the x or y halo of the valid cells next to it
+  const bool wx = sc.wrap & 1, wy = sc.wrap & 2, wz = sc.wrap & 4;
+  const int pc = (i < sh.It && j < sh.Jt) ? wrap_coord(i, sh.It, sh.mi, wx) +
+                                                sh.It * wrap_coord(j, sh.Jt, sh.mj, wy)
+                                          : 0;
   const int i0 = bx * 64 + sh.mi - 2, j0 = by * TY + sh.mj - 2;
   const bool has0 = tid < NH;
   const bool has1 = TWO && tid + NT < NH;
   const int p0 = rk_halo_pos<TY>(has0 ? tid : 0), p1 = rk_halo_pos<TY>(has1 ? tid + NT : 0);
-  const int g0i = i0 + p0 % RK_TX, g0j = j0 + p0 / RK_TX;
-  const int g1i = i0 + p1 % RK_TX, g1j = j0 + p1 / RK_TX;
+  // a halo cell on a ghost column or row of a wrapped axis is read at its
+  // periodic source (the value the ghost fill would have copied there)
+  const int g0i = wrap_coord(i0 + p0 % RK_TX, sh.It, sh.mi, wx);
+  const int g0j = wrap_coord(j0 + p0 / RK_TX, sh.Jt, sh.mj, wy);
+  const int g1i = wrap_coord(i0 + p1 % RK_TX, sh.It, sh.mi, wx);
+  const int g1j = wrap_coord(j0 + p1 / RK_TX, sh.Jt, sh.mj, wy);
   const int h0 = (g0i < sh.It && g0j < sh.Jt) ? g0i + sh.It * g0j : 0;
   const int h1 = (g1i < sh.It && g1j < sh.Jt) ? g1i + sh.It * g1j : 0;
   const int tc = (ty + 2) * RK_TX + tx + 2;
@@ -417,16 +433,19 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   const R dt = R(sc.dt), ch2 = R(sc.ch2), damp = R(sc.damp);
   auto sl = [&](int k, int m) -> R & { return slot[((k & 1) * 9 + m) * NT + tid]; };
 
+  // plane q's own cell (a ghost plane of a wrapped z axis: its source)
+  auto zcell = [&](int q) { return pc + wrap_coord(q, sh.Kt, sh.mk, wz) * plane; };
   R zm2[9], zm1[9];  // own cell at planes k-2, k-1
   R zn[9];           // own cell at plane k+2, stored to its slot before the z phase
   {
     const int c = pc + kb * plane, h = kb * plane;
+    const int cm2 = zcell(kb - 2), cm1 = zcell(kb - 1), cp1 = zcell(kb + 1), cp2 = zcell(kb + 2);
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-      zm2[m] = argm.q[m][c - 2 * plane];
-      zm1[m] = argm.q[m][c - plane];
-      sl(kb + 1, m) = argm.q[m][c + plane];
-      zn[m] = argm.q[m][c + 2 * plane];
+      zm2[m] = argm.q[m][cm2];
+      zm1[m] = argm.q[m][cm1];
+      sl(kb + 1, m) = argm.q[m][cp1];
+      zn[m] = argm.q[m][cp2];
       lds[m * TS + tc] = argm.q[m][c];
       if (has0) lds[m * TS + p0] = argm.q[m][h + h0];
       if (has1) lds[m * TS + p1] = argm.q[m][h + h1];
@@ -436,7 +455,19 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
 
   for (int k = kb; k < ke; ++k) {
     const bool more = k + 1 < ke;  // uniform over the block
-    // plane k's orgn values, issued first so that their latency hides
+    // plane k+1's halo, issued first: it lands during the x and y phases
+    // and is parked in LDS before the z phase (the 4-row tile, which has no
+    // room to park it, loads it before the update instead)
+    R ha[9], hb[9];
+    if (PARK && more) {
+      const int h = (k + 1) * plane;
+#pragma unroll
+      for (int m = 0; m < 9; ++m) {
+        ha[m] = has0 ? argm.q[m][h + h0] : R(0);
+        hb[m] = has1 ? argm.q[m][h + h1] : R(0);
+      }
+    }
+    // plane k's orgn values, issued next so that their latency hides
     // behind the x, y and z derivatives (not in the FP64 4-row tile, which
     // would spill more registers)
     constexpr bool EARLY = !SAME && (TY >= 8 || sizeof(R) < 8);
@@ -478,9 +509,14 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     a_ph = a_ph - S[14];
     a_ei = a_ei - S[15];
     t_ei = t_ei + S[16];
-    // plane k+2 (loaded during the previous plane) into the slot plane k left
+    // plane k+2 (loaded during the previous plane) into the slot plane k
+    // left, plane k+1's halo into the parking cells
 #pragma unroll
-    for (int m = 0; m < 9; ++m) sl(k + 2, m) = zn[m];
+    for (int m = 0; m < 9; ++m) {
+      sl(k + 2, m) = zn[m];
+      if (PARK && more && has0) hbuf[m * NH + tid] = ha[m];
+      if (PARK && more && has1) hbuf[m * NH + tid + NT] = hb[m];
+    }
     // ---- z (slots and registers)
     {
       R zp2[9], zp1[9];
@@ -505,9 +541,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     a_ph = a_ph - S[14];
     a_ei = a_ei - S[15];
     t_ei = t_ei + S[16];
-    // plane k+1's halo lands during the update
-    R ha[9], hb[9];
-    if (more) {
+    if (!PARK && more) {
       const int h = (k + 1) * plane;
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
@@ -515,32 +549,34 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
         hb[m] = has1 ? argm.q[m][h + h1] : R(0);
       }
     }
-    R ctr[9];  // plane k's own values: the tile centre
-#pragma unroll
-    for (int m = 0; m < 9; ++m) ctr[m] = lds[m * TS + tc];
     if (valid) {
+      // plane k's own values are the tile centre (orgn of the first stage)
       R o[9];
 #pragma unroll
-      for (int m = 0; m < 9; ++m) o[m] = SAME ? ctr[m] : EARLY ? og[m] : orgn.q[m][pc + k * plane];
-      rk_update<R>(o, rslt, pc + k * plane, ctr[0], sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x,
-                   t_vx, a_vy, d4y, d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph,
-                   a_ei, t_ei);
+      for (int m = 0; m < 9; ++m)
+        o[m] = SAME ? lds[m * TS + tc] : EARLY ? og[m] : orgn.q[m][pc + k * plane];
+      rk_update<R>(o, rslt, pc + k * plane, SAME ? o[0] : lds[tc], sc, dt, ch2, damp, a_ro,
+                   a_vx, d4x, d5x, t_vx, a_vy, d4y, d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by,
+                   a_bz, a_ph, a_ei, t_ei);
     }
     // plane k+3, used in the next plane's z phase
     if (more && k + 3 < sh.Kt) {
-      const int cn = pc + (k + 3) * plane;
+      const int cn = zcell(k + 3);
 #pragma unroll
       for (int m = 0; m < 9; ++m) zn[m] = argm.q[m][cn];
     }
     __syncthreads();  // every wave is done with plane k's tile
     if (more) {
+      // the tile of plane k+1: own centre from the slot (plane k's centre,
+      // which only this thread wrote, moves to the register window first),
+      // halo from the parking cells this thread filled
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
-        lds[m * TS + tc] = sl(k + 1, m);
-        if (has0) lds[m * TS + p0] = ha[m];
-        if (has1) lds[m * TS + p1] = hb[m];
         zm2[m] = zm1[m];
-        zm1[m] = ctr[m];
+        zm1[m] = lds[m * TS + tc];
+        lds[m * TS + tc] = sl(k + 1, m);
+        if (has0) lds[m * TS + p0] = PARK ? hbuf[m * NH + tid] : ha[m];
+        if (has1) lds[m * TS + p1] = PARK ? hbuf[m * NH + tid + NT] : hb[m];
       }
     }
     __syncthreads();

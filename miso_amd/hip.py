@@ -97,6 +97,7 @@ SIGNATURES = {
                        _P(MisohipFields), _P(MisohipWork), C.c_double],
     "misohip_step": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams), _P(MisohipBc),
                      _P(MisohipFields), _P(MisohipWork), _P(C.c_double)],
+    "misohip_stream_bench": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_size_t],
     "misohip_update_noalloc": [C.c_void_p, _P(MisohipGrid), C.c_int, _P(MisohipParams),
                                _P(MisohipBc), _P(MisohipFields), C.c_double],
     "misohip_table1d": [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double,

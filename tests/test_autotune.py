@@ -99,9 +99,11 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
 @pytest.mark.parametrize("ybc", ["periodic", "mirror", "fixed"])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_fused_xy_sweeps_compute_the_same_bits(ctx, ybc, dtype):
-    """The x and y AV sweeps as one walk equal the separate sweeps bit for
-    bit (3D, partial x segments, small and uneven y extents and chunks), for
-    periodic, mirrored (with sign flips) and fixed y boundaries."""
+    """The x and y AV sweeps as one walk (compiled for three or for two waves
+    per SIMD) equal the separate sweeps bit for bit (3D, partial x segments,
+    small and uneven y extents and chunks, both z-march RK tiles, which with
+    the walk read periodic ghosts at their sources), for periodic, mirrored
+    (with sign flips) and fixed y boundaries."""
     import numpy as np
     import torch
     from miso_amd import mhd, problems
@@ -125,7 +127,8 @@ def test_fused_xy_sweeps_compute_the_same_bits(ctx, ybc, dtype):
         start = m.qq.data.clone()
         outs = []
         for tiles in ((2, 8, 64, 0, 32, 0), (2, 8, 64, 0, 32, 1, 64), (2, 8, 4, 0, 32, 1, 4),
-                      (1, 1, 7, 5, 16, 1, 5), (1, 1, 256, 5, 256, 1, 256)):
+                      (1, 1, 7, 5, 16, 1, 5), (1, 1, 256, 5, 256, 1, 256),
+                      (1, 1, 7, 5, 16, 2, 5), (1, 1, 256, 4, 256, 2, 256)):
             m.qq.data.copy_(start)
             set_tiles(ctx, *tiles)
             dts = m.integrator.run(bc, m.qq, 2)

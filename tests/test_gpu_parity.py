@@ -316,14 +316,16 @@ def test_step_fp64_z_march(ctx, case, variant, zchunk):
 
 @pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
                                   "stratified_gravity_16"])
-def test_step_fp64_fused_xy_sweeps(ctx, case):
-    """The one-walk x and y AV sweeps against the oracle, bit for bit."""
+@pytest.mark.parametrize("fuse", [1, 2])
+def test_step_fp64_fused_xy_sweeps(ctx, case, fuse):
+    """The one-walk x and y AV sweeps (three or two waves per SIMD) against
+    the oracle, bit for bit."""
     from miso_amd.autotune import get_tiles, set_tiles
     problem, sizes, n, kw = STEP_CASES[case]
     t0 = get_tiles(ctx)
     default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
                t0.av_fuse_xy, t0.av_xy_chunk)
-    set_tiles(ctx, 1, 1, 16, 5, 32, 1, 16)
+    set_tiles(ctx, 1, 1, 16, 5, 32, fuse, 16)
     try:
         out, ref, dts, ref_dts = _run_both(ctx, problem, sizes, n, use_run=True, **kw)
     finally:
