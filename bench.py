@@ -85,6 +85,15 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the Orszag-Tang accuracy leg")
     ap.add_argument("--accuracy-only", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 timing of the 3D step")
+    ap.add_argument("--trace-markers", action="store_true",
+                    help="launch a one-block int16 fill kernel right before and right after "
+                         "each timed region of the radiative leg (markers 1-2: plain RT at "
+                         "--rmhd-plain-n, 3-4: plain RT at --rmhd-n, 5-6: learned RT), so a "
+                         "kernel trace can be cut to those steps (--trace-window)")
+    ap.add_argument("--trace-window", nargs=3, metavar=("CSV", "FIRST", "STEPS"), default=None,
+                    help="read a rocprofv3 kernel-trace CSV, take the kernels between marker "
+                         "FIRST and marker FIRST+1 (counted from 1), print their time per step "
+                         "over STEPS steps as a markdown table, and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="time a plain device copy of known size (HBM reference for the "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -416,9 +425,13 @@ def accuracy_3d(ctx, args, n=64, nsteps=50):
     ref_dts = O.run(og, p, ref, nsteps)
     cpu_s = time.perf_counter() - t0
     h = 1.0 / n
+    # the oracle's run ends without a ghost fill (misohip_run ends with one):
+    # div B reads face ghosts, so it is taken on a filled copy
+    ref_filled = ref.copy()
+    O.ghost_fill(og, p, ref_filled)
     out = {"workload": f"3D turbulence {n}^3, gamma 5/3, periodic, CFL 0.4, {nsteps} steps "
                        "(misohip_run)",
-           "cpu_oracle_seconds": cpu_s, "div_b_rel_oracle_end": _div_b(ref, h, h, h)}
+           "cpu_oracle_seconds": cpu_s, "div_b_rel_oracle_end": _div_b(ref_filled, h, h, h)}
     sl = (slice(None),) + g64.interior()
     names = ("ro", "vx", "vy", "vz", "bx", "by", "bz", "ei", "ph")
     for dt_name, dt in (("f64", np.float64), ("f32", np.float32)):
@@ -525,6 +538,55 @@ def ensemble_leg(args):
     return out
 
 
+TRACE_MARKER = "FillFunctor<short>"
+
+
+def trace_marker():
+    """A one-block int16 fill on the current stream, after the device is
+    idle: no other kernel of the benchmark fills int16, so it marks a point
+    in a kernel trace."""
+    import torch
+
+    torch.cuda.synchronize()
+    torch.full((64,), 1, dtype=torch.int16, device="cuda:0")
+    torch.cuda.synchronize()
+
+
+def trace_window(path, first, steps):
+    """Per-kernel time per step between trace markers first and first + 1
+    of a rocprofv3 kernel-trace CSV: launches per step, ms per step and
+    share of the kernel time, then the window's wall time per step (its
+    markers' ends and starts) and its summed kernel time per step (kernels
+    on two streams can overlap, so the sum can exceed the wall time)."""
+    import csv
+    import re
+
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    marks = [i for i, r in enumerate(rows) if TRACE_MARKER in r["Kernel_Name"]]
+    if len(marks) < first + 1:
+        sys.exit(f"trace_window: {len(marks)} markers in {path}, need {first + 1}")
+    a, b = marks[first - 1], marks[first]
+    wall = int(rows[b]["Start_Timestamp"]) - int(rows[a]["End_Timestamp"])
+    per = {}
+    busy = 0
+    for r in rows[a + 1:b]:
+        name = re.sub(r"\(.*$", "", re.sub(r"^void ", "", r["Kernel_Name"]))
+        name = name.replace("misohip::", "")
+        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        c = per.setdefault(name, [0, 0])
+        c[0] += 1
+        c[1] += d
+        busy += d
+    print("| kernel | launches per step | ms per step | share of kernel time |")
+    print("|---|---|---|---|")
+    for name, (c, t) in sorted(per.items(), key=lambda kv: -kv[1][1]):
+        if t / 1e6 / steps >= 0.005:
+            print(f"| `{name[:90]}` | {c / steps:.1f} | {t / 1e6 / steps:.3f} | "
+                  f"{100.0 * t / busy:.1f} % |")
+    print(f"\nwindow: {wall / 1e6 / steps:.3f} ms per step wall, {busy / 1e6 / steps:.3f} ms "
+          f"per step of kernel time, {steps:g} steps")
+
+
 def rmhd_leg(args):
     """Radiative MHD on the stratified box: plain GPU RT on every step
     (args.rmhd_plain_n^3 and args.rmhd_n^3) against the learned RT
@@ -561,6 +623,8 @@ def rmhd_leg(args):
         torch.cuda.synchronize()
         i0 = len(box.rt_iters)
         evs = []
+        if args.trace_markers:
+            trace_marker()
         t0 = time.perf_counter()
         for _ in range(steps):  # box.step_plain(), with events between its two parts
             e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
@@ -572,6 +636,8 @@ def rmhd_leg(args):
             evs.append(e)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        if args.trace_markers:
+            trace_marker()
         rt_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
         mhd_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
         # a learned RT can at best remove the solve: (MHD + RT) / MHD
@@ -626,12 +692,16 @@ def rmhd_leg(args):
                          slab_tolerance=args.rmhd_slab_tol)
     for _ in range(every):
         box.step_learned()
+    if args.trace_markers:
+        trace_marker()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(nsteps):
         box.step_learned()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    if args.trace_markers:
+        trace_marker()
     learned = {"cells": n ** 3, "steps": nsteps, "rt_every": every,
                "ms_per_step": 1e3 * el / nsteps, "steps_per_s": nsteps / el,
                "flagged_column_fraction": float(box.flags.float().mean()),
@@ -696,6 +766,10 @@ def main():
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.trace_window:
+        trace_window(args.trace_window[0], int(args.trace_window[1]),
+                     float(args.trace_window[2]))
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return

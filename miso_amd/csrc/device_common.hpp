@@ -39,6 +39,10 @@ __device__ __forceinline__ double dfabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dfabs(float x) { return __builtin_fabsf(x); }
 __device__ __forceinline__ double dcopysign(double x, double y) { return __builtin_copysign(x, y); }
 __device__ __forceinline__ float dcopysign(float x, float y) { return __builtin_copysignf(x, y); }
+// a*b + c rounded once, in the storage type (__builtin_fma alone would take
+// float operands to double: FP64-rate instructions and two conversions)
+__device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (linear block L runs
 // on XCD L % 8, and each XCD takes its blocks in increasing L).  Mapping L to
@@ -162,8 +166,8 @@ __device__ __forceinline__ R dqq_eval(R qq_dw, R qq_cn, R qq_up, R ep) {
 template <typename R>
 __device__ __forceinline__ R flux_core(R qq_dw, R qq_up, R dqq_dw, R dqq_up, R cc, R fh) {
   // 0.5*d is exact, so q + 0.5*d is one correctly rounded fma
-  const R qql = __builtin_fma(R(0.5), dqq_dw, qq_dw);
-  const R qqr = __builtin_fma(R(-0.5), dqq_up, qq_up);
+  const R qql = dfma(R(0.5), dqq_dw, qq_dw);
+  const R qqr = dfma(R(-0.5), dqq_up, qq_up);
   R dqq = qq_up - qq_dw;
   dqq = dcopysign(nmax2(dfabs(dqq), R(1e-20)), dqq);
   const R ra = nmin2(R(1.0), (qqr - qql) / dqq);

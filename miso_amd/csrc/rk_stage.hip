@@ -98,13 +98,13 @@ __device__ __forceinline__ void dir_derivs(const Acc &a, int c, int s, R dd, R g
   for (int m = 0; m < 9; ++m) f[m] = a(m, c + s);
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
-  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(8), t[n], S[n]);
+  for (int n = 0; n < 17; ++n) S[n] = dfma(R(8), t[n], S[n]);
   if (SPLIT) fence17(c, S);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a(m, c - s);
   point_terms<R, D>(f, gm1, t);
 #pragma unroll
-  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(-8), t[n], S[n]);
+  for (int n = 0; n < 17; ++n) S[n] = dfma(R(-8), t[n], S[n]);
   if (SPLIT) fence17(c, S);
 #pragma unroll
   for (int m = 0; m < 9; ++m) f[m] = a(m, c - 2 * s);
@@ -370,10 +370,10 @@ __device__ __forceinline__ void derivs_pts(const R (&p2)[9], const R (&p1)[9], c
   for (int n = 0; n < 17; ++n) S[n] = -t[n];
   point_terms<R, D>(p1, gm1, t);
 #pragma unroll
-  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(8), t[n], S[n]);
+  for (int n = 0; n < 17; ++n) S[n] = dfma(R(8), t[n], S[n]);
   point_terms<R, D>(m1, gm1, t);
 #pragma unroll
-  for (int n = 0; n < 17; ++n) S[n] = __builtin_fma(R(-8), t[n], S[n]);
+  for (int n = 0; n < 17; ++n) S[n] = dfma(R(-8), t[n], S[n]);
   point_terms<R, D>(m2, gm1, t);
 #pragma unroll
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
