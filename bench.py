@@ -108,7 +108,8 @@ def rk_kernel_name(variant, dtype="f64"):
     return {0: f"rk_stage_kernel<{t}, true, true, true, 0>",
             1: f"rk_stage_kernel<{t}, true, true, true, 1>",
             2: f"rk_stage_kernel_w4<{t}, true, true, true>", 3: f"rk_stage_kernel_tile<{t}>",
-            4: f"rk_stage_kernel_march<{t}, 4", 5: f"rk_stage_kernel_march<{t}, 8"}[variant]
+            4: f"rk_stage_kernel_march<{t}, 4", 5: f"rk_stage_kernel_march<{t}, 8",
+            6: f"rk_stage_kernel_march<{t}, {12 if dtype == 'f32' else 8}"}[variant]
 
 
 def rk_traffic(n, dtype, variant=0):
@@ -561,22 +562,26 @@ def trace_window(path, first, steps):
     import csv
     import re
 
-    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
-    marks = [i for i, r in enumerate(rows) if TRACE_MARKER in r["Kernel_Name"]]
+    names = {}  # kernel names interned: a training run traces millions of dispatches
+    rows = []
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            n = names.setdefault(r["Kernel_Name"], r["Kernel_Name"])
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), n))
+    rows.sort(key=lambda t: t[0])
+    marks = [i for i, t in enumerate(rows) if TRACE_MARKER in t[2]]
     if len(marks) < first + 1:
         sys.exit(f"trace_window: {len(marks)} markers in {path}, need {first + 1}")
     a, b = marks[first - 1], marks[first]
-    wall = int(rows[b]["Start_Timestamp"]) - int(rows[a]["End_Timestamp"])
+    wall = rows[b][0] - rows[a][1]
     per = {}
     busy = 0
-    for r in rows[a + 1:b]:
-        name = re.sub(r"\(.*$", "", re.sub(r"^void ", "", r["Kernel_Name"]))
-        name = name.replace("misohip::", "")
-        d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    for s, e, n in rows[a + 1:b]:
+        name = re.sub(r"\(.*$", "", re.sub(r"^void ", "", n)).replace("misohip::", "")
         c = per.setdefault(name, [0, 0])
         c[0] += 1
-        c[1] += d
-        busy += d
+        c[1] += e - s
+        busy += e - s
     print("| kernel | launches per step | ms per step | share of kernel time |")
     print("|---|---|---|---|")
     for name, (c, t) in sorted(per.items(), key=lambda kv: -kv[1][1]):

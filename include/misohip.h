@@ -424,7 +424,9 @@ typedef struct misohip_tiles {
                      neighbourhood staged in LDS, 4 = z-march (a block walks
                      a 64 x 4 column through rk_zchunk planes, z neighbours
                      in registers and LDS, x-y neighbourhood in LDS), 5 = the
-                     z-march over 64 x 8 columns (512 threads) */
+                     z-march over 64 x 8 columns (512 threads), 6 = the
+                     z-march over 64 x 12 columns (768 threads, three waves
+                     per SIMD) for FP32; FP64 runs 6 as 5 */
   int rk_zchunk;  /* planes per block of the z-march kernels, 1..4096 */
   int av_fuse_xy; /* 3D runs: 1 = the x and y AV sweeps as one walk that
                      x-sweeps every row it takes in (the x sweep's ghost fill

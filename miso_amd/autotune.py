@@ -94,6 +94,17 @@ def _profile(ctx, fn, totals=False):
     return {hip.KIND_NAMES[k]: ms[k] / (1 if totals else cnt[k]) for k in range(nk) if cnt[k]}
 
 
+def _timed_rounds(cands, time_one, rounds: int = 2):
+    """{candidate: least time over `rounds` passes}, every other pass in
+    reverse order, so that a drift of the device's clock over the passes
+    does not favour the candidates timed first."""
+    best = {}
+    for r in range(rounds):
+        for c in (cands if r % 2 == 0 else cands[::-1]):
+            best[c] = min(best.get(c, float("inf")), time_one(c))
+    return best
+
+
 def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNKS,
                 zchunks=ZCHUNKS, cache: TileCache | None = None, force: bool = False):
     """Choose and set the RK pencil, variant and z chunk and the sweep chunk
@@ -103,7 +114,8 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
     import torch
     cache = cache or TileCache()
     g = model.grid
-    key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}"
+    # |v6: entries tuned before the 12-row z-march existed are not reused
+    key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}|v6"
     hit = None if force else cache.get(key)
     if hit:
         set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"], hit.get("rk_variant", 0),
@@ -120,44 +132,58 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
         model.qq.data.copy_(saved)
 
     run()  # warm-up (code objects, caches)
-    rk_times = {}
-    variants = (0, 1, 2, 3, 4, 5) if min(g.i_size, g.j_size, g.k_size) > 1 else (0,)
+    rk_cands = []
+    # 6 (the 12-row z-march) is an FP32 kernel; FP64 would run it as 5
+    variants = ((0, 1, 2, 3, 4, 5) + ((6,) if g.dtype == np.float32 else ())
+                if min(g.i_size, g.j_size, g.k_size) > 1 else (0,))
     for var in variants:
-        if var in (4, 5):  # the z-march kernels have no pencils; their knob is the z chunk
+        if var in (4, 5, 6):  # the z-march kernels have no pencils; their knob is the z chunk
             cands = [(1, 1, z) for z in zchunks]
         elif var in (0, 3):
             cands = [(bx, by, 32) for bx, by in pencils]
         else:  # the register-capped variants are tried on the default pencil only
             cands = [(2, 8, 32)]
         for bx, by, z in cands:
-            set_tiles(ctx, bx, by, base.sweep_chunk, var, z)
-            rk_times[(bx, by, var, z)] = _profile(ctx, run).get("rk_stage", float("inf"))
+            rk_cands.append((bx, by, var, z))
+
+    def rk_time(c):
+        set_tiles(ctx, c[0], c[1], base.sweep_chunk, c[2], c[3])
+        return _profile(ctx, run).get("rk_stage", float("inf"))
+
+    rk_times = _timed_rounds(rk_cands, rk_time)
     best_p = min(rk_times, key=rk_times.get)
     bx, by, var, zc = best_p
     # AV: summed time of the sweeps and ghost fills, separate x/y sweeps for
     # every walk length, then (3D) the one-walk x-y sweeps for every walk
     # length at the best z walk
     kinds = ("av_x", "av_y", "av_z", "ghost_fill")
-    sweep_times, z_times = {}, {}
-    for c in chunks:
-        set_tiles(ctx, bx, by, c, var, zc)
+    z_times = {}
+
+    def sep_time(c):
+        set_tiles(ctx, bx, by, c[1], var, zc)
         t = _profile(ctx, run, totals=True)
-        sweep_times[(0, c, 0)] = sum(t.get(k, 0.0) for k in kinds)
-        z_times[c] = t.get("av_z", 0.0)
+        z_times[c[1]] = min(z_times.get(c[1], float("inf")), t.get("av_z", 0.0))
+        return sum(t.get(k, 0.0) for k in kinds)
+
+    sweep_times = _timed_rounds([(0, c, 0) for c in chunks], sep_time)
     if min(g.i_size, g.j_size, g.k_size) > 1:
         cz = min(z_times, key=z_times.get)
-        for fz in (1, 2) if g.dtype == np.float64 else (1,):
-            for xc in chunks:
-                set_tiles(ctx, bx, by, cz, var, zc, fz, xc)
-                t = _profile(ctx, run, totals=True)
-                sweep_times[(fz, cz, xc)] = sum(t.get(k, 0.0) for k in kinds)
+
+        def xy_time(c):
+            set_tiles(ctx, bx, by, c[1], var, zc, c[0], c[2])
+            t = _profile(ctx, run, totals=True)
+            return sum(t.get(k, 0.0) for k in kinds)
+
+        sweep_times.update(_timed_rounds(
+            [(fz, cz, xc) for fz in ((1, 2) if g.dtype == np.float64 else (1,))
+             for xc in chunks], xy_time))
     fz, best_c, xc = min(sweep_times, key=sweep_times.get)
     xc = xc or 64
     set_tiles(ctx, bx, by, best_c, var, zc, fz, xc)
     torch.cuda.synchronize()
     out = {"rk_pencil": [bx, by], "rk_variant": int(var), "rk_zchunk": int(zc),
            "sweep_chunk": int(best_c), "av_fuse_xy": int(fz), "av_xy_chunk": int(xc),
-           "times_ms": {"rk_stage": {(f"z{z}v{v}" if v in (4, 5) else f"{a}x{b}v{v}"): t
+           "times_ms": {"rk_stage": {(f"z{z}v{v}" if v in (4, 5, 6) else f"{a}x{b}v{v}"): t
                                      for (a, b, v, z), t in rk_times.items()},
                         "av_and_ghost_per_run": {
                             (f"xy{x}z{c}" + ("w2" if f == 2 else "") if f else f"sep{c}"): v

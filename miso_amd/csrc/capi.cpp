@@ -223,7 +223,7 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   const bool fuse_xy = g->is && g->js && g->ks && ctx->tile.av_fuse_xy && g->j_size >= 2 &&
                        !ctx->bc_fn;
   int wrap = 0;
-  if (fuse_xy && (ctx->tile.rk_variant == 4 || ctx->tile.rk_variant == 5))
+  if (fuse_xy && ctx->tile.rk_variant >= 4 && ctx->tile.rk_variant <= 6)
     for (int a = 0; a < 3; ++a)
       if (bc->periodic[a] == 1) wrap |= 1 << a;
   // stage 1 reads qq as both orgn and argm; later stages ping-pong between
@@ -434,7 +434,7 @@ int misohip_create(int device, void *stream, misohip_ctx **out) {
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t) {
   if (!ctx || !t) return MISOHIP_E_ARG;
   if (t->rk_pencil_bx < 1 || t->rk_pencil_by < 1 || t->sweep_chunk < 4 || t->sweep_chunk > 4096 ||
-      t->rk_variant < 0 || t->rk_variant > 5 || t->rk_zchunk < 1 || t->rk_zchunk > 4096 ||
+      t->rk_variant < 0 || t->rk_variant > 6 || t->rk_zchunk < 1 || t->rk_zchunk > 4096 ||
       t->av_fuse_xy < 0 || t->av_fuse_xy > 2 || t->av_xy_chunk < 4 || t->av_xy_chunk > 4096)
     return MISOHIP_E_ARG;
   ctx->tile = *t;

@@ -381,7 +381,8 @@ __device__ __forceinline__ void derivs_pts(const R (&p2)[9], const R (&p1)[9], c
 
 // SAME: orgn is argm (the first stage); its values are the tile centre.
 template <typename R, int TY, bool SAME>
-__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+__global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY == 12 ? 3 : 2,
+                                                                        TY == 12 ? 3 : 2))) void
 rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
                       const R *__restrict__ dxi, const R *__restrict__ dyi,
                       const R *__restrict__ dzi, Shape sh, RkScalars sc, March mc) {
@@ -389,7 +390,11 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   constexpr int TS = RK_TX * (TY + 4);        // tile cells
   constexpr int NH = TS - NT;                 // halo cells
   constexpr bool TWO = NH > NT;               // some threads own two halo cells
-  __shared__ R lds[9 * TS];
+  // FP32 has LDS for two tiles: plane k+1's tile is built in the other one
+  // while waves may still read plane k's, so one barrier per plane is enough
+  constexpr bool DB = sizeof(R) == 4 && TY >= 8;
+  __shared__ R tiles[(DB ? 2 : 1) * 9 * TS];
+  int tb = 0;  // offset of plane k's tile in `tiles`
   __shared__ R slot[2 * 9 * NT];  // [plane parity][field][thread]
   // each thread parks the halo values of the next plane it loaded here
   // (thread-private cells) until the tile is rebuilt: held in registers
@@ -446,9 +451,9 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
       zm1[m] = argm.q[m][cm1];
       sl(kb + 1, m) = argm.q[m][cp1];
       zn[m] = argm.q[m][cp2];
-      lds[m * TS + tc] = argm.q[m][c];
-      if (has0) lds[m * TS + p0] = argm.q[m][h + h0];
-      if (has1) lds[m * TS + p1] = argm.q[m][h + h1];
+      tiles[m * TS + tc] = argm.q[m][c];
+      if (has0) tiles[m * TS + p0] = argm.q[m][h + h0];
+      if (has1) tiles[m * TS + p1] = argm.q[m][h + h1];
     }
   }
   __syncthreads();
@@ -478,6 +483,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     }
     R S[17];
     // ---- x (LDS tile)
+    const R *lds = tiles + tb;
     dir_derivs<R, 0, 1>(TileAcc<R, TS>{lds}, tc, 1, ddx, gm1, S);
     R a_ro = -S[0];
     R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
@@ -565,21 +571,23 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
 #pragma unroll
       for (int m = 0; m < 9; ++m) zn[m] = argm.q[m][cn];
     }
-    __syncthreads();  // every wave is done with plane k's tile
+    if (!DB) __syncthreads();  // every wave is done with plane k's tile
     if (more) {
       // the tile of plane k+1: own centre from the slot (plane k's centre,
       // which only this thread wrote, moves to the register window first),
       // halo from the parking cells this thread filled
+      const int nb = DB ? 9 * TS - tb : 0;
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
         zm2[m] = zm1[m];
-        zm1[m] = lds[m * TS + tc];
-        lds[m * TS + tc] = sl(k + 1, m);
-        if (has0) lds[m * TS + p0] = PARK ? hbuf[m * NH + tid] : ha[m];
-        if (has1) lds[m * TS + p1] = PARK ? hbuf[m * NH + tid + NT] : hb[m];
+        zm1[m] = tiles[tb + m * TS + tc];
+        tiles[nb + m * TS + tc] = sl(k + 1, m);
+        if (has0) tiles[nb + m * TS + p0] = PARK ? hbuf[m * NH + tid] : ha[m];
+        if (has1) tiles[nb + m * TS + p1] = PARK ? hbuf[m * NH + tid + NT] : hb[m];
       }
+      if (DB) tb = nb;
     }
-    __syncthreads();
+    __syncthreads();  // plane k+1's tile is complete (and, with DB, plane k's is free)
   }
 }
 
@@ -588,8 +596,11 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
                               const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
                               const Shape &sh, const RkScalars &sc) {
   const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
-  if (HX && HY && HZ && (sc.variant == 4 || sc.variant == 5)) {
-    const int ty = sc.variant == 4 ? 4 : 8;
+  if (HX && HY && HZ && (sc.variant >= 4 && sc.variant <= 6)) {
+    // variant 6: the 12-row column at three waves per SIMD, for FP32 (an
+    // FP64 stage needs more than the 170 registers that allows: it runs the
+    // 8-row column)
+    const int ty = sc.variant == 4 ? 4 : (sc.variant == 6 && sizeof(R) == 4) ? 12 : 8;
     March mc;
     if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 ||
         !mc.init((ni + 63) / 64, (nj + ty - 1) / ty, nk, sc.zchunk))
@@ -602,7 +613,16 @@ static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<c
     else if (ty == 4)
       hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, false>), dim3(mc.nb), dim3(256), 0, st, o,
                          a, r, dxi, dyi, dzi, sh, sc, mc);
-    else if (same)
+    else if (ty == 12) {
+      if constexpr (sizeof(R) == 4) {
+        if (same)
+          hipLaunchKernelGGL((rk_stage_kernel_march<R, 12, true>), dim3(mc.nb), dim3(768), 0, st,
+                             o, a, r, dxi, dyi, dzi, sh, sc, mc);
+        else
+          hipLaunchKernelGGL((rk_stage_kernel_march<R, 12, false>), dim3(mc.nb), dim3(768), 0, st,
+                             o, a, r, dxi, dyi, dzi, sh, sc, mc);
+      }
+    } else if (same)
       hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, true>), dim3(mc.nb), dim3(512), 0, st, o,
                          a, r, dxi, dyi, dzi, sh, sc, mc);
     else

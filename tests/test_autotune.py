@@ -68,15 +68,21 @@ def test_tune_ensemble_keeps_results(ctx, tmp_path):
 
 
 @pytest.mark.gpu
-def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_rk_variants_and_pencils_compute_the_same_bits(ctx, dtype):
+    """Every RK variant, pencil and z chunk gives the same bits, on grids
+    with partial x and y tiles (for the 4-, 8- and 12-row columns) and
+    uneven z chunks; variant 6 is the FP32 12-row column (FP64 runs it as 5)."""
+    import numpy as np
     import torch
     from miso_amd import mhd, problems
     from miso_amd.autotune import get_tiles, set_tiles
     t0 = get_tiles(ctx)
     default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
                t0.av_fuse_xy, t0.av_xy_chunk)
-    for sizes in ((70, 36, 20), (66, 37, 13)):  # partial x and y tiles, uneven z chunks
-        g = mhd.Grid(*sizes, 2, device="cuda:0")
+    dt = np.float64 if dtype == "f64" else np.float32
+    for sizes in ((70, 36, 20), (66, 37, 13), (130, 25, 9)):
+        g = mhd.Grid(*sizes, 2, dtype=dt, device="cuda:0")
         bc = mhd.BoundaryCondition((True, True, True))
         m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
         m.qq.load_ijk(problems.turbulence(g, device="cpu"))
@@ -85,7 +91,8 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx):
         outs = []
         for tiles in ((2, 8, 64, 0), (1, 4, 16, 1), (4, 16, 128, 2), (2, 4, 32, 2), (4, 8, 64, 3),
                       (1, 4, 64, 3), (1, 1, 64, 4, 1), (1, 1, 64, 4, 3), (1, 1, 32, 4, 8),
-                      (1, 1, 64, 4, 64), (1, 1, 64, 5, 1), (1, 1, 64, 5, 7), (1, 1, 64, 5, 32)):
+                      (1, 1, 64, 4, 64), (1, 1, 64, 5, 1), (1, 1, 64, 5, 7), (1, 1, 64, 5, 32),
+                      (1, 1, 64, 6, 1), (1, 1, 64, 6, 7), (1, 1, 64, 6, 32)):
             m.qq.data.copy_(start)
             set_tiles(ctx, *tiles)
             m.integrator.step(bc, m.qq)
@@ -128,7 +135,8 @@ def test_fused_xy_sweeps_compute_the_same_bits(ctx, ybc, dtype):
         outs = []
         for tiles in ((2, 8, 64, 0, 32, 0), (2, 8, 64, 0, 32, 1, 64), (2, 8, 4, 0, 32, 1, 4),
                       (1, 1, 7, 5, 16, 1, 5), (1, 1, 256, 5, 256, 1, 256),
-                      (1, 1, 7, 5, 16, 2, 5), (1, 1, 256, 4, 256, 2, 256)):
+                      (1, 1, 7, 5, 16, 2, 5), (1, 1, 256, 4, 256, 2, 256),
+                      (1, 1, 7, 6, 16, 1, 5), (1, 1, 256, 6, 5, 2, 256)):
             m.qq.data.copy_(start)
             set_tiles(ctx, *tiles)
             dts = m.integrator.run(bc, m.qq, 2)
