@@ -70,6 +70,9 @@ def parse():
     ap.add_argument("--rmhd-epochs", type=int, default=400)
     ap.add_argument("--rmhd-slab-tol", type=float, default=None,
                     help="tolerance of the bottom-slab solves (default: the RT tolerance)")
+    ap.add_argument("--rmhd-slab-iters", type=int, default=2,
+                    help="iterations of a bottom-slab solve (at most; the slab starts from its "
+                         "last solve, or from the last full solve after one)")
     ap.add_argument("--rmhd-slab-share", type=float, default=0.99,
                     help="share of the network's residual at a solve whose bottom layers are "
                          "solved plainly on every step (<= 0: no slab)")
@@ -694,7 +697,8 @@ def rmhd_leg(args):
     box = make_box(n, 7)
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
                          correct=not args.rmhd_no_correct, slab_share=slab_share,
-                         slab_tolerance=args.rmhd_slab_tol)
+                         slab_tolerance=args.rmhd_slab_tol,
+                         slab_max_iters=args.rmhd_slab_iters)
     for _ in range(every):
         box.step_learned()
     if args.trace_markers:
@@ -713,6 +717,7 @@ def rmhd_leg(args):
                "bottom_slab": None if box.slab is None else {
                    "kept_layers": box.slab.layers, "buffer_layers": box.slab.buffer,
                    "residual_share": slab_share, "tolerance": box.slab_kw["tolerance"],
+                   "max_iters": box.slab_kw["max_iters"],
                    "rt_iterations_mean": float(np.mean(box.slab.iters)) if box.slab.iters
                    else None}}
     del box
@@ -721,7 +726,8 @@ def rmhd_leg(args):
     box = make_box(n, 7)
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
                          correct=not args.rmhd_no_correct, slab_share=slab_share,
-                         slab_tolerance=args.rmhd_slab_tol)
+                         slab_tolerance=args.rmhd_slab_tol,
+                         slab_max_iters=args.rmhd_slab_iters)
     errs = {"learned": [], "network_only": [], "network_corrected": [], "stale_rt": []}
     by_age = {}
     tmp = torch.empty_like(box.q_rt)

@@ -27,23 +27,36 @@ namespace misohip {
 // Induction products (the a*b operands of the induction terms along D):
 //  D=x: vx*by vy*bx vx*bz vz*bx;  D=y: vy*bx vx*by vy*bz vz*by;
 //  D=z: vz*bx vx*bz vz*by vy*bz.
-template <typename R, int D>
-__device__ __forceinline__ void point_terms(const R f[9], R gm1, R o[17]) {
+// The seven products of a cell that every direction's terms use:
+//  0..2 ro*vx, ro*vy, ro*vz   3 pr   4 bb   5 ht   6 vb
+// (eos.hpp:30 for pr = (gm - 1) * ro * ei; mhd_integrator_host.hpp:69-89).
+template <typename R>
+__device__ __forceinline__ void cell_products(const R f[9], R gm1, R p[7]) {
   const R ro = f[0], vx = f[1], vy = f[2], vz = f[3];
-  const R bx = f[4], by = f[5], bz = f[6], ei = f[7], ph = f[8];
+  const R bx = f[4], by = f[5], bz = f[6], ei = f[7];
+  p[0] = ro * vx;
+  p[1] = ro * vy;
+  p[2] = ro * vz;
+  p[3] = gm1 * ro * ei;
+  p[4] = bx * bx + by * by + bz * bz;
+  p[5] = ro * ei + p[3] + p[4] * Const<R>::pii4 + R(0.5) * ro * (vx * vx + vy * vy + vz * vz);
+  p[6] = vx * bx + vy * by + vz * bz;
+}
+
+// The 17 terms along D from a cell's fields (ro and ei are not read) and
+// its products.
+template <typename R, int D>
+__device__ __forceinline__ void point_terms_p(const R f[9], const R p[7], R o[17]) {
+  const R vx = f[1], vy = f[2], vz = f[3];
+  const R bx = f[4], by = f[5], bz = f[6], ph = f[8];
   const R vd = D == 0 ? vx : (D == 1 ? vy : vz);
   const R bd = D == 0 ? bx : (D == 1 ? by : bz);
-  const R rvx = ro * vx, rvy = ro * vy, rvz = ro * vz;
-  o[0] = D == 0 ? rvx : (D == 1 ? rvy : rvz);
-  o[1] = rvx * vd;
-  o[2] = rvy * vd;
-  o[3] = rvz * vd;
-  // eos.hpp:30: pr = (gm - 1) * ro * ei
-  const R pr = gm1 * ro * ei;
-  o[4] = pr;
-  // mhd_integrator_host.hpp:69-89
-  const R bb = bx * bx + by * by + bz * bz;
-  o[5] = bb;
+  o[0] = p[D];
+  o[1] = p[0] * vd;
+  o[2] = p[1] * vd;
+  o[3] = p[2] * vd;
+  o[4] = p[3];
+  o[5] = p[4];
   o[6] = bx * bd;
   o[7] = by * bd;
   o[8] = bz * bd;
@@ -56,10 +69,15 @@ __device__ __forceinline__ void point_terms(const R f[9], R gm1, R o[17]) {
   }
   o[13] = ph;
   o[14] = bd;
-  const R ht = ro * ei + pr + bb * Const<R>::pii4 + R(0.5) * ro * (vx * vx + vy * vy + vz * vz);
-  o[15] = ht * vd;
-  const R vb = vx * bx + vy * by + vz * bz;
-  o[16] = vb * bd;
+  o[15] = p[5] * vd;
+  o[16] = p[6] * bd;
+}
+
+template <typename R, int D>
+__device__ __forceinline__ void point_terms(const R f[9], R gm1, R o[17]) {
+  R p[7];
+  cell_products(f, gm1, p);
+  point_terms_p<R, D>(f, p, o);
 }
 
 // Register fence: the 17 running sums are materialised and `idx` becomes
@@ -379,6 +397,44 @@ __device__ __forceinline__ void derivs_pts(const R (&p2)[9], const R (&p1)[9], c
   for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
 }
 
+// derivs_pts with each point's products given beside its fields.
+template <typename R, int D>
+__device__ __forceinline__ void derivs_pts_p(const R (&f2)[9], const R (&q2)[7],
+                                             const R (&f1)[9], const R (&q1)[7],
+                                             const R (&g1)[9], const R (&r1)[7],
+                                             const R (&g2)[9], const R (&r2)[7], R dd, R S[17]) {
+  R t[17];
+  point_terms_p<R, D>(f2, q2, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = -t[n];
+  point_terms_p<R, D>(f1, q1, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = dfma(R(8), t[n], S[n]);
+  point_terms_p<R, D>(g1, r1, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = dfma(R(-8), t[n], S[n]);
+  point_terms_p<R, D>(g2, r2, t);
+#pragma unroll
+  for (int n = 0; n < 17; ++n) S[n] = (S[n] + t[n]) * Const<R>::inv12 * dd;
+}
+
+// dir_derivs over an LDS tile of fields (tile) and one of the cells'
+// products (ptile), both TS cells per field.
+template <typename R, int D, int TS>
+__device__ __forceinline__ void dir_derivs_tp(const R *tile, const R *ptile, int c, int s,
+                                              R dd, R S[17]) {
+  R f[4][9], q[4][7];
+  const int at[4] = {c + 2 * s, c + s, c - s, c - 2 * s};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+#pragma unroll
+    for (int m = 0; m < 9; ++m) f[a][m] = (m == 0 || m == 7) ? R(0) : tile[m * TS + at[a]];
+#pragma unroll
+    for (int n = 0; n < 7; ++n) q[a][n] = ptile[n * TS + at[a]];
+  }
+  derivs_pts_p<R, D>(f[0], q[0], f[1], q[1], f[2], q[2], f[3], q[3], dd, S);
+}
+
 // SAME: orgn is argm (the first stage); its values are the tile centre.
 template <typename R, int TY, bool SAME>
 __global__ __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(TY == 12 ? 3 : 2,
@@ -402,6 +458,14 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   // access waits for every global load before it, prefetches included
   constexpr bool PARK = TY >= 8;  // the 4-row tile keeps its halo in registers
   __shared__ R hbuf[PARK ? 9 * NH : 1];
+  // PRE (FP32, 8-row column): each cell's seven products (cell_products)
+  // are formed once, when its values enter the tile or the thread's z
+  // window, and kept beside them, in a second pair of LDS tiles and in
+  // registers; the derivatives read them instead of forming them at each of
+  // the cell's 12 stencil uses (the same operations on the same values, so
+  // the same bits)
+  constexpr bool PRE = DB && TY == 8 && !TWO;
+  __shared__ R ptiles[PRE ? 2 * 7 * TS : 1];
   const int V = xcd_virtual_block(blockIdx.x, mc.nb);
   const int bx = V % mc.nbx, rr = V / mc.nbx, by = rr % mc.nby, zc = rr / mc.nby;
   const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
@@ -442,18 +506,40 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   auto zcell = [&](int q) { return pc + wrap_coord(q, sh.Kt, sh.mk, wz) * plane; };
   R zm2[9], zm1[9];  // own cell at planes k-2, k-1
   R zn[9];           // own cell at plane k+2, stored to its slot before the z phase
+  // PRE: products of the own cell at planes k-2, k-1, k (the tile centre),
+  // k+1 and k+2
+  R pm2[PRE ? 7 : 1], pm1[PRE ? 7 : 1], pc0[PRE ? 7 : 1], pp1[PRE ? 7 : 1], pp2[PRE ? 7 : 1];
+  auto put_products = [&](const R (&f)[9], int at, R (&p)[7]) {
+    cell_products(f, gm1, p);
+#pragma unroll
+    for (int n = 0; n < 7; ++n) ptiles[n * TS + at] = p[n];
+  };
   {
     const int c = pc + kb * plane, h = kb * plane;
     const int cm2 = zcell(kb - 2), cm1 = zcell(kb - 1), cp1 = zcell(kb + 1), cp2 = zcell(kb + 2);
+    R fc[9], fh[9], fp[9];
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
       zm2[m] = argm.q[m][cm2];
       zm1[m] = argm.q[m][cm1];
-      sl(kb + 1, m) = argm.q[m][cp1];
+      fp[m] = argm.q[m][cp1];
+      sl(kb + 1, m) = fp[m];
       zn[m] = argm.q[m][cp2];
-      tiles[m * TS + tc] = argm.q[m][c];
-      if (has0) tiles[m * TS + p0] = argm.q[m][h + h0];
+      fc[m] = argm.q[m][c];
+      tiles[m * TS + tc] = fc[m];
+      fh[m] = has0 ? argm.q[m][h + h0] : R(0);
+      if (has0) tiles[m * TS + p0] = fh[m];
       if (has1) tiles[m * TS + p1] = argm.q[m][h + h1];
+    }
+    if constexpr (PRE) {
+      cell_products(zm2, gm1, pm2);
+      cell_products(zm1, gm1, pm1);
+      cell_products(fp, gm1, pp1);
+      put_products(fc, tc, pc0);
+      if (has0) {
+        R ph[7];
+        put_products(fh, p0, ph);
+      }
     }
   }
   __syncthreads();
@@ -484,7 +570,11 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     R S[17];
     // ---- x (LDS tile)
     const R *lds = tiles + tb;
-    dir_derivs<R, 0, 1>(TileAcc<R, TS>{lds}, tc, 1, ddx, gm1, S);
+    const R *plds = ptiles + (PRE ? tb / 9 * 7 : 0);
+    if constexpr (PRE)
+      dir_derivs_tp<R, 0, TS>(lds, plds, tc, 1, ddx, S);
+    else
+      dir_derivs<R, 0, 1>(TileAcc<R, TS>{lds}, tc, 1, ddx, gm1, S);
     R a_ro = -S[0];
     R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
     R a_vy = -S[2], t_vy = S[7];
@@ -499,7 +589,10 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
                  "+v"(a_vy), "+v"(t_vy), "+v"(a_vz), "+v"(t_vz), "+v"(a_by), "+v"(k_by),
                  "+v"(a_bz), "+v"(k_bz), "+v"(ph_bx), "+v"(a_ph), "+v"(a_ei), "+v"(t_ei));
     // ---- y (LDS tile)
-    dir_derivs<R, 1, 1>(TileAcc<R, TS>{lds}, cy, RK_TX, ddy, gm1, S);
+    if constexpr (PRE)
+      dir_derivs_tp<R, 1, TS>(lds, plds, cy, RK_TX, ddy, S);
+    else
+      dir_derivs<R, 1, 1>(TileAcc<R, TS>{lds}, cy, RK_TX, ddy, gm1, S);
     a_ro = a_ro - S[0];
     a_vx = a_vx - S[1];
     t_vx = t_vx + S[6];
@@ -524,7 +617,13 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
       if (PARK && more && has1) hbuf[m * NH + tid + NT] = hb[m];
     }
     // ---- z (slots and registers)
-    {
+    if constexpr (PRE) {
+      R zp1[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) zp1[m] = sl(k + 1, m);
+      cell_products(zn, gm1, pp2);
+      derivs_pts_p<R, 2>(zn, pp2, zp1, pp1, zm1, pm1, zm2, pm2, dzi[k], S);
+    } else {
       R zp2[9], zp1[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
@@ -584,6 +683,27 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
         tiles[nb + m * TS + tc] = sl(k + 1, m);
         if (has0) tiles[nb + m * TS + p0] = PARK ? hbuf[m * NH + tid] : ha[m];
         if (has1) tiles[nb + m * TS + p1] = PARK ? hbuf[m * NH + tid + NT] : hb[m];
+      }
+      if constexpr (PRE) {
+        // the new tile's products: its centre is plane k+1 (pp1), its halo
+        // the parked values; then the register window moves up a plane
+        const int pnb = nb / 9 * 7;
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+          ptiles[pnb + n * TS + tc] = pp1[n];
+          pm2[n] = pm1[n];
+          pm1[n] = pc0[n];
+          pc0[n] = pp1[n];
+          pp1[n] = pp2[n];
+        }
+        if (has0) {
+          R fh[9], ph[7];
+#pragma unroll
+          for (int m = 0; m < 9; ++m) fh[m] = hbuf[m * NH + tid];
+          cell_products(fh, gm1, ph);
+#pragma unroll
+          for (int n = 0; n < 7; ++n) ptiles[pnb + n * TS + p0] = ph[n];
+        }
       }
       if (DB) tb = nb;
     }

@@ -67,6 +67,14 @@ class BottomSlabRT:
         self.rt = RT(self.grid, num_rays)
         self.iters = []
 
+    def warm_start(self, full_rt) -> None:
+        """Take the slab's intensities from a solve of the whole box: every
+        ray's field keeps k slowest in both of its layouts, so the slab's
+        planes are the first ones of the box's fields."""
+        n = self.rt.rint[0].numel()
+        self.rt.rint.view(self.rt.num_rays, -1).copy_(
+            full_rt.rint.view(full_rt.num_rays, -1)[:, :n])
+
     def apply(self, ctx, qq, gm, kappa0, s_rad, q_full, **rt_kw):
         self.rt.source_from_fields(ctx, qq, gm, kappa0, s_rad)
         it, _ = self.rt.solve(ctx, **dict(rt_kw, **SLAB_FACES))
@@ -237,7 +245,8 @@ class RadiativeMHD:
     def attach_surrogate(self, surrogate, rt_every: int = 16, flag_tol: float | None = None,
                          correct: bool = True, slab_share: float | None = 0.99,
                          slab_tau: float = 10.0, slab_max_frac: float = 0.25,
-                         slab_tolerance: float | None = None) -> None:
+                         slab_tolerance: float | None = None,
+                         slab_max_iters: int | None = None) -> None:
         """Learned RT: the network every step, a plain solve every rt_every
         steps.  correct: add the column correction the last solve left (the
         solve minus the network on the same state), so the network carries
@@ -247,8 +256,9 @@ class RadiativeMHD:
         the bottom layers holding this share of the network's squared
         residual, topped by slab_tau of optical depth, are handed to a
         plain slab solve on every step (to slab_tolerance, default the
-        run's RT tolerance), if they fit in slab_max_frac of the
-        height (None: no slab)."""
+        run's RT tolerance, in at most slab_max_iters iterations, default
+        the run's limit), if they fit in slab_max_frac of the height (None:
+        no slab)."""
         import torch
         self.surrogate = surrogate
         self.rt_every = int(rt_every)
@@ -258,6 +268,8 @@ class RadiativeMHD:
         self.slab_kw = dict(self.rt_kw)
         if slab_tolerance is not None:
             self.slab_kw["tolerance"] = float(slab_tolerance)
+        if slab_max_iters is not None:
+            self.slab_kw["max_iters"] = int(slab_max_iters)
         self.slab = None
         ncol = self.grid.i_size * self.grid.j_size
         self.flags = torch.zeros(ncol, dtype=torch.int32, device=self.grid.device)
@@ -336,6 +348,9 @@ class RadiativeMHD:
             self.slab = None
         elif cur is None or kept > cur.layers or 2 * kept < cur.layers or buf != cur.buffer:
             self.slab = BottomSlabRT(g, kept, buf)
+        if self.slab is not None:
+            # the solve just made is the best start for the slab's next solve
+            self.slab.warm_start(self.rt)
 
 
 def rel_l2(a, b, grid) -> float:

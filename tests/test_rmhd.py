@@ -216,7 +216,10 @@ def test_learned_rt_beats_stale_rt_between_solves(ctx, features):
 @pytest.mark.parametrize("kept", [2, 6])
 def test_bottom_slab_solve_matches_the_full_solve(ctx, kept):
     """A plain RT solve of the bottom kept + buffer layers (buffer: optical
-    depth 10) reproduces the full solve's Q_rad on the kept layers."""
+    depth 10) reproduces the full solve's Q_rad on the kept layers; so does
+    one iteration of a slab warm-started from the full solve of the same
+    state (the copy takes each ray's first planes in the ray's own layout:
+    from zero intensities one iteration lacks the periodic inflow)."""
     import torch
     from miso_amd.rmhd import BottomSlabRT, slab_layers
     box = _box(ctx, 32, 64)
@@ -240,6 +243,20 @@ def test_bottom_slab_solve_matches_the_full_solve(ctx, kept):
     print(f"kept {kept} buffer {buf}: slab vs full rel L2 {err:.2e}", flush=True)
     assert err < 1e-5
     assert torch.count_nonzero(q[m + kept:]) == 0
+    errs = {}
+    for warm in (False, True):
+        s2 = BottomSlabRT(g, k, buf)
+        if warm:
+            s2.warm_start(box._rt_ref)
+        q2 = torch.zeros_like(full)
+        s2.apply(ctx, box.model.qq, box.strat.gm, box.kappa0, box.s_rad, q2,
+                 **dict(box.rt_kw, max_iters=1))
+        assert s2.iters == [1]
+        a2 = q2[m:m + kept, m:m + g.j_size, m:m + g.i_size]
+        errs[warm] = float((a2 - b).norm() / b.norm())
+    print(f"one iteration: cold {errs[False]:.2e}, warm {errs[True]:.2e}", flush=True)
+    assert errs[True] < 1e-5
+    assert errs[False] > 10 * errs[True]
 
 
 @pytest.mark.gpu
@@ -269,7 +286,8 @@ def test_learned_rt_meets_the_accuracy_bar_at_full_column_height(ctx):
     three hidden layers of 256, BF16 weights, FP32 accumulation, trained on
     the GPU for 400 epochs on 400,000 columns the plain GPU RT solver
     produces for six boxes with other seeds; a plain solve every 16 steps,
-    the solve's correction and the bottom slab on.  At every learned step of
+    the solve's correction and the bottom slab on (two iterations per step,
+    warm-started from each solve).  At every learned step of
     three cycles (and the untimed one before them) its Q_rad is within
     relative L2 1e-2 of a converged FP64 RT solve of the same state, and it
     beats both the network without the solve's correction and the stale
@@ -299,7 +317,8 @@ def test_learned_rt_meets_the_accuracy_bar_at_full_column_height(ctx):
     print(f"{F.shape[0]} columns: train mse {mse:.3e}, validation rel L2 {val:.3e}", flush=True)
     del F, Tq
     box = box_(7)
-    box.attach_surrogate(sur, rt_every=every, correct=True, slab_share=0.99)
+    # the benchmark's slab: two iterations per step, warm-started from each solve
+    box.attach_surrogate(sur, rt_every=every, correct=True, slab_share=0.99, slab_max_iters=2)
     errs = {"learned": [], "network": [], "stale": []}
     tmp = torch.empty_like(box.q_rt)
 
@@ -321,7 +340,7 @@ def test_learned_rt_meets_the_accuracy_bar_at_full_column_height(ctx):
     print(f"bottom slab: {None if box.slab is None else (box.slab.layers, box.slab.buffer)}",
           flush=True)
     assert len(e["learned"]) == nsteps + every - (cycles + 1)
-    assert box.slab is not None
+    assert box.slab is not None and max(box.slab.iters) <= 2
     assert e["learned"].max() <= 1e-2
     assert e["learned"].mean() < e["network"].mean()
     assert e["learned"].mean() < e["stale"].mean()
