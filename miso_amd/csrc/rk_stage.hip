@@ -446,25 +446,27 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   constexpr int TS = RK_TX * (TY + 4);        // tile cells
   constexpr int NH = TS - NT;                 // halo cells
   constexpr bool TWO = NH > NT;               // some threads own two halo cells
-  // FP32 has LDS for two tiles: plane k+1's tile is built in the other one
-  // while waves may still read plane k's, so one barrier per plane is enough
-  constexpr bool DB = sizeof(R) == 4 && TY >= 8;
+  // DB (the 8- and 12-row columns): two x-y tiles.  While the block works
+  // on plane k's tile, each thread writes its share of plane k+1's into the
+  // other one -- its own cell from its z slot, its halo cell as soon as the
+  // y phase is done (the halo loads were issued at the top of the
+  // iteration, so they do not stay in registers across the z phase and the
+  // update, where they would spill, and a scratch access waits for every
+  // global load before it, prefetches included) -- so one block barrier per
+  // plane is enough, and one slot (plane k+1, then k+2) per thread.  The
+  // 4-row column (registers too few to issue the halo early, LDS too small
+  // for nothing) keeps one tile, two slots and two barriers per plane.
+  constexpr bool DB = TY >= 8;
   __shared__ R tiles[(DB ? 2 : 1) * 9 * TS];
   int tb = 0;  // offset of plane k's tile in `tiles`
-  __shared__ R slot[2 * 9 * NT];  // [plane parity][field][thread]
-  // each thread parks the halo values of the next plane it loaded here
-  // (thread-private cells) until the tile is rebuilt: held in registers
-  // across the update they would be spilled to scratch, and a scratch
-  // access waits for every global load before it, prefetches included
-  constexpr bool PARK = TY >= 8;  // the 4-row tile keeps its halo in registers
-  __shared__ R hbuf[PARK ? 9 * NH : 1];
+  __shared__ R slot[(DB ? 1 : 2) * 9 * NT];  // [plane parity][field][thread]
   // PRE (FP32, 8-row column): each cell's seven products (cell_products)
   // are formed once, when its values enter the tile or the thread's z
   // window, and kept beside them, in a second pair of LDS tiles and in
   // registers; the derivatives read them instead of forming them at each of
   // the cell's 12 stencil uses (the same operations on the same values, so
   // the same bits)
-  constexpr bool PRE = DB && TY == 8 && !TWO;
+  constexpr bool PRE = DB && sizeof(R) == 4 && TY == 8 && !TWO;
   __shared__ R ptiles[PRE ? 2 * 7 * TS : 1];
   const int V = xcd_virtual_block(blockIdx.x, mc.nb);
   const int bx = V % mc.nbx, rr = V / mc.nbx, by = rr % mc.nby, zc = rr / mc.nby;
@@ -500,7 +502,9 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   const R pii8 = Const<R>::pii8;
   const R ddx = dxi[valid ? i : sh.mi], ddy = dyi[valid ? j : sh.mj];
   const R dt = R(sc.dt), ch2 = R(sc.ch2), damp = R(sc.damp);
-  auto sl = [&](int k, int m) -> R & { return slot[((k & 1) * 9 + m) * NT + tid]; };
+  auto sl = [&](int k, int m) -> R & {
+    return slot[((DB ? 0 : (k & 1)) * 9 + m) * NT + tid];
+  };
 
   // plane q's own cell (a ghost plane of a wrapped z axis: its source)
   auto zcell = [&](int q) { return pc + wrap_coord(q, sh.Kt, sh.mk, wz) * plane; };
@@ -547,10 +551,10 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   for (int k = kb; k < ke; ++k) {
     const bool more = k + 1 < ke;  // uniform over the block
     // plane k+1's halo, issued first: it lands during the x and y phases
-    // and is parked in LDS before the z phase (the 4-row tile, which has no
-    // room to park it, loads it before the update instead)
+    // and goes into the next tile before the z phase (the 4-row tile loads
+    // it before the update instead)
     R ha[9], hb[9];
-    if (PARK && more) {
+    if (DB && more) {
       const int h = (k + 1) * plane;
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
@@ -608,22 +612,37 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     a_ph = a_ph - S[14];
     a_ei = a_ei - S[15];
     t_ei = t_ei + S[16];
-    // plane k+2 (loaded during the previous plane) into the slot plane k
-    // left, plane k+1's halo into the parking cells
-#pragma unroll
-    for (int m = 0; m < 9; ++m) {
-      sl(k + 2, m) = zn[m];
-      if (PARK && more && has0) hbuf[m * NH + tid] = ha[m];
-      if (PARK && more && has1) hbuf[m * NH + tid + NT] = hb[m];
-    }
     // ---- z (slots and registers)
-    if constexpr (PRE) {
+    const int nb = DB ? 9 * TS - tb : 0;  // plane k+1's tile
+    if constexpr (DB) {
+      // plane k+1 from the slot into the next tile's centre, plane k+2
+      // (loaded during the previous plane) into the slot, plane k+1's halo
+      // into the next tile
       R zp1[9];
 #pragma unroll
-      for (int m = 0; m < 9; ++m) zp1[m] = sl(k + 1, m);
-      cell_products(zn, gm1, pp2);
-      derivs_pts_p<R, 2>(zn, pp2, zp1, pp1, zm1, pm1, zm2, pm2, dzi[k], S);
+      for (int m = 0; m < 9; ++m) {
+        zp1[m] = sl(k + 1, m);
+        tiles[nb + m * TS + tc] = zp1[m];
+        sl(k + 2, m) = zn[m];
+        if (more && has0) tiles[nb + m * TS + p0] = ha[m];
+      }
+      if constexpr (PRE) {
+        const int pnb = nb / 9 * 7;
+        cell_products(zn, gm1, pp2);
+        R ph[7];
+        if (more && has0) cell_products(ha, gm1, ph);
+#pragma unroll
+        for (int n = 0; n < 7; ++n) {
+          ptiles[pnb + n * TS + tc] = pp1[n];
+          if (more && has0) ptiles[pnb + n * TS + p0] = ph[n];
+        }
+        derivs_pts_p<R, 2>(zn, pp2, zp1, pp1, zm1, pm1, zm2, pm2, dzi[k], S);
+      } else {
+        derivs_pts<R, 2>(zn, zp1, zm1, zm2, dzi[k], gm1, S);
+      }
     } else {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) sl(k + 2, m) = zn[m];
       R zp2[9], zp1[9];
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
@@ -646,7 +665,7 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
     a_ph = a_ph - S[14];
     a_ei = a_ei - S[15];
     t_ei = t_ei + S[16];
-    if (!PARK && more) {
+    if (!DB && more) {
       const int h = (k + 1) * plane;
 #pragma unroll
       for (int m = 0; m < 9; ++m) {
@@ -670,44 +689,44 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
 #pragma unroll
       for (int m = 0; m < 9; ++m) zn[m] = argm.q[m][cn];
     }
-    if (!DB) __syncthreads();  // every wave is done with plane k's tile
-    if (more) {
-      // the tile of plane k+1: own centre from the slot (plane k's centre,
-      // which only this thread wrote, moves to the register window first),
-      // halo from the parking cells this thread filled
-      const int nb = DB ? 9 * TS - tb : 0;
+    if constexpr (DB) {
+      if (more) {
+        // plane k's centre (which only this thread wrote) moves to the
+        // register window, and plane k+1's tile becomes the current one
 #pragma unroll
-      for (int m = 0; m < 9; ++m) {
-        zm2[m] = zm1[m];
-        zm1[m] = tiles[tb + m * TS + tc];
-        tiles[nb + m * TS + tc] = sl(k + 1, m);
-        if (has0) tiles[nb + m * TS + p0] = PARK ? hbuf[m * NH + tid] : ha[m];
-        if (has1) tiles[nb + m * TS + p1] = PARK ? hbuf[m * NH + tid + NT] : hb[m];
-      }
-      if constexpr (PRE) {
-        // the new tile's products: its centre is plane k+1 (pp1), its halo
-        // the parked values; then the register window moves up a plane
-        const int pnb = nb / 9 * 7;
-#pragma unroll
-        for (int n = 0; n < 7; ++n) {
-          ptiles[pnb + n * TS + tc] = pp1[n];
-          pm2[n] = pm1[n];
-          pm1[n] = pc0[n];
-          pc0[n] = pp1[n];
-          pp1[n] = pp2[n];
+        for (int m = 0; m < 9; ++m) {
+          zm2[m] = zm1[m];
+          zm1[m] = tiles[tb + m * TS + tc];
         }
-        if (has0) {
-          R fh[9], ph[7];
+        if constexpr (PRE) {
 #pragma unroll
-          for (int m = 0; m < 9; ++m) fh[m] = hbuf[m * NH + tid];
-          cell_products(fh, gm1, ph);
+          for (int n = 0; n < 7; ++n) {
+            pm2[n] = pm1[n];
+            pm1[n] = pc0[n];
+            pc0[n] = pp1[n];
+            pp1[n] = pp2[n];
+          }
+        }
+        tb = nb;
+      }
+      __syncthreads();  // plane k+1's tile is complete and plane k's is free
+    } else {
+      __syncthreads();  // every wave is done with plane k's tile
+      if (more) {
+        // the tile of plane k+1: own centre from the slot (plane k's centre,
+        // which only this thread wrote, moves to the register window first),
+        // halo from the loads issued before the update
 #pragma unroll
-          for (int n = 0; n < 7; ++n) ptiles[pnb + n * TS + p0] = ph[n];
+        for (int m = 0; m < 9; ++m) {
+          zm2[m] = zm1[m];
+          zm1[m] = tiles[m * TS + tc];
+          tiles[m * TS + tc] = sl(k + 1, m);
+          if (has0) tiles[m * TS + p0] = ha[m];
+          if (has1) tiles[m * TS + p1] = hb[m];
         }
       }
-      if (DB) tb = nb;
+      __syncthreads();
     }
-    __syncthreads();  // plane k+1's tile is complete (and, with DB, plane k's is free)
   }
 }
 
