@@ -37,7 +37,8 @@ constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;  // LDS row pitch in bf
 //   TB = true : B stored [N][ldb] (B(k,n) at n*ldb+k);  false: [K][ldb] (k*ldb+n)
 // K must be a multiple of 32 within each split; M, N arbitrary.
 // EPI 0: raw fp32 to C (+ split offset);  1: pre = acc + bias -> C, and
-// bf16(silu(pre)) -> H;  2: acc + bias -> C (output layer).
+// bf16(silu(pre)) -> H;  2: acc + bias -> C (output layer);  3: as 1
+// without C (inference: the pre-activations are kept only for training).
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t *__restrict__ A,
                                                         const uint16_t *__restrict__ B,
@@ -147,8 +148,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t *__restri
         if (m >= M || n >= N) continue;
         float x = acc[a][b][v];
         if (EPI >= 1) x += bias[n];
-        Cs[(long)m * ldc + n] = x;
-        if (EPI == 1) {
+        if (EPI != 3) Cs[(long)m * ldc + n] = x;
+        if (EPI == 1 || EPI == 3) {
           const float s = x / (1.f + expf(-x));
           H[(long)m * ldc + n] = f2bf(s);
         }
@@ -378,6 +379,7 @@ hipError_t launch_gemm_bf16(hipStream_t st, int ta, int tb, int epi, const uint1
   if (!ta && tb) {
     if (epi == 0) return gemm_t<false, true, 0>(st, A, B, C, H, bias, M, N, K, lda, ldb, ldc, splits);
     if (epi == 1) return gemm_t<false, true, 1>(st, A, B, C, H, bias, M, N, K, lda, ldb, ldc, 1);
+    if (epi == 3) return gemm_t<false, true, 3>(st, A, B, C, H, bias, M, N, K, lda, ldb, ldc, 1);
     return gemm_t<false, true, 2>(st, A, B, C, H, bias, M, N, K, lda, ldb, ldc, 1);
   }
   if (!ta && !tb) return gemm_t<false, false, 0>(st, A, B, C, H, bias, M, N, K, lda, ldb, ldc, splits);

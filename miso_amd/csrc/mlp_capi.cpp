@@ -63,12 +63,15 @@ template <typename T> hipError_t alloc(misohip_mlp *m, T **p, size_t n) {
   return e;
 }
 
-int forward(misohip_ctx *ctx, misohip_mlp *m, const uint16_t *X, int batch, float *Y) {
+// keep_pre: store the hidden layers' pre-activations (the backward pass
+// reads them; inference does not need them)
+int forward(misohip_ctx *ctx, misohip_mlp *m, const uint16_t *X, int batch, float *Y,
+            bool keep_pre = true) {
   const uint16_t *inp = X;
   for (int l = 0; l < m->n_layers; ++l) {
     const bool last = l == m->n_layers - 1;
     float *C = last && Y ? Y : m->pre[l];
-    MISOHIP_TRY(launch_gemm_bf16(ctx->stream, 0, 1, last ? 2 : 1, inp, m->wb[l], C,
+    MISOHIP_TRY(launch_gemm_bf16(ctx->stream, 0, 1, last ? 2 : keep_pre ? 1 : 3, inp, m->wb[l], C,
                                  last ? nullptr : m->act[l], m->b[l], batch, m->out[l], m->in[l],
                                  m->in[l], m->in[l], m->out[l], 1));
     inp = m->act[l];
@@ -327,7 +330,7 @@ int misohip_mlp_qrad(misohip_ctx *ctx, misohip_mlp *m, const misohip_grid *g, co
   for (int c0 = 0; c0 < ncol; c0 += m->max_batch) {
     const int nb = ncol - c0 < m->max_batch ? ncol - c0 : m->max_batch;
     if (int rc = forward(ctx, m, static_cast<const uint16_t *>(X) + (long)c0 * m->n_in, nb,
-                         Y + (long)c0 * m->n_out))
+                         Y + (long)c0 * m->n_out, false))
       return rc;
   }
   MISOHIP_TRY(launch_column_output(ctx->stream, Y, shape_of(g), m->mu_out, m->sd_out, flags,

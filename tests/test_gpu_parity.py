@@ -375,6 +375,29 @@ def test_step_shock_tube(ctx, axis):
     assert_bitwise(out, ref, f"shock tube axis {axis}")
 
 
+@pytest.mark.parametrize("variant,zchunk", [(5, 16), (6, 7)])
+def test_run_fp32_turbulence_3d(ctx, variant, zchunk):
+    """FP32 3D decaying turbulence (the benchmark's initial-condition family)
+    through misohip_run on the FP32 z-march columns the tuner picks (8 rows
+    with each cell's products kept beside it, 12 rows), against the FP64
+    oracle: every field within 2e-5 relative L-infinity after 30 steps (the
+    benchmark's 64^3 x 50 steps measure 3.2e-6), every dt within 1e-5."""
+    from miso_amd.autotune import get_tiles, set_tiles
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
+    set_tiles(ctx, 1, 1, 64, variant, zchunk, 1, 64)
+    try:
+        out, ref, dts, ref_dts = _run_both(ctx, problems.turbulence, (48, 44, 40), 30,
+                                           dtype=np.float32, use_run=True)
+    finally:
+        set_tiles(ctx, *default)
+    err = rel_linf(out, ref)
+    print(f"FP32 3D variant {variant}: max rel linf {max(err.values()):.2e}", flush=True)
+    assert max(err.values()) < 2e-5, err
+    assert np.allclose(dts, ref_dts, rtol=1e-5), (dts, ref_dts)
+
+
 def test_step_fp32_orszag_tang(ctx):
     """FP32 storage with FP64 energy recovery against the FP64 oracle."""
     from miso_amd import problems
