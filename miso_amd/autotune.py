@@ -95,14 +95,15 @@ def _profile(ctx, fn, totals=False):
 
 
 def _timed_rounds(cands, time_one, rounds: int = 2):
-    """{candidate: least time over `rounds` passes}, every other pass in
-    reverse order, so that a drift of the device's clock over the passes
-    does not favour the candidates timed first."""
-    best = {}
+    """{candidate: mean time over `rounds` passes}, every other pass in
+    reverse order: a candidate timed k-th in one pass is timed k-th from
+    the end in the next, so a steady drift of the device's clock over the
+    passes adds the same to every candidate's mean."""
+    tot = {}
     for r in range(rounds):
         for c in (cands if r % 2 == 0 else cands[::-1]):
-            best[c] = min(best.get(c, float("inf")), time_one(c))
-    return best
+            tot[c] = tot.get(c, 0.0) + time_one(c)
+    return {c: t / rounds for c, t in tot.items()}
 
 
 def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNKS,

@@ -145,3 +145,26 @@ def test_fused_xy_sweeps_compute_the_same_bits(ctx, ybc, dtype):
         for n, (o, d) in enumerate(outs[1:], 1):
             assert d == outs[0][1], (sizes, n)
             assert torch.equal(o, outs[0][0]), (sizes, n, (o - outs[0][0]).abs().max().item())
+
+
+def test_timed_rounds_cancel_a_steady_clock_drift():
+    """Two passes, the second in reverse order, each candidate's mean: a
+    clock that slows every later timing by the same step adds the same to
+    every candidate, so the ranking is the candidates' own; one pass would
+    have ranked the first-timed candidate fastest."""
+    from miso_amd.autotune import _timed_rounds
+    order = []
+    own = {3: 1.03, 1: 1.01, 2: 1.02, 0: 1.00}  # candidate 0 is the fastest
+
+    def time_one(c):
+        order.append(c)
+        return own[c] + 0.05 * (len(order) - 1)   # 5 % slower per timing
+    cands = [3, 1, 2, 0]
+    got = _timed_rounds(cands, time_one)
+    assert order == cands + cands[::-1]
+    assert min(got, key=got.get) == 0
+    assert sorted(got, key=got.get) == [0, 1, 2, 3]
+    drift = {c: got[c] - own[c] for c in cands}
+    assert max(drift.values()) - min(drift.values()) < 1e-12
+    one_pass = {c: own[c] + 0.05 * n for n, c in enumerate(cands)}
+    assert min(one_pass, key=one_pass.get) == 3

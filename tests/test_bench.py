@@ -26,3 +26,34 @@ def test_rk_traffic_reads_the_traffic_table_only():
     # the traffic of an RK stage is tens of GB at 512^3, not a clock or a count
     assert 2e10 < got < 6e10
     assert bench.rk_traffic(96, "f64", 5) is None
+
+
+def test_trace_window_cuts_the_trace_at_the_markers(tmp_path, capsys):
+    """--trace-window: kernels strictly between marker FIRST and FIRST+1,
+    per step; the wall time runs from the first marker's end to the
+    second's start, whatever order the CSV lists the dispatches in."""
+    import csv
+    rows = [("void misohip::before(int)", 0, 50),
+            ("void at::native::vectorized_elementwise_kernel<4, at::native::FillFunctor<short>, "
+             "std::array<char*, 1ul> >(int)", 100, 110),
+            ("void misohip::rk(int)", 1_000_110, 3_000_110),
+            ("void misohip::walk(int)", 3_000_110, 4_000_110),
+            ("void misohip::rk(int)", 5_000_110, 7_000_110),
+            ("void at::native::vectorized_elementwise_kernel<4, at::native::FillFunctor<short>, "
+             "std::array<char*, 1ul> >(int)", 8_000_110, 8_000_120),
+            ("void misohip::after(int)", 9_000_000, 9_500_000)]
+    path = tmp_path / "trace.csv"
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["Kernel_Name", "Start_Timestamp", "End_Timestamp"])
+        for r in reversed(rows):
+            w.writerow(r)
+    bench.trace_window(str(path), 1, 2)
+    out = capsys.readouterr().out
+    lines = {ln.split("|")[1].strip(): ln for ln in out.splitlines() if ln.startswith("| `")}
+    assert set(lines) == {"`rk`", "`walk`"}
+    rk = [c.strip() for c in lines["`rk`"].split("|")]
+    assert (rk[2], rk[3]) == ("1.0", "2.000")        # 2 launches / 2 steps, 4 ms / 2 steps
+    walk = [c.strip() for c in lines["`walk`"].split("|")]
+    assert (walk[2], walk[3]) == ("0.5", "0.500")
+    assert "window: 4.000 ms per step wall, 2.500 ms per step of kernel time" in out

@@ -345,3 +345,26 @@ def test_learned_rt_meets_the_accuracy_bar_at_full_column_height(ctx):
     assert e["learned"].mean() < e["network"].mean()
     assert e["learned"].mean() < e["stale"].mean()
     sur.close()
+
+
+def test_slab_warm_start_takes_the_first_planes_of_every_ray():
+    """BottomSlabRT.warm_start copies, for every ray, the first planes of the
+    box's intensity field (k is the slowest index of a ray's field in both
+    of its layouts, x-fastest and y-fastest), and nothing else."""
+    import torch
+    from miso_amd.mhd import Grid
+    from miso_amd.rmhd import BottomSlabRT
+    from miso_amd.rt import RT
+    g = Grid(12, 10, 20, 2, bounds=((0.0, 1.0),) * 3, device="cpu")
+    full = RT(g, 24)
+    full.rint.copy_(torch.arange(full.rint.numel(), dtype=torch.float64).view_as(full.rint))
+    slab = BottomSlabRT(g, 3, 2)
+    kt = slab.grid.k_total
+    assert kt == 3 + 2 + 2 * g.k_margin
+    slab.warm_start(full)
+    per_ray = full.rint[0].numel()
+    n = slab.rt.rint[0].numel()
+    for r in range(24):
+        want = torch.arange(r * per_ray, r * per_ray + n, dtype=torch.float64)
+        assert torch.equal(slab.rt.rint[r].reshape(-1), want)
+        assert torch.equal(slab.rt.rint[r], full.rint[r, :kt])
