@@ -774,7 +774,7 @@ def rmhd_leg(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); the "
+        print(f"bench.py: the benchmark runs on exactly one GPU (asked for {args.gpus}); the "
               "multi-GPU sharding is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
     if args.trace_window:

@@ -19,7 +19,7 @@ bz ei ph, each (i_total, j_total, k_total) with k fastest (array3d.hpp:54-60).
 The RT file has no element-size header: an int32 ray count, the weights and
 the three direction cosines of every ray, then S, kappa and the intensities
 (num_rays, i_total, j_total, k_total), all in the solver's element type.
-This build runs one rank, so coords.csv holds rank 0 at (0, 0, 0).
+A run is one rank on one GPU, so coords.csv holds rank 0 at (0, 0, 0).
 """
 from __future__ import annotations
 
