@@ -85,6 +85,8 @@ def parse():
     ap.add_argument("--ens-seed", type=int, default=2026)
     ap.add_argument("--ens-graph-steps", type=int, default=10)
     ap.add_argument("--ens-replays", type=int, default=5)
+    ap.add_argument("--ens-chunk", type=int, default=0,
+                    help="walk length of every run's y sweep (default: tuned per run size)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the Orszag-Tang accuracy leg")
     ap.add_argument("--accuracy-only", action="store_true")
     ap.add_argument("--no-f32", action="store_true", help="skip the FP32 timing of the 3D step")
@@ -491,7 +493,10 @@ def ensemble_leg(args):
     ens = Ensemble(ctx, [(n, n, 1) for n in sizes], init, dtype=np.float32)
     setup_s = time.perf_counter() - t0
     tuned = None
-    if not args.no_tune:
+    if args.ens_chunk > 0:
+        ens.set_chunks([args.ens_chunk] * ens.n)
+        tuned = {"chunks": {"all": args.ens_chunk}}
+    elif not args.no_tune:
         from miso_amd.autotune import summary, tune_ensemble
         tuned = summary(tune_ensemble(
             ctx, ens, lambda s: Ensemble(ctx, [s, s], init, dtype=np.float32)))

@@ -478,29 +478,7 @@ __global__ __launch_bounds__(256) void av_walk_kernel(Fld<const R> qq, Fld<R> rs
 // interface flux at its own point once and takes its neighbours' from the
 // adjacent lanes.  It also stores the frozen cc for the y and z sweeps over
 // the interior plus a one-cell ring in y and z.
-// Whole-wave shifts by one lane as DPP moves (VALU, no LDS round trip):
-// wave_shr:1 (0x138) gives lane i the value of lane i-1, wave_shl:1 (0x130)
-// that of lane i+1; the end lanes, which no caller stores, get 0.
-template <int CTRL> __device__ __forceinline__ int dpp_move(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
-}
-template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
-  return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
-}
-template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)dpp_move<CTRL>((int)(unsigned)(b & 0xffffffffLL));
-  const unsigned hi = (unsigned)dpp_move<CTRL>((int)(unsigned)((unsigned long long)b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-template <typename R>
-__device__ __forceinline__ R lane_up(R v) {  // value of lane - 1
-  return dpp_move<0x138>(v);
-}
-template <typename R>
-__device__ __forceinline__ R lane_dn(R v) {  // value of lane + 1
-  return dpp_move<0x130>(v);
-}
+// Lane shifts: dpp_move, lane_up, lane_dn (device_common.hpp).
 
 template <typename R>
 __device__ __forceinline__ void av_xshfl_block(const Fld<const R> &qq, const Fld<R> &rslt,

@@ -63,7 +63,11 @@ Shape shape_of(const misohip_grid *g) {
 // block counts of one instance per phase, matching the kernels' decoding
 void phase_blocks(const Shape &sh, int chunk, int mode_y, int mode_z, long out[P_COUNT]) {
   const long ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
-  out[P_RK] = ((ni + 63) / 64) * ((nj + 3) / 4) * nk;
+  if (sh.is && sh.js && !sh.ks)  // x-y runs: y-walk waves, four per block
+    out[P_RK] = (((ni + RK_YWALK_OUT - 1) / RK_YWALK_OUT) *
+                     ((nj + RK_YWALK_ROWS - 1) / RK_YWALK_ROWS) + 3) / 4;
+  else
+    out[P_RK] = ((ni + 63) / 64) * ((nj + 3) / 4) * nk;
   const long nx = 2L * sh.mi * sh.Jt * sh.Kt, ny = 2L * sh.mj * ni * sh.Kt,
              nz = 2L * sh.mk * ni * nj;
   out[P_GHOST] = (nx + ny + nz + 255) / 256;

@@ -44,6 +44,30 @@ __device__ __forceinline__ float dcopysign(float x, float y) { return __builtin_
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Whole-wave shifts by one lane as DPP moves (VALU, no LDS round trip):
+// wave_shr:1 (0x138) gives lane i the value of lane i-1, wave_shl:1 (0x130)
+// that of lane i+1; the end lanes, which no caller stores, get 0.
+template <int CTRL> __device__ __forceinline__ int dpp_move(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(dpp_move<CTRL>(__float_as_int(v)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_move<CTRL>((int)(unsigned)(b & 0xffffffffLL));
+  const unsigned hi = (unsigned)dpp_move<CTRL>((int)(unsigned)((unsigned long long)b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename R>
+__device__ __forceinline__ R lane_up(R v) {  // value of lane - 1
+  return dpp_move<0x138>(v);
+}
+template <typename R>
+__device__ __forceinline__ R lane_dn(R v) {  // value of lane + 1
+  return dpp_move<0x130>(v);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (linear block L runs
 // on XCD L % 8, and each XCD takes its blocks in increasing L).  Mapping L to
 // the virtual block V below gives every XCD one contiguous range of V, i.e. a

@@ -175,6 +175,12 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ start, int n, 
   return lo;
 }
 
+// The batched RK stage of runs with x and y axes and no z axis walks y:
+// each wave owns RK_YWALK_OUT x outputs over RK_YWALK_ROWS rows, four waves
+// per block (batch_rk_ywalk_kernel; the batch's block counts use these).
+constexpr int RK_YWALK_OUT = 60;
+constexpr int RK_YWALK_ROWS = 16;
+
 template <typename R>
 hipError_t launch_batch_rk(hipStream_t, const BatchInst<R> *, const BatchDyn<R> *, const int *,
                            int, int, int, int, int, const RkScalars &, int);

@@ -815,6 +815,159 @@ template <typename R> __device__ __forceinline__ Fld<const R> as_const(const Fld
 // One RK stage of every instance of a batch in one launch: orgn = qq, argm =
 // field set in_set, rslt = out_set; the stage step, ch2 and damp are the
 // instance's own (device memory).  Blocks run 64 x 4 cells, x fastest.
+// ------------------------------------------------------------ y walk (2D)
+// Runs with x and y axes and no z axis: a wave owns 64 consecutive x cells
+// -- 60 outputs and two halo cells at either end -- and walks `rows` rows
+// up y from interior row y0.  A lane forms its own cell's x terms once and
+// takes its neighbours' by DPP lane shifts; the y stencil reads a register
+// window of five rows of fields, the next row loaded a step ahead.  Each
+// cell is read once per stage instead of once per stencil point, and the
+// wave needs no LDS and no barrier.  The arithmetic of a cell is
+// rk_block's, term for term and in the same order (the z sums of a run
+// without a z axis are rk_block's zeros), so the bits are the same.
+// (RK_YWALK_OUT outputs and RK_YWALK_ROWS rows per wave: launch.hpp)
+
+template <typename R>
+__device__ __forceinline__ void rk_ywalk_wave(const Fld<const R> &orgn, const Fld<const R> &argm,
+                                              const Fld<R> &rslt, const R *__restrict__ dxi,
+                                              const R *__restrict__ dyi, const Shape &sh,
+                                              const RkScalars &sc, const R dt, const R ch2,
+                                              const R damp, int strip, int y0, int rows) {
+  const int lane = threadIdx.x & 63;
+  const int xi = strip * RK_YWALK_OUT + lane - 2;  // interior x of the lane's cell
+  // lanes past the array read its last column: no output lane uses them
+  const int i = min(xi + sh.mi, sh.It - 1);
+  const bool out = lane >= 2 && lane < 62 && xi < sh.It - 2 * sh.mi;
+  const R gm1 = R(sc.gm) - R(1);
+  const R pii8 = Const<R>::pii8;
+  const R ddx = dxi[i];
+  const int j0 = y0 + sh.mj;  // array row of the first output row
+  R w[5][9];                  // rows j-2 .. j+2 of the lane's column
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    const int c = sh.idx(i, j0 - 2 + r, 0);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) w[r][m] = argm.q[m][c];
+  }
+  R nx[9] = {};  // row j+3
+  if (rows > 1) {
+    const int c = sh.idx(i, j0 + 3, 0);
+#pragma unroll
+    for (int m = 0; m < 9; ++m) nx[m] = argm.q[m][c];
+  }
+  for (int t = 0; t < rows; ++t) {
+    const int j = j0 + t;
+    const int c = sh.idx(i, j, 0);
+    R o[9];
+    if (out) {
+#pragma unroll
+      for (int m = 0; m < 9; ++m) o[m] = orgn.q[m][c];
+    }
+    R S[17];
+    // ---- x: the lane's own terms, its neighbours' by lane shifts
+    {
+      R tx[17];
+      point_terms<R, 0>(w[2], gm1, tx);
+#pragma unroll
+      for (int n = 0; n < 17; ++n) {
+        const R p1 = lane_dn(tx[n]), p2 = lane_dn(p1);
+        const R m1 = lane_up(tx[n]), m2 = lane_up(m1);
+        R a = -p2;
+        a = dfma(R(8), p1, a);
+        a = dfma(R(-8), m1, a);
+        S[n] = (a + m2) * Const<R>::inv12 * ddx;
+      }
+    }
+    R a_ro = -S[0];
+    R a_vx = -S[1], d4x = S[4], d5x = S[5] * pii8, t_vx = S[6];
+    R a_vy = -S[2], t_vy = S[7];
+    R a_vz = -S[3], t_vz = S[8];
+    R a_by = -S[9], k_by = S[10];
+    R a_bz = -S[11], k_bz = S[12];
+    R ph_bx = S[13];
+    R a_ph = -S[14];
+    R a_ei = -S[15], t_ei = S[16];
+    // ---- y: the register window
+    derivs_pts<R, 1>(w[4], w[3], w[1], w[0], dyi[j], gm1, S);
+    a_ro = a_ro - S[0];
+    a_vx = a_vx - S[1];
+    t_vx = t_vx + S[6];
+    a_vy = a_vy - S[2];
+    const R d4y = S[4], d5y = S[5] * pii8;
+    t_vy = t_vy + S[7];
+    a_vz = a_vz - S[3];
+    t_vz = t_vz + S[8];
+    R a_bx = -S[9];
+    const R k_bx = S[10];
+    const R ph_by = S[13];
+    a_bz = a_bz - S[11] + k_bz + S[12];
+    a_ph = a_ph - S[14];
+    a_ei = a_ei - S[15];
+    t_ei = t_ei + S[16];
+    // ---- z: no axis (rk_block's zero sums)
+#pragma unroll
+    for (int n = 0; n < 17; ++n) S[n] = R(0);
+    a_ro = a_ro - S[0];
+    a_vx = a_vx - S[1];
+    t_vx = t_vx + S[6];
+    a_vy = a_vy - S[2];
+    t_vy = t_vy + S[7];
+    a_vz = a_vz - S[3];
+    const R d4z = S[4], d5z = S[5] * pii8;
+    t_vz = t_vz + S[8];
+    a_bx = a_bx - S[9] + k_bx + S[10] - ph_bx;
+    a_by = a_by - S[11] + k_by + S[12] - ph_by;
+    a_bz = a_bz - S[13];
+    a_ph = a_ph - S[14];
+    a_ei = a_ei - S[15];
+    t_ei = t_ei + S[16];
+    if (out) {
+      const R gro = sc.src_kind == 1 ? w[2][0] : R(0);
+      rk_update<R>(o, rslt, c, gro, sc, dt, ch2, damp, a_ro, a_vx, d4x, d5x, t_vx, a_vy, d4y,
+                   d5y, t_vy, a_vz, d4z, d5z, t_vz, a_bx, a_by, a_bz, a_ph, a_ei, t_ei);
+    }
+    // the window moves up a row; row j+4 is loaded for the step after next
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+      w[0][m] = w[1][m];
+      w[1][m] = w[2][m];
+      w[2][m] = w[3][m];
+      w[3][m] = w[4][m];
+      w[4][m] = nx[m];
+    }
+    if (t + 2 < rows) {  // row j+4 is inside the array when j+2 is an output row
+      const int cn = sh.idx(i, j + 4, 0);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) nx[m] = argm.q[m][cn];
+    }
+  }
+}
+
+// The batch's x-y runs: four waves per block, each a (strip, row chunk) of
+// one run; a run has ceil(ni / RK_YWALK_OUT) strips and ceil(nj / RK_YWALK_ROWS) chunks.
+template <typename R>
+__global__ __launch_bounds__(256) void batch_rk_ywalk_kernel(const BatchInst<R> *__restrict__ inst,
+                                                             const BatchDyn<R> *__restrict__ dyn,
+                                                             const int *__restrict__ start, int n,
+                                                             int total, int stage, int in_set,
+                                                             int out_set, RkScalars sc) {
+  const int V = xcd_virtual_block(blockIdx.x, total);
+  const int m = batch_find(start, n, V);
+  const BatchInst<R> &I = inst[m];
+  const Shape sh = I.sh;
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
+  const int nstrip = (ni + RK_YWALK_OUT - 1) / RK_YWALK_OUT;
+  const int nch = (nj + RK_YWALK_ROWS - 1) / RK_YWALK_ROWS;
+  const int W = (V - start[m]) * 4 + (threadIdx.x >> 6);
+  if (W >= nstrip * nch) return;  // uniform over the wave
+  const int strip = W % nstrip, ch = W / nstrip;
+  const int y0 = ch * RK_YWALK_ROWS;
+  const BatchDyn<R> &D = dyn[m];
+  rk_ywalk_wave<R>(as_const(I.st[0]), as_const(I.st[in_set]), I.st[out_set], I.dxi, I.dyi, sh,
+                   sc, D.dts[stage], D.ch2, D.damp[stage], strip, y0,
+                   min(RK_YWALK_ROWS, nj - y0));
+}
+
 template <typename R, bool HX, bool HY, bool HZ>
 __global__ __launch_bounds__(256) void batch_rk_kernel(const BatchInst<R> *__restrict__ inst,
                                                        const BatchDyn<R> *__restrict__ dyn,
@@ -850,7 +1003,10 @@ hipError_t launch_batch_rk(hipStream_t st, const BatchInst<R> *inst, const Batch
                            int out_set, const RkScalars &sc, int key) {
   switch (key) {
     case 7: return launch_batch_rk_t<R, true, true, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
-    case 6: return launch_batch_rk_t<R, true, true, false>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
+    case 6:  // x-y runs: the y walk (its own block count, batch_rk_blocks)
+      hipLaunchKernelGGL((batch_rk_ywalk_kernel<R>), dim3(total), dim3(256), 0, st, inst, dyn,
+                         start, n, total, stage, in_set, out_set, sc);
+      return hipGetLastError();
     case 5: return launch_batch_rk_t<R, true, false, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
     case 3: return launch_batch_rk_t<R, false, true, true>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);
     case 4: return launch_batch_rk_t<R, true, false, false>(st, inst, dyn, start, n, total, stage, in_set, out_set, sc);

@@ -6,7 +6,10 @@ import pytest
 
 from miso_amd.ensemble import kh_sizes
 
-SHAPES = [(32, 32, 1), (48, 40, 1), (64, 64, 1), (40, 72, 1), (96, 32, 1)]
+# x extents off and on the y walk's 60-output strips (61: a strip with one
+# output), y extents off and on its 16-row chunks (33, 65: a one-row chunk)
+SHAPES = [(32, 32, 1), (48, 40, 1), (64, 64, 1), (40, 72, 1), (96, 32, 1), (61, 33, 1),
+          (120, 65, 1)]
 
 
 def test_kh_sizes_are_seeded_draws():
