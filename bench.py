@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--rmhd-slab-iters", type=int, default=2,
                     help="iterations of a bottom-slab solve (at most; the slab starts from its "
                          "last solve, or from the last full solve after one)")
+    ap.add_argument("--rmhd-solve-iters", type=int, default=None,
+                    help="most iterations of the learned run's plain solve every rt_every "
+                         "steps (default: the RT limit)")
     ap.add_argument("--rmhd-slab-share", type=float, default=0.99,
                     help="share of the network's residual at a solve whose bottom layers are "
                          "solved plainly on every step (<= 0: no slab)")
@@ -703,7 +706,8 @@ def rmhd_leg(args):
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
                          correct=not args.rmhd_no_correct, slab_share=slab_share,
                          slab_tolerance=args.rmhd_slab_tol,
-                         slab_max_iters=args.rmhd_slab_iters)
+                         slab_max_iters=args.rmhd_slab_iters,
+                         solve_max_iters=args.rmhd_solve_iters)
     for _ in range(every):
         box.step_learned()
     if args.trace_markers:
@@ -732,7 +736,8 @@ def rmhd_leg(args):
     box.attach_surrogate(sur, rt_every=every, flag_tol=args.rmhd_flag_tol,
                          correct=not args.rmhd_no_correct, slab_share=slab_share,
                          slab_tolerance=args.rmhd_slab_tol,
-                         slab_max_iters=args.rmhd_slab_iters)
+                         slab_max_iters=args.rmhd_slab_iters,
+                         solve_max_iters=args.rmhd_solve_iters)
     errs = {"learned": [], "network_only": [], "network_corrected": [], "stale_rt": []}
     by_age = {}
     tmp = torch.empty_like(box.q_rt)

@@ -223,11 +223,12 @@ class RadiativeMHD:
         self._next_dt = None
 
     # -- radiative transfer -------------------------------------------------
-    def solve_rt(self):
+    def solve_rt(self, **kw):
         """Plain RT: opacity and source from the fields, a (warm-started)
-        solve, Q_rad into self.rt.qrad_field."""
+        solve (the run's RT settings, overridden by kw), Q_rad into
+        self.rt.qrad_field."""
         self.rt.source_from_fields(self.ctx, self.model.qq, self.strat.gm, self.kappa0, self.s_rad)
-        it, md = self.rt.solve(self.ctx, **self.rt_kw)
+        it, md = self.rt.solve(self.ctx, **dict(self.rt_kw, **kw))
         self.rt_iters.append(it)
         self.rt.qrad(self.ctx, **self.rt_kw)
         return self.rt.qrad_field
@@ -246,7 +247,8 @@ class RadiativeMHD:
                          correct: bool = True, slab_share: float | None = 0.99,
                          slab_tau: float = 10.0, slab_max_frac: float = 0.25,
                          slab_tolerance: float | None = None,
-                         slab_max_iters: int | None = None) -> None:
+                         slab_max_iters: int | None = None,
+                         solve_max_iters: int | None = None) -> None:
         """Learned RT: the network every step, a plain solve every rt_every
         steps.  correct: add the column correction the last solve left (the
         solve minus the network on the same state), so the network carries
@@ -258,7 +260,8 @@ class RadiativeMHD:
         plain slab solve on every step (to slab_tolerance, default the
         run's RT tolerance, in at most slab_max_iters iterations, default
         the run's limit), if they fit in slab_max_frac of the height (None:
-        no slab)."""
+        no slab); solve_max_iters caps the iterations of the plain solve
+        every rt_every steps (default: the run's RT limit)."""
         import torch
         self.surrogate = surrogate
         self.rt_every = int(rt_every)
@@ -270,6 +273,7 @@ class RadiativeMHD:
             self.slab_kw["tolerance"] = float(slab_tolerance)
         if slab_max_iters is not None:
             self.slab_kw["max_iters"] = int(slab_max_iters)
+        self.solve_kw = {} if solve_max_iters is None else {"max_iters": int(solve_max_iters)}
         self.slab = None
         ncol = self.grid.i_size * self.grid.j_size
         self.flags = torch.zeros(ncol, dtype=torch.int32, device=self.grid.device)
@@ -310,7 +314,7 @@ class RadiativeMHD:
         steps).  probe(q, solved) is called with the Q_rad about to be used,
         before the state advances."""
         if self.steps_done % self.rt_every == 0:
-            q = self.solve_rt()
+            q = self.solve_rt(**self.solve_kw)
             self.q_rt.copy_(q)
             # network on the same state -> correction and residual flags
             self.surrogate.predict(self.grid, self.model.qq, self.strat.gm, self.q_mlp,
