@@ -134,11 +134,15 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
 
     run()  # warm-up (code objects, caches)
     rk_cands = []
-    # 6 (the 12-row z-march) is an FP32 kernel; FP64 would run it as 5
+    # 6 (the 12-row z-march) is an FP32 kernel; FP64 would run it as 5; on
+    # an x-y grid the z-march variants walk y instead (one candidate)
+    xy = g.i_size > 1 and g.j_size > 1 and g.k_size == 1
     variants = ((0, 1, 2, 3, 4, 5) + ((6,) if g.dtype == np.float32 else ())
-                if min(g.i_size, g.j_size, g.k_size) > 1 else (0,))
+                if min(g.i_size, g.j_size, g.k_size) > 1 else (0, 4) if xy else (0,))
     for var in variants:
-        if var in (4, 5, 6):  # the z-march kernels have no pencils; their knob is the z chunk
+        if var in (4, 5, 6) and xy:
+            cands = [(1, 1, 32)]
+        elif var in (4, 5, 6):  # the z-march kernels have no pencils; their knob is the z chunk
             cands = [(1, 1, z) for z in zchunks]
         elif var in (0, 3):
             cands = [(bx, by, 32) for bx, by in pencils]

@@ -180,6 +180,9 @@ __device__ __forceinline__ int batch_find(const int *__restrict__ start, int n, 
 // per block (batch_rk_ywalk_kernel; the batch's block counts use these).
 constexpr int RK_YWALK_OUT = 60;
 constexpr int RK_YWALK_ROWS = 16;
+// a single x-y run walks y only when that makes at least this many waves
+// (eight per CU); smaller runs keep the one-plane kernel
+constexpr int RK_YWALK_MIN_WAVES = 2048;
 
 template <typename R>
 hipError_t launch_batch_rk(hipStream_t, const BatchInst<R> *, const BatchDyn<R> *, const int *,

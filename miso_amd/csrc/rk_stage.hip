@@ -730,91 +730,6 @@ rk_stage_kernel_march(Fld<const R> orgn, Fld<const R> argm, Fld<R> rslt,
   }
 }
 
-template <typename R, bool HX, bool HY, bool HZ>
-static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
-                              const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
-                              const Shape &sh, const RkScalars &sc) {
-  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
-  if (HX && HY && HZ && (sc.variant >= 4 && sc.variant <= 6)) {
-    // variant 6: the 12-row column at three waves per SIMD, for FP32 (an
-    // FP64 stage needs more than the 170 registers that allows: it runs the
-    // 8-row column)
-    const int ty = sc.variant == 4 ? 4 : (sc.variant == 6 && sizeof(R) == 4) ? 12 : 8;
-    March mc;
-    if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 ||
-        !mc.init((ni + 63) / 64, (nj + ty - 1) / ty, nk, sc.zchunk))
-      return hipErrorInvalidValue;
-    bool same = true;
-    for (int m = 0; m < 9; ++m) same = same && o.q[m] == a.q[m];
-    if (ty == 4 && same)
-      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, true>), dim3(mc.nb), dim3(256), 0, st, o,
-                         a, r, dxi, dyi, dzi, sh, sc, mc);
-    else if (ty == 4)
-      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, false>), dim3(mc.nb), dim3(256), 0, st, o,
-                         a, r, dxi, dyi, dzi, sh, sc, mc);
-    else if (ty == 12) {
-      if constexpr (sizeof(R) == 4) {
-        if (same)
-          hipLaunchKernelGGL((rk_stage_kernel_march<R, 12, true>), dim3(mc.nb), dim3(768), 0, st,
-                             o, a, r, dxi, dyi, dzi, sh, sc, mc);
-        else
-          hipLaunchKernelGGL((rk_stage_kernel_march<R, 12, false>), dim3(mc.nb), dim3(768), 0, st,
-                             o, a, r, dxi, dyi, dzi, sh, sc, mc);
-      }
-    } else if (same)
-      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, true>), dim3(mc.nb), dim3(512), 0, st, o,
-                         a, r, dxi, dyi, dzi, sh, sc, mc);
-    else
-      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, false>), dim3(mc.nb), dim3(512), 0, st, o,
-                         a, r, dxi, dyi, dzi, sh, sc, mc);
-    return hipGetLastError();
-  }
-  Pencils pc;
-  if (!pc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.pencil_bx, sc.pencil_by))
-    return hipErrorInvalidValue;
-  if (HX && HY && HZ && sc.variant == 1)
-    hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 1>), dim3(pc.nb), dim3(256), 0, st, o, a,
-                       r, dxi, dyi, dzi, sh, sc, pc);
-  else if (HX && HY && HZ && sc.variant == 2)
-    hipLaunchKernelGGL((rk_stage_kernel_w4<R, HX, HY, HZ>), dim3(pc.nb), dim3(256), 0, st, o, a,
-                       r, dxi, dyi, dzi, sh, sc, pc);
-  else if (HX && HY && HZ && sc.variant == 3)
-    hipLaunchKernelGGL((rk_stage_kernel_tile<R>), dim3(pc.nb), dim3(256), 0, st, o, a, r, dxi,
-                       dyi, dzi, sh, sc, pc);
-  else
-    hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 0>), dim3(pc.nb), dim3(256), 0, st, o, a,
-                       r, dxi, dyi, dzi, sh, sc, pc);
-  return hipGetLastError();
-}
-
-template <typename R>
-hipError_t launch_rk_stage(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
-                           const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
-                           const Shape &sh, const RkScalars &sc) {
-  const int key = (sh.is ? 4 : 0) | (sh.js ? 2 : 0) | (sh.ks ? 1 : 0);
-  switch (key) {
-    case 7: return launch_rk_t<R, true, true, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    case 6: return launch_rk_t<R, true, true, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    case 5: return launch_rk_t<R, true, false, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    case 3: return launch_rk_t<R, false, true, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    case 4: return launch_rk_t<R, true, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    case 2: return launch_rk_t<R, false, true, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    case 1: return launch_rk_t<R, false, false, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-    default: return launch_rk_t<R, false, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
-  }
-}
-
-// ---------------------------------------------------------------- batches
-template <typename R> __device__ __forceinline__ Fld<const R> as_const(const Fld<R> &f) {
-  Fld<const R> o;
-#pragma unroll
-  for (int m = 0; m < 9; ++m) o.q[m] = f.q[m];
-  return o;
-}
-
-// One RK stage of every instance of a batch in one launch: orgn = qq, argm =
-// field set in_set, rslt = out_set; the stage step, ch2 and damp are the
-// instance's own (device memory).  Blocks run 64 x 4 cells, x fastest.
 // ------------------------------------------------------------ y walk (2D)
 // Runs with x and y axes and no z axis: a wave owns 64 consecutive x cells
 // -- 60 outputs and two halo cells at either end -- and walks `rows` rows
@@ -943,6 +858,122 @@ __device__ __forceinline__ void rk_ywalk_wave(const Fld<const R> &orgn, const Fl
   }
 }
 
+// One x-y run (the z-march variants' choice on a grid without a z axis):
+// four waves per block, each a (strip, row chunk) of the run.
+template <typename R>
+__global__ __launch_bounds__(256) void rk_stage_kernel_ywalk(Fld<const R> orgn, Fld<const R> argm,
+                                                             Fld<R> rslt,
+                                                             const R *__restrict__ dxi,
+                                                             const R *__restrict__ dyi, Shape sh,
+                                                             RkScalars sc, int nb) {
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj;
+  const int nstrip = (ni + RK_YWALK_OUT - 1) / RK_YWALK_OUT;
+  const int nch = (nj + RK_YWALK_ROWS - 1) / RK_YWALK_ROWS;
+  const int W = xcd_virtual_block(blockIdx.x, nb) * 4 + (threadIdx.x >> 6);
+  if (W >= nstrip * nch) return;  // uniform over the wave
+  const int strip = W % nstrip, ch = W / nstrip;
+  const int y0 = ch * RK_YWALK_ROWS;
+  rk_ywalk_wave<R>(orgn, argm, rslt, dxi, dyi, sh, sc, R(sc.dt), R(sc.ch2), R(sc.damp), strip,
+                   y0, min(RK_YWALK_ROWS, nj - y0));
+}
+
+template <typename R, bool HX, bool HY, bool HZ>
+static hipError_t launch_rk_t(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
+                              const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
+                              const Shape &sh, const RkScalars &sc) {
+  const int ni = sh.It - 2 * sh.mi, nj = sh.Jt - 2 * sh.mj, nk = sh.Kt - 2 * sh.mk;
+  if (HX && HY && HZ && (sc.variant >= 4 && sc.variant <= 6)) {
+    // variant 6: the 12-row column at three waves per SIMD, for FP32 (an
+    // FP64 stage needs more than the 170 registers that allows: it runs the
+    // 8-row column)
+    const int ty = sc.variant == 4 ? 4 : (sc.variant == 6 && sizeof(R) == 4) ? 12 : 8;
+    March mc;
+    if (sh.mi < 2 || sh.mj < 2 || sh.mk < 2 ||
+        !mc.init((ni + 63) / 64, (nj + ty - 1) / ty, nk, sc.zchunk))
+      return hipErrorInvalidValue;
+    bool same = true;
+    for (int m = 0; m < 9; ++m) same = same && o.q[m] == a.q[m];
+    if (ty == 4 && same)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, true>), dim3(mc.nb), dim3(256), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
+    else if (ty == 4)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 4, false>), dim3(mc.nb), dim3(256), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
+    else if (ty == 12) {
+      if constexpr (sizeof(R) == 4) {
+        if (same)
+          hipLaunchKernelGGL((rk_stage_kernel_march<R, 12, true>), dim3(mc.nb), dim3(768), 0, st,
+                             o, a, r, dxi, dyi, dzi, sh, sc, mc);
+        else
+          hipLaunchKernelGGL((rk_stage_kernel_march<R, 12, false>), dim3(mc.nb), dim3(768), 0, st,
+                             o, a, r, dxi, dyi, dzi, sh, sc, mc);
+      }
+    } else if (same)
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, true>), dim3(mc.nb), dim3(512), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
+    else
+      hipLaunchKernelGGL((rk_stage_kernel_march<R, 8, false>), dim3(mc.nb), dim3(512), 0, st, o,
+                         a, r, dxi, dyi, dzi, sh, sc, mc);
+    return hipGetLastError();
+  }
+  const long yw_waves = (long)((ni + RK_YWALK_OUT - 1) / RK_YWALK_OUT) *
+                        ((nj + RK_YWALK_ROWS - 1) / RK_YWALK_ROWS);
+  if (HX && HY && !HZ && sc.variant >= 4 && sc.variant <= 6 && yw_waves >= RK_YWALK_MIN_WAVES) {
+    // an x-y run large enough to fill the chip with y-walk waves (a smaller
+    // one runs the one-plane kernel below: 256^2 is 80 waves)
+    if (sh.mi < 2 || sh.mj < 2) return hipErrorInvalidValue;
+    const long nb = (yw_waves + 3) / 4;
+    if (nb > 2147483647L) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((rk_stage_kernel_ywalk<R>), dim3((unsigned)nb), dim3(256), 0, st, o, a, r,
+                       dxi, dyi, sh, sc, (int)nb);
+    return hipGetLastError();
+  }
+  Pencils pc;
+  if (!pc.init((ni + 63) / 64, (nj + 3) / 4, nk, sc.pencil_bx, sc.pencil_by))
+    return hipErrorInvalidValue;
+  if (HX && HY && HZ && sc.variant == 1)
+    hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 1>), dim3(pc.nb), dim3(256), 0, st, o, a,
+                       r, dxi, dyi, dzi, sh, sc, pc);
+  else if (HX && HY && HZ && sc.variant == 2)
+    hipLaunchKernelGGL((rk_stage_kernel_w4<R, HX, HY, HZ>), dim3(pc.nb), dim3(256), 0, st, o, a,
+                       r, dxi, dyi, dzi, sh, sc, pc);
+  else if (HX && HY && HZ && sc.variant == 3)
+    hipLaunchKernelGGL((rk_stage_kernel_tile<R>), dim3(pc.nb), dim3(256), 0, st, o, a, r, dxi,
+                       dyi, dzi, sh, sc, pc);
+  else
+    hipLaunchKernelGGL((rk_stage_kernel<R, HX, HY, HZ, 0>), dim3(pc.nb), dim3(256), 0, st, o, a,
+                       r, dxi, dyi, dzi, sh, sc, pc);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_rk_stage(hipStream_t st, const Fld<const R> &o, const Fld<const R> &a,
+                           const Fld<R> &r, const R *dxi, const R *dyi, const R *dzi,
+                           const Shape &sh, const RkScalars &sc) {
+  const int key = (sh.is ? 4 : 0) | (sh.js ? 2 : 0) | (sh.ks ? 1 : 0);
+  switch (key) {
+    case 7: return launch_rk_t<R, true, true, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 6: return launch_rk_t<R, true, true, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 5: return launch_rk_t<R, true, false, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 3: return launch_rk_t<R, false, true, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 4: return launch_rk_t<R, true, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 2: return launch_rk_t<R, false, true, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    case 1: return launch_rk_t<R, false, false, true>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+    default: return launch_rk_t<R, false, false, false>(st, o, a, r, dxi, dyi, dzi, sh, sc);
+  }
+}
+
+// ---------------------------------------------------------------- batches
+template <typename R> __device__ __forceinline__ Fld<const R> as_const(const Fld<R> &f) {
+  Fld<const R> o;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) o.q[m] = f.q[m];
+  return o;
+}
+
+// One RK stage of every instance of a batch in one launch: orgn = qq, argm =
+// field set in_set, rslt = out_set; the stage step, ch2 and damp are the
+// instance's own (device memory).  Blocks run 64 x 4 cells, x fastest.
 // The batch's x-y runs: four waves per block, each a (strip, row chunk) of
 // one run; a run has ceil(ni / RK_YWALK_OUT) strips and ceil(nj / RK_YWALK_ROWS) chunks.
 template <typename R>

@@ -72,7 +72,9 @@ def test_tune_ensemble_keeps_results(ctx, tmp_path):
 def test_rk_variants_and_pencils_compute_the_same_bits(ctx, dtype):
     """Every RK variant, pencil and z chunk gives the same bits, on grids
     with partial x and y tiles (for the 4-, 8- and 12-row columns) and
-    uneven z chunks; variant 6 is the FP32 12-row column (FP64 runs it as 5)."""
+    uneven z chunks; variant 6 is the FP32 12-row column (FP64 runs it as 5).
+    On the x-y grid (z of extent one, large enough for the single-run walk)
+    variants 4-6 walk y (60-output strips, 16-row chunks, both cut short)."""
     import numpy as np
     import torch
     from miso_amd import mhd, problems
@@ -81,7 +83,9 @@ def test_rk_variants_and_pencils_compute_the_same_bits(ctx, dtype):
     default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
                t0.av_fuse_xy, t0.av_xy_chunk)
     dt = np.float64 if dtype == "f64" else np.float32
-    for sizes in ((70, 36, 20), (66, 37, 13), (130, 25, 9)):
+    # 1536 x 1313 x 1: 26 strips (the last cut short) x 83 row chunks (the
+    # last one row) = 2158 waves, over the single run's y-walk threshold
+    for sizes in ((70, 36, 20), (66, 37, 13), (130, 25, 9), (1536, 1313, 1)):
         g = mhd.Grid(*sizes, 2, dtype=dt, device="cuda:0")
         bc = mhd.BoundaryCondition((True, True, True))
         m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
