@@ -256,6 +256,10 @@ int misohip_update_heated(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype
  * CFL reduction is carried out by the previous step's last AV sweep on the
  * values it stores (3D runs and 2D x-y runs; others reduce separately), so
  * dts[] (optional, nsteps values) equal those of nsteps misohip_step calls.
+ * Where the step reads periodic ghosts at their sources (3D, x and y AV
+ * sweeps as one walk, a z-march RK variant, no user boundary), the periodic
+ * faces are filled after the last step only; the fields returned are the
+ * same bits as those of nsteps misohip_step calls.
  * The caller must not modify qq between the steps -- use misohip_step for
  * that (e.g. to add a source between steps). */
 int misohip_run(misohip_ctx *ctx, const misohip_grid *g, misohip_dtype dt,

@@ -204,7 +204,8 @@ int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const Bc
 template <typename R>
 int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
              const misohip_bc *bc, const misohip_fields *qq, const misohip_work *w, R dt,
-             bool fuse_cfl = false, const double *heat_q = nullptr) {
+             bool fuse_cfl = false, const double *heat_q = nullptr,
+             bool fill_periodic = true) {
   const R cfl_number = (R)p->cfl_number;
   const R min_dxyz = (R)g->min_dxyz;
   // divb_parameters_set (337-341)
@@ -295,8 +296,11 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
       rc = av_sweep_t<R>(ctx, g, p, dir, dt, cur, dst, w->cc, fuse_xy ? wrap : 0);
     }
     if (rc) return rc;
-    // after the x-y walk only the z walk reads the state, along z alone
-    const int axes = last ? 7 : (dir == 0 && fuse_xy && wrap) ? 4 & ~wrap : 7;
+    // after the x-y walk only the z walk reads the state, along z alone; the
+    // next step of a run reads the periodic faces of this step's result at
+    // their sources, so a run fills them after its last step only
+    const int axes = last ? (fill_periodic ? 7 : 7 & ~wrap)
+                          : (dir == 0 && fuse_xy && wrap) ? 4 & ~wrap : 7;
     rc = ghost_t<R>(ctx, dst, sh, b, axes);
     if (rc) return rc;
     if (dir == 1 && fuse_z) break;
@@ -329,7 +333,8 @@ int run_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p, cons
       d = (double)(R)f;
     }
     if (dts) dts[s] = d;
-    int rc = update_t<R>(ctx, g, p, bc, qq, w, (R)d, can_fuse && s + 1 < nsteps);
+    int rc = update_t<R>(ctx, g, p, bc, qq, w, (R)d, can_fuse && s + 1 < nsteps, nullptr,
+                         s + 1 == nsteps);
     if (rc) return rc;
   }
   return MISOHIP_OK;

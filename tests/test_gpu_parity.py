@@ -219,11 +219,12 @@ def test_cfl(ctx, geom):
 
 def _run_both(ctx, problem, sizes, nsteps, dtype=np.float64, periodic=(1, 1, 1), sym=None,
               cfl=0.4, av=None, gm=5.0 / 3.0, bounds=((0.0, 1.0),) * 3, src=None,
-              use_run=False):
+              use_run=False, ghosts=False):
     """Oracle and GPU runs of the same initial state; a fixed axis (2) takes
     its ghost values from the initial state.  use_run: the GPU side advances
     through one misohip_run call (CFL of later steps fused into the last
-    sweep) instead of nsteps misohip_step calls."""
+    sweep) instead of nsteps misohip_step calls.  ghosts: return the whole
+    arrays, ghost cells included, instead of the interiors."""
     from miso_amd import mhd, problems
     from oracle import oracle as O
     src = src or {}
@@ -250,7 +251,7 @@ def _run_both(ctx, problem, sizes, nsteps, dtype=np.float64, periodic=(1, 1, 1),
     else:
         dts = [m.integrator.step(bc, m.qq) for _ in range(nsteps)]
     out = m.qq.to_host(ctx).astype(np.float64)
-    sl = (slice(None),) + g.interior()
+    sl = (slice(None),) + (() if ghosts else g.interior())
     return out[sl], ref[sl], np.array(dts), ref_dts
 
 
@@ -332,6 +333,28 @@ def test_step_fp64_fused_xy_sweeps(ctx, case, fuse):
         set_tiles(ctx, *default)
     assert np.array_equal(dts, ref_dts), (dts, ref_dts)
     assert_bitwise(out, ref, f"fused x-y sweeps {case}")
+
+
+@pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
+                                  "stratified_gravity_16"])
+def test_run_fills_every_ghost_after_its_last_step(ctx, case):
+    """With periodic ghosts read at their sources (z-march RK, one-walk x-y
+    sweeps) misohip_run fills the periodic faces after its last step only:
+    the whole arrays, ghost cells included, equal those of the same number
+    of misohip_step calls, bit for bit."""
+    from miso_amd.autotune import get_tiles, set_tiles
+    problem, sizes, n, kw = STEP_CASES[case]
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
+    set_tiles(ctx, 1, 1, 16, 5, 32, 1, 16)
+    try:
+        run, _, run_dts, _ = _run_both(ctx, problem, sizes, n, use_run=True, ghosts=True, **kw)
+        steps, _, step_dts, _ = _run_both(ctx, problem, sizes, n, ghosts=True, **kw)
+    finally:
+        set_tiles(ctx, *default)
+    assert np.array_equal(run_dts, step_dts), (run_dts, step_dts)
+    assert_bitwise(run, steps, f"run vs steps, ghosts included, {case}")
 
 
 @pytest.mark.parametrize("sizes", [(96, 1, 1), (1, 80, 1), (1, 1, 72), (40, 1, 24)])
