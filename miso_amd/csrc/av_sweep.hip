@@ -322,7 +322,8 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
                                               const R *__restrict__ dd_arr, const Shape &sh,
                                               int chunk, int nchunks, int nbx,
                                               const AvScalars &sc, const R dt, int V,
-                                              R dz0 = R(0), const CflFuse<R> *cf = nullptr) {
+                                              R dz0 = R(0), const CflFuse<R> *cf = nullptr,
+                                              const R *__restrict__ hw = nullptr) {
   const int ch = V % nchunks, r = V / nchunks;
   const int bx = r % nbx, bo = r / nbx;
   const int wx = walk_waves_x(DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj);
@@ -379,32 +380,53 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
   a1 = a2;
   // the fields and speed of row p+2 arrive two steps ahead of their use
   // (row p+3 is in flight meanwhile): two waves per SIMD leave the registers
-  // for it, and one step of arithmetic does not cover a load from HBM
-  R nf[9], ncc, nf2[9] = {}, ncc2 = R(0);
+  // for it, and one step of arithmetic does not cover a load from HBM.  The
+  // two rows in flight sit in two register sets that the loop, unrolled by
+  // two, uses in turn: a copy from one set to the other at the end of a
+  // step would wait for the load issued in that step.
+  R nb0[9], nb1[9] = {}, cb0, cb1 = R(0);
   {
     const int c2 = fc(2);
 #pragma unroll
-    for (int m = 0; m < 9; ++m) nf[m] = qq.q[m][c2];
+    for (int m = 0; m < 9; ++m) nb0[m] = qq.q[m][c2];
   }
-  ncc = cc[c + 2 * s];
+  cb0 = cc[c + 2 * s];
   if (len > 1) {  // row p0+3 is inside the array when p0+1 is an output cell
     const int c3 = fc(3);
 #pragma unroll
-    for (int m = 0; m < 9; ++m) nf2[m] = qq.q[m][c3];
-    ncc2 = cc[c + 3 * s];
+    for (int m = 0; m < 9; ++m) nb1[m] = qq.q[m][c3];
+    cb1 = cc[c + 3 * s];
   }
-  for (int t = 0; t < len; ++t) {
+  // the cell widths of the CFL candidates along the two other axes (those
+  // along DIR, hw, are read through a kernel argument the compiler knows no
+  // store aliases: a scalar load, which the waits for the rows in flight
+  // do not count)
+  R hx = R(0), hy = R(0), hz = R(0);
+  if (CFLF) {
+    hx = cf->dx[i];
+    if (DIR == 2) hy = cf->dy[j];
+    if (DIR == 1) hz = cf->dz[k];
+  }
+  // one output cell: nb, cb hold row p+2, and take row p+4 afterwards
+  auto step = [&](R (&nb)[9], R &cb, int t) {
     AvCons<R> an;
-    cons_of(nf, an);
-    const R cc_2 = ncc;
+    {
+      R fr[9];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) nf[m] = nf2[m];
-    ncc = ncc2;
-    if (t + 2 < len) {  // row p+4 is inside the array when p+2 is an output cell
-      const int c4 = fc(4);
+      for (int m = 0; m < 9; ++m) fr[m] = reg_copy(nb[m]);
+      cons_of(fr, an);
+    }
+    const R cc_2 = reg_copy(cb);
+    {
+      // row p+4 is inside the array when p+2 is an output cell; otherwise
+      // the load takes row p+2 again (unused), so that every step issues the
+      // same loads: a load under a branch makes the wait for the rows in
+      // flight assume it was not issued, and wait for younger loads
+      const int q = t + 2 < len ? 4 : 2;
+      const int c4 = fc(q);
 #pragma unroll
-      for (int m = 0; m < 9; ++m) nf2[m] = qq.q[m][c4];
-      ncc2 = cc[c + 4 * s];
+      for (int m = 0; m < 9; ++m) nb[m] = qq.q[m][c4];
+      cb = cc[c + q * s];
     }
     const R cup = R(0.5) * (cc_0 + cc_1);
     R fup[9];
@@ -414,7 +436,7 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
       fup[f] = flux_core(a0.v[f], a1.v[f], d0[f], d1, cup, fh);
       d0[f] = d1;
     }
-    const R dd = dd_arr[m_dir + p0 + t];
+    const R dd = dd_arr[__builtin_amdgcn_readfirstlane(m_dir + p0 + t)];
     if (CFLF) {
       R st[9];
       av_cell_update<R, DEGEN>(a0, fdw, fup, dd, dt, rslt, c, cc_0, ep, fh, dz0, st);
@@ -423,8 +445,8 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
         st[7] = R(e);
         rslt.q[7][c] = st[7];
       }
-      const int jc = DIR == 1 ? m_dir + p0 + t : j, kc = DIR == 2 ? m_dir + p0 + t : k;
-      const R hmin = nmin3(cf->dx[i], cf->dy[jc], cf->dz[kc]);
+      const int pw = __builtin_amdgcn_readfirstlane(m_dir + p0 + t);
+      const R hmin = nmin3(hx, DIR == 1 ? hw[pw] : hy, DIR == 2 ? hw[pw] : hz);
       const float cand = cfl_candidate_v(st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7],
                                          R(cf->gm), R(cf->cfl), hmin);
       best = min(best, __float_as_uint(cand));
@@ -439,7 +461,18 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
     cc_1 = cc_2;
     c += s;
     ++pd;
+  };
+  // the loop is entered with no load in flight: the waits the compiler
+  // places inside it hold for every way in, and with rows p0+2 and p0+3
+  // still in flight on entry they would wait at every step for the row
+  // loaded one step before
+  __builtin_amdgcn_s_waitcnt(0);
+  int t = 0;
+  for (; t + 1 < len; t += 2) {
+    step(nb0, cb0, t);
+    step(nb1, cb1, t + 1);
   }
+  if (t < len) step(nb0, cb0, t);
   }
   if (CFLF) {
     best = wave_umin(best);
@@ -454,10 +487,11 @@ __global__ __launch_bounds__(256) void av_walk_cfl_kernel(Fld<const R> qq, Fld<R
                                                          int chunk, int nchunks, int nbx, int nb,
                                                          AvScalars sc,
                                                          const R *__restrict__ dz_arr,
-                                                         CflFuse<R> cf) {
+                                                         CflFuse<R> cf,
+                                                         const R *__restrict__ hw) {
   av_walk_block<R, DIR, DEGEN, true>(qq, rslt, cc, dd_arr, sh, chunk, nchunks, nbx, sc,
                                      R(sc.dt), xcd_virtual_block(blockIdx.x, nb),
-                                     DEGEN ? dz_arr[0] : R(0), &cf);
+                                     DEGEN ? dz_arr[0] : R(0), &cf, hw);
 }
 
 template <typename R, int DIR, bool DEGEN>
@@ -879,7 +913,8 @@ static hipError_t launch_walk_cfl_t(hipStream_t st, const Fld<const R> &q, const
   const long nbl = (long)nchunks * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   hipLaunchKernelGGL((av_walk_cfl_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st,
-                     q, r, cc_in, dd, sh, chunk, nchunks, nbx, (int)nbl, sc, dz_arr, cf);
+                     q, r, cc_in, dd, sh, chunk, nchunks, nbx, (int)nbl, sc, dz_arr, cf,
+                     DIR == 2 ? cf.dz : cf.dy);
   return hipGetLastError();
 }
 

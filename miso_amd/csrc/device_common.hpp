@@ -44,6 +44,20 @@ __device__ __forceinline__ float dcopysign(float x, float y) { return __builtin_
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// x in a register of its own: the register a load wrote is free from here
+// on, so a loop can load the next value into it without keeping the old one
+// alive (a register copy at the loop's end would have to wait for the load)
+__device__ __forceinline__ double reg_copy(double x) {
+  double y;
+  asm volatile("v_mov_b64 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+__device__ __forceinline__ float reg_copy(float x) {
+  float y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+  return y;
+}
+
 // Whole-wave shifts by one lane as DPP moves (VALU, no LDS round trip):
 // wave_shr:1 (0x138) gives lane i the value of lane i-1, wave_shl:1 (0x130)
 // that of lane i+1; the end lanes, which no caller stores, get 0.
