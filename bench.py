@@ -870,7 +870,11 @@ def main():
     rk = kernels.get("rk_stage")
     rk_bytes = ncell * es * sum(RK_VALUES) * args.steps  # 4 launches per step
     rk_achieved = rk_bytes / (ms[0] * 1e-3) if rk else None
-    for name, vals in (("av_x", 18 + 1), ("av_y", 18 + 1), ("av_z", 18 + 1)):
+    if tiles and tiles.get("av_fuse_xy", 0) >= 3 and "av_x" in kernels and "av_z" not in kernels:
+        # the z walk overlapped with the x-y walk: one record for both
+        kernels["av_xy_and_z_overlapped"] = kernels.pop("av_x")
+    for name, vals in (("av_x", 18 + 1), ("av_y", 18 + 1), ("av_z", 18 + 1),
+                       ("av_xy_and_z_overlapped", 2 * (18 + 1))):
         if name in kernels:
             kernels[name]["achieved_GBps"] = ncell * es * vals / (kernels[name]["avg_ms"] * 1e-3) / 1e9
     if rk:

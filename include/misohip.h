@@ -436,8 +436,12 @@ typedef struct misohip_tiles {
                      x-sweeps every row it takes in (the x sweep's ghost fill
                      drops out), FP64 code for three waves per SIMD; 2 = the
                      same walk compiled for two waves per SIMD (no registers
-                     parked in scratch); 0 = separate x and y sweeps.  Same
-                     results. */
+                     parked in scratch); 3, 4 = 1, 2 with the z walk
+                     overlapped with the x-y walk where z is periodic (the z
+                     walk's chunks in up to eight slabs, each on a second
+                     stream of the context once the x-y walk has produced
+                     the planes it reads); 0 = separate x and y sweeps.
+                     Same results. */
   int av_xy_chunk; /* rows walked per wave by that walk, 4..4096 */
 } misohip_tiles;
 int misohip_set_tiles(misohip_ctx *ctx, const misohip_tiles *t);

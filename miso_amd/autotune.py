@@ -115,8 +115,8 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
     import torch
     cache = cache or TileCache()
     g = model.grid
-    # |v6: entries tuned before the 12-row z-march existed are not reused
-    key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}|v6"
+    # |v7: entries tuned before the overlapped AV walks existed are not reused
+    key = f"{_device_key()}|single|{g.i_size}x{g.j_size}x{g.k_size}|{g.dtype.name}|v7"
     hit = None if force else cache.get(key)
     if hit:
         set_tiles(ctx, *hit["rk_pencil"], hit["sweep_chunk"], hit.get("rk_variant", 0),
@@ -179,9 +179,15 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
             t = _profile(ctx, run, totals=True)
             return sum(t.get(k, 0.0) for k in kinds)
 
-        sweep_times.update(_timed_rounds(
-            [(fz, cz, xc) for fz in ((1, 2) if g.dtype == np.float64 else (1,))
-             for xc in chunks], xy_time))
+        cands = [(fz, cz, xc) for fz in ((1, 2) if g.dtype == np.float64 else (1,))
+                 for xc in chunks]
+        if bc.periodic[2] == 1:
+            # 3 and 4: 1 and 2 with the z walk overlapped with the x-y walk
+            # (a periodic z axis), over z walk lengths that give it a few
+            # slabs; the x-y walk lengths around the usual choice
+            cands += [(fz, c, xc) for fz in ((3, 4) if g.dtype == np.float64 else (3,))
+                      for c in sorted({cz, 32, 64, 128}) for xc in (32, 64, 128) if xc in chunks]
+        sweep_times.update(_timed_rounds(cands, xy_time))
     fz, best_c, xc = min(sweep_times, key=sweep_times.get)
     xc = xc or 64
     # the three fastest RK candidates again, over whole steps with the chosen
@@ -205,7 +211,8 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
                         "step_top3": {(f"z{z}v{v}" if v in (4, 5, 6) else f"{a}x{b}v{v}"): t
                                       for (a, b, v, z), t in step_times.items()},
                         "av_and_ghost_per_run": {
-                            (f"xy{x}z{c}" + ("w2" if f == 2 else "") if f else f"sep{c}"): v
+                            (f"xy{x}z{c}" + ("w2" if f in (2, 4) else "") +
+                             ("o" if f >= 3 else "") if f else f"sep{c}"): v
                             for (f, c, x), v in sweep_times.items()}}}
     cache.put(key, out)
     return dict(out, cached=False, seconds=time.perf_counter() - t0)

@@ -324,7 +324,8 @@ __device__ __forceinline__ void av_walk_block(const Fld<const R> &qq, const Fld<
                                               const AvScalars &sc, const R dt, int V,
                                               R dz0 = R(0), const CflFuse<R> *cf = nullptr,
                                               const R *__restrict__ hw = nullptr) {
-  const int ch = V % nchunks, r = V / nchunks;
+  const int nchl = sc.nparts ? sc.nparts : nchunks;  // chunks in this launch
+  const int ch = sc.part0 + V % nchl, r = V / nchl;
   const int bx = r % nbx, bo = r / nbx;
   const int wx = walk_waves_x(DIR == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj);
   const int wave = threadIdx.x >> 6;
@@ -716,7 +717,9 @@ __global__ __launch_bounds__(256) __attribute__((
                                                         int nseg, int nwaves, AvScalars sc) {
   const int w = xcd_virtual_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (w >= nwaves) return;  // whole wave; the kernel has no block barrier
-  const int seg = w % nseg, rr = w / nseg, ch = rr % nchunks, k = sh.mk - 1 + rr / nchunks;
+  const int seg = w % nseg, rr = w / nseg, ch = rr % nchunks;
+  int k = (sc.nparts ? sc.part0 : sh.mk - 1) + rr / nchunks;
+  if (k > sh.Kt - sh.mk) k -= sh.Kt - 2 * sh.mk + 2;  // past the top ring plane: the bottom one on
   const int lane = threadIdx.x & 63;
   const int i = sh.mi + seg * 60 - 2 + lane;
   // a lane on a ghost column of a wrapped x axis reads its periodic source
@@ -848,7 +851,9 @@ static hipError_t launch_walk_t(hipStream_t st, const Fld<const R> &q, const Fld
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
   int chunk, nchunks;  // even walk lengths
   walk_split(n_dir, 256L * nbx * nbo, sc.chunk, chunk, nchunks);
-  const long nbl = (long)nchunks * nbx * nbo;
+  if (sc.nparts && (sc.part0 < 0 || sc.nparts < 0 || sc.part0 + sc.nparts > nchunks))
+    return hipErrorInvalidValue;
+  const long nbl = (long)(sc.nparts ? sc.nparts : nchunks) * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   hipLaunchKernelGGL((av_walk_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st, q, r,
                      cc_in, dd, sh, chunk, nchunks, nbx, nbo, (int)nbl, sc, dz_arr);
@@ -910,7 +915,9 @@ static hipError_t launch_walk_cfl_t(hipStream_t st, const Fld<const R> &q, const
   const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
   int chunk, nchunks;  // even walk lengths
   walk_split(n_dir, 256L * nbx * nbo, sc.chunk, chunk, nchunks);
-  const long nbl = (long)nchunks * nbx * nbo;
+  if (sc.nparts && (sc.part0 < 0 || sc.nparts < 0 || sc.part0 + sc.nparts > nchunks))
+    return hipErrorInvalidValue;
+  const long nbl = (long)(sc.nparts ? sc.nparts : nchunks) * nbx * nbo;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   hipLaunchKernelGGL((av_walk_cfl_kernel<R, DIR, DEGEN>), dim3((unsigned)nbl), dim3(256), 0, st,
                      q, r, cc_in, dd, sh, chunk, nchunks, nbx, (int)nbl, sc, dz_arr, cf,
@@ -940,7 +947,11 @@ hipError_t launch_av_sweep_xy(hipStream_t st, const Fld<const R> &q, const Fld<R
   const int nseg = (sh.It - 2 * sh.mi + 59) / 60;
   int chunk, nchunks;  // even walk lengths
   walk_split(nj, 64L * nseg * (sh.Kt - 2 * sh.mk + 2), sc.chunk, chunk, nchunks);
-  const long nw = (long)nseg * nchunks * (sh.Kt - 2 * sh.mk + 2);
+  const int npl = sh.Kt - 2 * sh.mk + 2;  // interior planes and the two z ring planes
+  if (sc.nparts && (sc.nparts < 0 || sc.nparts > npl || sc.part0 < sh.mk - 1 ||
+                    sc.part0 > sh.Kt - sh.mk))
+    return hipErrorInvalidValue;
+  const long nw = (long)nseg * nchunks * (sc.nparts ? sc.nparts : npl);
   const long nbl = (nw + 3) / 4;
   if (nbl > 2147483647L) return hipErrorInvalidValue;
   if (sc.xy_waves == 2)
@@ -1030,6 +1041,15 @@ hipError_t launch_batch_av(hipStream_t st, int dir, int mode, const BatchInst<R>
   if (dir == 2 && mode == 2)
     return launch_batch_av_t<R, 2, 2>(st, inst, dyn, start, n, total, in_set, out_set, sc);
   return hipErrorInvalidValue;
+}
+
+void av_walk_split(const Shape &sh, int dir, int chunk_req, int *chunk, int *nchunks) {
+  const int n_dir = dir == 1 ? sh.Jt - 2 * sh.mj : sh.Kt - 2 * sh.mk;
+  const int n_oth = dir == 1 ? sh.Kt - 2 * sh.mk : sh.Jt - 2 * sh.mj;
+  const int wx = walk_waves_x(n_oth);
+  const int nbx = ((sh.It - 2 * sh.mi + 63) / 64 + wx - 1) / wx;
+  const int nbo = (n_oth + 4 / wx - 1) / (4 / wx);
+  walk_split(n_dir, 256L * nbx * nbo, chunk_req, *chunk, *nchunks);
 }
 
 template hipError_t launch_batch_av<double>(hipStream_t, int, int, const BatchInst<double> *,

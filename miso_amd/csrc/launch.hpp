@@ -32,6 +32,10 @@ struct AvScalars {
   // map to periodic source cells instead of reading them
   int wrap = 0;
   int xy_waves = 3;  // FP64 x-y walk: waves per SIMD it is compiled for (3 or 2)
+  // a launch over part of the grid: the z walk's chunks [part0, part0 +
+  // nparts), the x-y walk's planes part0, part0 + 1, ... (nparts of them; a
+  // range past the top z ring plane continues at the bottom one); 0: all
+  int part0 = 0, nparts = 0;
 };
 
 // Boundary description on the device: axis kind (0 mirror, 1 periodic,
@@ -220,6 +224,8 @@ hipError_t launch_av_last_sweep_cfl(hipStream_t, const Fld<const R> &, const Fld
 template <typename R>
 hipError_t launch_av_sweep_xy(hipStream_t, const Fld<const R> &, const Fld<R> &, R *, const R *,
                               const R *, const Shape &, const BcDev &, const AvScalars &);
+// the chunks a y (dir 1) or z (dir 2) walk of this shape is cut into
+void av_walk_split(const Shape &sh, int dir, int chunk_req, int *chunk, int *nchunks);
 template <typename R>
 hipError_t launch_char_vel(hipStream_t, const Fld<const R> &, R *, int, const AvScalars &);
 template <typename R>

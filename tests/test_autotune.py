@@ -140,7 +140,8 @@ def test_fused_xy_sweeps_compute_the_same_bits(ctx, ybc, dtype):
         for tiles in ((2, 8, 64, 0, 32, 0), (2, 8, 64, 0, 32, 1, 64), (2, 8, 4, 0, 32, 1, 4),
                       (1, 1, 7, 5, 16, 1, 5), (1, 1, 256, 5, 256, 1, 256),
                       (1, 1, 7, 5, 16, 2, 5), (1, 1, 256, 4, 256, 2, 256),
-                      (1, 1, 7, 6, 16, 1, 5), (1, 1, 256, 6, 5, 2, 256)):
+                      (1, 1, 7, 6, 16, 1, 5), (1, 1, 256, 6, 5, 2, 256),
+                      (1, 1, 7, 5, 16, 3, 5), (1, 1, 4, 6, 16, 4, 4)):
             m.qq.data.copy_(start)
             set_tiles(ctx, *tiles)
             dts = m.integrator.run(bc, m.qq, 2)

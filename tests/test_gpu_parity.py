@@ -315,12 +315,14 @@ def test_step_fp64_z_march(ctx, case, variant, zchunk):
     assert_bitwise(out, ref, f"z-march {case} variant {variant} zchunk {zchunk}")
 
 
-@pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
-                                  "stratified_gravity_16"])
-@pytest.mark.parametrize("fuse", [1, 2])
+@pytest.mark.parametrize("case", ["turbulence_24", "turbulence_20x16x30",
+                                  "stratified_box_fixed_z_16x16x64", "stratified_gravity_16"])
+@pytest.mark.parametrize("fuse", [1, 2, 3, 4])
 def test_step_fp64_fused_xy_sweeps(ctx, case, fuse):
-    """The one-walk x and y AV sweeps (three or two waves per SIMD) against
-    the oracle, bit for bit."""
+    """The one-walk x and y AV sweeps (three or two waves per SIMD; 3 and 4:
+    with the z walk overlapped with them in slabs where z is periodic, the
+    4-plane z chunks of these grids one or two to a slab) against the
+    oracle, bit for bit."""
     from miso_amd.autotune import get_tiles, set_tiles
     problem, sizes, n, kw = STEP_CASES[case]
     t0 = get_tiles(ctx)
@@ -337,7 +339,8 @@ def test_step_fp64_fused_xy_sweeps(ctx, case, fuse):
 
 @pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
                                   "stratified_gravity_16"])
-def test_run_fills_every_ghost_after_its_last_step(ctx, case):
+@pytest.mark.parametrize("fuse", [1, 3])
+def test_run_fills_every_ghost_after_its_last_step(ctx, case, fuse):
     """With periodic ghosts read at their sources (z-march RK, one-walk x-y
     sweeps) misohip_run fills the periodic faces after its last step only:
     the whole arrays, ghost cells included, equal those of the same number
@@ -347,7 +350,7 @@ def test_run_fills_every_ghost_after_its_last_step(ctx, case):
     t0 = get_tiles(ctx)
     default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
                t0.av_fuse_xy, t0.av_xy_chunk)
-    set_tiles(ctx, 1, 1, 16, 5, 32, 1, 16)
+    set_tiles(ctx, 1, 1, 16, 5, 32, fuse, 16)
     try:
         run, _, run_dts, _ = _run_both(ctx, problem, sizes, n, use_run=True, ghosts=True, **kw)
         steps, _, step_dts, _ = _run_both(ctx, problem, sizes, n, ghosts=True, **kw)
@@ -398,8 +401,8 @@ def test_step_shock_tube(ctx, axis):
     assert_bitwise(out, ref, f"shock tube axis {axis}")
 
 
-@pytest.mark.parametrize("variant,zchunk", [(5, 16), (6, 7)])
-def test_run_fp32_turbulence_3d(ctx, variant, zchunk):
+@pytest.mark.parametrize("variant,zchunk,fuse", [(5, 16, 1), (6, 7, 1), (6, 7, 3)])
+def test_run_fp32_turbulence_3d(ctx, variant, zchunk, fuse):
     """FP32 3D decaying turbulence (the benchmark's initial-condition family)
     through misohip_run on the FP32 z-march columns the tuner picks (8 rows
     with each cell's products kept beside it, 12 rows), against the FP64
@@ -409,7 +412,7 @@ def test_run_fp32_turbulence_3d(ctx, variant, zchunk):
     t0 = get_tiles(ctx)
     default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
                t0.av_fuse_xy, t0.av_xy_chunk)
-    set_tiles(ctx, 1, 1, 64, variant, zchunk, 1, 64)
+    set_tiles(ctx, 1, 1, 64, variant, zchunk, fuse, 64)
     try:
         out, ref, dts, ref_dts = _run_both(ctx, problems.turbulence, (48, 44, 40), 30,
                                            dtype=np.float32, use_run=True)

@@ -35,13 +35,20 @@ def _with_tiles(ctx, tiles, fn):
         set_tiles(ctx, *saved)
 
 
-@pytest.mark.parametrize("sizes,nsteps", [((256, 256, 256), 5), ((64, 64, 64), 50)])
-def test_turbulence_fp64_headline_tiles(ctx, sizes, nsteps):
+# the same with the z walk overlapped with the x-y walk (av_fuse_xy 4); the
+# z walk cuts 64^3 into 16 chunks of 4 planes, run as eight slabs
+OVERLAP_TILES = (1, 1, 16, 5, 128, 4, 64)
+
+
+@pytest.mark.parametrize("sizes,nsteps,tiles", [((256, 256, 256), 5, HEADLINE_TILES),
+                                                ((64, 64, 64), 50, HEADLINE_TILES),
+                                                ((64, 64, 64), 50, OVERLAP_TILES)])
+def test_turbulence_fp64_headline_tiles(ctx, sizes, nsteps, tiles):
     from oracle import oracle as O
     O.set_threads(min(16, O.get_threads()))
-    print(f"turbulence {sizes} FP64, {nsteps} steps, tiles {HEADLINE_TILES}", flush=True)
+    print(f"turbulence {sizes} FP64, {nsteps} steps, tiles {tiles}", flush=True)
     out, ref, dts, ref_dts = _with_tiles(
-        ctx, HEADLINE_TILES,
+        ctx, tiles,
         lambda: _run_both(ctx, problems.turbulence, sizes, nsteps, use_run=True))
     assert np.array_equal(dts, ref_dts), (dts, ref_dts)
     err = rel_linf(out, ref)
