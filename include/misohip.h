@@ -438,7 +438,7 @@ typedef struct misohip_tiles {
                      same walk compiled for two waves per SIMD (no registers
                      parked in scratch); 3, 4 = 1, 2 with the z walk
                      overlapped with the x-y walk where z is periodic (the z
-                     walk's chunks in up to eight slabs, each on a second
+                     walk's chunks in up to sixteen slabs, each on a second
                      stream of the context once the x-y walk has produced
                      the planes it reads); 0 = separate x and y sweeps.
                      Same results. */

@@ -186,7 +186,7 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
             # (a periodic z axis), over z walk lengths that give it a few
             # slabs; the x-y walk lengths around the usual choice
             cands += [(fz, c, xc) for fz in ((3, 4) if g.dtype == np.float64 else (3,))
-                      for c in sorted({cz, 32, 64, 128}) for xc in (32, 64, 128) if xc in chunks]
+                      for c in sorted({cz, 16, 32, 64, 128}) for xc in (32, 64, 128) if xc in chunks]
         sweep_times.update(_timed_rounds(cands, xy_time))
     fz, best_c, xc = min(sweep_times, key=sweep_times.get)
     xc = xc or 64

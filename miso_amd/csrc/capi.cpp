@@ -14,6 +14,7 @@
 #include <new>
 
 #include "../../include/misohip.h"
+#include "tail_plan.hpp"
 #include "launch.hpp"
 
 
@@ -201,88 +202,87 @@ int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const Bc
 // workspace ping-pong: every stage writes rslt, which becomes the next argm.
 // fuse_cfl: the last AV sweep also reduces the next step's CFL candidates
 // into ctx->d_slot (3D and 2D x-y runs).
-// The AV sweeps of a 3D step with a periodic z axis, the x-y walk and the z
-// walk overlapped: the z walk's chunks are cut into up to AV_SLABS slabs;
-// x-y launch s covers every plane z slab s reads that no earlier x-y launch
-// covered (launch 0 also the two top interior planes and the top ring
-// plane, which z slab 0 reads as its periodic ghosts), and z slab s runs on
-// a second stream once x-y launch s is done, beside x-y launch s+1.  The
-// x-y launches write planes no z slab running beside them reads, and read
+// The AV sweeps of a 3D step with a periodic z axis as a pipeline over z
+// slabs (tail_plan.hpp): the one-walk x-y sweeps in slabs on a second
+// stream of the context, the z walk in slabs on a third, each z slab once
+// the x-y slabs it reads from are done, beside the x-y walk's next slabs.
+// The x-y slabs write planes no z slab running beside them reads and read
 // the post-RK state, which no z slab writes; the z slabs write disjoint
-// planes of qq.  So the results are the bits of the two walks in sequence.
-// AV_NOT_TAKEN: the grid is too short in z for two slabs, or the stream is
-// being captured (the sequential walks run instead).
-constexpr int AV_SLABS = 8;
-static_assert(sizeof(misohip_ctx::av_ev) / sizeof(hipEvent_t) == AV_SLABS + 1, "one event per slab and the join");
-constexpr int AV_NOT_TAKEN = 1;
+// planes of qq.  So every value is what the two walks in sequence produce:
+// the same bits.
+// TAIL_NOT_TAKEN: no plan for this shape, or the stream is being captured;
+// the sequential walks run instead.
+constexpr int TAIL_NOT_TAKEN = 1;
+static_assert(sizeof(misohip_ctx::tail_ev) / sizeof(hipEvent_t) == TAIL_SLABS + 3,
+              "one event per slab, the fork and two joins");
 
 template <typename R>
-int av_overlap_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
-                 const BcDev &b, const Shape &sh, R dt, const misohip_fields *cur,
-                 const misohip_fields *dst, const misohip_fields *qq, const misohip_work *w,
-                 int wrap, bool fuse_cfl, const double *heat_q) {
+int tail_overlap_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
+                   const BcDev &b, const Shape &sh, R dt, const misohip_fields *cur,
+                   const misohip_fields *dst, const misohip_fields *qq, const misohip_work *w,
+                   int wrap, bool fuse_cfl, const double *heat_q) {
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   MISOHIP_TRY(hipStreamIsCapturing(ctx->stream, &cs));
-  if (cs != hipStreamCaptureStatusNone) return AV_NOT_TAKEN;
+  if (cs != hipStreamCaptureStatusNone) return TAIL_NOT_TAKEN;
   int zch, znch;
   av_walk_split(sh, 2, ctx->tile.sweep_chunk, &zch, &znch);
-  const int ns = std::min(znch, AV_SLABS);
-  const int nk = sh.Kt - 2 * sh.mk, top0 = sh.Kt - sh.mk - 2;
-  if (ns < 2) return AV_NOT_TAKEN;
-  int c1[AV_SLABS], xy0[AV_SLABS], xyn[AV_SLABS];
-  for (int s = 0; s < ns; ++s) c1[s] = (int)((long)(s + 1) * znch / ns);
-  // x-y launch s: planes [xy0, xy0 + xyn), launch 0 wrapping from the top
-  for (int s = 0; s < ns; ++s) {
-    const int hi = s + 1 < ns ? sh.mk + c1[s] * zch + 2 : top0;
-    xy0[s] = s == 0 ? top0 : sh.mk + c1[s - 1] * zch + 2;
-    xyn[s] = s == 0 ? 3 + hi - (sh.mk - 1) : hi - xy0[s];
-    if (xyn[s] < 1 || (s + 1 < ns && c1[s] * zch + 4 >= nk)) return AV_NOT_TAKEN;
+  TailPlan P;
+  if (!tail_plan(sh.Kt, sh.mk, zch, znch, &P)) return TAIL_NOT_TAKEN;
+  if (!ctx->tail_side[0]) {
+    for (hipStream_t &st : ctx->tail_side)
+      MISOHIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (hipEvent_t &e : ctx->tail_ev)
+      MISOHIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
-  if (!ctx->av_side) {
-    MISOHIP_TRY(hipStreamCreateWithFlags(&ctx->av_side, hipStreamNonBlocking));
-    for (hipEvent_t &e : ctx->av_ev) MISOHIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  }
-  // one record for both walks (the z slabs run on the second stream): from
-  // the first x-y launch to the join
+  hipEvent_t *ev_xy = ctx->tail_ev;
+  hipEvent_t ev_fork = ctx->tail_ev[TAIL_SLABS], ev_jb = ctx->tail_ev[TAIL_SLABS + 1];
+  hipEvent_t ev_jz = ctx->tail_ev[TAIL_SLABS + 2];
+  hipStream_t sb = ctx->tail_side[0], sz = ctx->tail_side[1];
+  // one record for both walks, from the fork to the joins
   Timed t_(ctx, MISOHIP_KIND_AV_X);
+  if (fuse_cfl) MISOHIP_TRY(launch_cfl_init(ctx->stream, ctx->d_slot));
+  MISOHIP_TRY(hipEventRecord(ev_fork, ctx->stream));
+  MISOHIP_TRY(hipStreamWaitEvent(sb, ev_fork, 0));
+  MISOHIP_TRY(hipStreamWaitEvent(sz, ev_fork, 0));
   AvScalars xs = av_scalars<R>(p, dt, ctx->tile.av_xy_chunk);
   xs.wrap = wrap;
   xs.xy_waves = ctx->tile.av_fuse_xy == 4 ? 2 : 3;
+  R *cc = static_cast<R *>(w->cc);
+  for (int s = 0; s < P.ns; ++s) {
+    xs.part0 = P.xy0[s];
+    xs.nparts = P.xyn[s];
+    MISOHIP_TRY(launch_av_sweep_xy<R>(sb, cfld<R>(cur), fld<R>(dst), cc,
+                                      static_cast<const R *>(g->dxi),
+                                      static_cast<const R *>(g->dyi), sh, b, xs));
+    MISOHIP_TRY(hipEventRecord(ev_xy[s], sb));
+  }
   AvScalars zs = av_scalars<R>(p, dt, ctx->tile.sweep_chunk);
   zs.wrap = wrap;
   CflFuse<R> cf;
-  if (fuse_cfl) {
-    cf.dx = static_cast<const R *>(g->dx);
-    cf.dy = static_cast<const R *>(g->dy);
-    cf.dz = static_cast<const R *>(g->dz);
-    cf.gm = (double)(R)p->gm;
-    cf.cfl = (double)(R)p->cfl_number;
-    cf.slot = ctx->d_slot;
-    cf.heat_q = heat_q;
-    cf.heat_dt = (double)dt;
-    MISOHIP_TRY(launch_cfl_init(ctx->stream, ctx->d_slot));
-  }
-  R *cc = static_cast<R *>(w->cc);
-  for (int s = 0; s < ns; ++s) {
-    xs.part0 = xy0[s];
-    xs.nparts = xyn[s];
-    MISOHIP_TRY(launch_av_sweep_xy<R>(ctx->stream, cfld<R>(cur), fld<R>(dst), cc,
-                                      static_cast<const R *>(g->dxi),
-                                      static_cast<const R *>(g->dyi), sh, b, xs));
-    MISOHIP_TRY(hipEventRecord(ctx->av_ev[s], ctx->stream));
-    MISOHIP_TRY(hipStreamWaitEvent(ctx->av_side, ctx->av_ev[s], 0));
-    zs.part0 = s == 0 ? 0 : c1[s - 1];
-    zs.nparts = c1[s] - zs.part0;
+  cf.dx = static_cast<const R *>(g->dx);
+  cf.dy = static_cast<const R *>(g->dy);
+  cf.dz = static_cast<const R *>(g->dz);
+  cf.gm = (double)(R)p->gm;
+  cf.cfl = (double)(R)p->cfl_number;
+  cf.slot = ctx->d_slot;
+  cf.heat_q = heat_q;
+  cf.heat_dt = (double)dt;
+  for (int s = 0; s < P.ns; ++s) {
+    if (P.z_xy[s] >= 0) MISOHIP_TRY(hipStreamWaitEvent(sz, ev_xy[P.z_xy[s]], 0));
+    zs.part0 = s == 0 ? 0 : P.c1[s - 1];
+    zs.nparts = P.c1[s] - zs.part0;
     if (fuse_cfl)
-      MISOHIP_TRY(launch_av_last_sweep_cfl<R>(ctx->av_side, cfld<R>(dst), fld<R>(qq), cc,
+      MISOHIP_TRY(launch_av_last_sweep_cfl<R>(sz, cfld<R>(dst), fld<R>(qq), cc,
                                               static_cast<const R *>(g->dyi),
                                               static_cast<const R *>(g->dzi), sh, zs, cf));
     else
-      MISOHIP_TRY(launch_av_sweep<R>(ctx->av_side, 2, cfld<R>(dst), fld<R>(qq), cc, cc,
+      MISOHIP_TRY(launch_av_sweep<R>(sz, 2, cfld<R>(dst), fld<R>(qq), cc, cc,
                                      static_cast<const R *>(g->dzi), sh, zs));
   }
-  MISOHIP_TRY(hipEventRecord(ctx->av_ev[AV_SLABS], ctx->av_side));
-  MISOHIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->av_ev[AV_SLABS], 0));
+  MISOHIP_TRY(hipEventRecord(ev_jb, sb));
+  MISOHIP_TRY(hipEventRecord(ev_jz, sz));
+  MISOHIP_TRY(hipStreamWaitEvent(ctx->stream, ev_jb, 0));
+  MISOHIP_TRY(hipStreamWaitEvent(ctx->stream, ev_jz, 0));
   return MISOHIP_OK;
 }
 
@@ -339,8 +339,9 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   const misohip_fields *cur = argm;
   const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
   if (fuse_xy && ctx->tile.av_fuse_xy >= 3 && (wrap & 4)) {
-    const int rc = av_overlap_t<R>(ctx, g, p, b, sh, dt, cur, other, qq, w, wrap, fuse_cfl, heat_q);
-    if (rc != AV_NOT_TAKEN) {
+    const int rc = tail_overlap_t<R>(ctx, g, p, b, sh, dt, cur, other, qq, w, wrap, fuse_cfl,
+                                     heat_q);
+    if (rc != TAIL_NOT_TAKEN) {
       if (rc) return rc;
       return ghost_t<R>(ctx, qq, sh, b, fill_periodic ? 7 : 7 & ~wrap);
     }
@@ -593,11 +594,12 @@ int misohip_destroy(misohip_ctx *ctx) {
   }
   if (ctx->rt_fork) (void)hipEventDestroy(ctx->rt_fork);
   if (ctx->rt_join) (void)hipEventDestroy(ctx->rt_join);
-  if (ctx->av_side) {
-    (void)hipStreamSynchronize(ctx->av_side);
-    (void)hipStreamDestroy(ctx->av_side);
-  }
-  for (hipEvent_t e : ctx->av_ev)
+  for (hipStream_t st : ctx->tail_side)
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  for (hipEvent_t e : ctx->tail_ev)
     if (e) (void)hipEventDestroy(e);
   delete ctx;
   return MISOHIP_OK;

@@ -29,10 +29,11 @@ struct misohip_ctx {
   // its multi-plane rays there while the plane rays sweep on `stream`
   hipStream_t rt_side = nullptr;
   hipEvent_t rt_fork = nullptr, rt_join = nullptr;
-  // a second stream (default priority) for the z walk overlapped with the
-  // x-y walk, and its events: one per slab and the join
-  hipStream_t av_side = nullptr;
-  hipEvent_t av_ev[9] = {};
+  // two more streams (default priority) for the AV sweeps run as a pipeline
+  // over z slabs (the x-y walk on the first, the z walk on the second), and
+  // its events: one per slab (16), the fork and the two joins
+  hipStream_t tail_side[2] = {nullptr, nullptr};
+  hipEvent_t tail_ev[19] = {};
   misohip_bc_fn bc_fn = nullptr;  // the user boundary callback and its argument
   void *bc_user = nullptr;
   misohip_src_fn src_fn = nullptr;  // the user source callback, its argument and arrays

@@ -36,7 +36,7 @@ def _with_tiles(ctx, tiles, fn):
 
 
 # the same with the z walk overlapped with the x-y walk (av_fuse_xy 4); the
-# z walk cuts 64^3 into 16 chunks of 4 planes, run as eight slabs
+# z walk cuts 64^3 into 16 chunks of 4 planes, one slab each
 OVERLAP_TILES = (1, 1, 16, 5, 128, 4, 64)
 
 
