@@ -4,10 +4,11 @@ The 3D decaying-turbulence workload is benchmarked at 512^3 with tile shapes
 the autotuner picks there; the parity cases in test_gpu_parity.py use small
 grids, where the z-march shortens its chunks and the walks their lengths.
 Here the same tiles run on a grid large enough that none of them shrinks
-(256^3: 4 x 32 march columns of 64 x 8 cells, two 128-plane chunks each, the
-256-cell z walk and the 64-row x-y walk unsplit), five steps of one
-misohip_run call against the oracle, fields and dts bit for bit; and 64^3
-over 50 steps, where rounding differences would have time to grow.
+(256^3: 4 x 32 march columns of 64 x 8 cells, two 128-plane chunks each; the
+64-row x-y walk and the 32-plane z walk overlapped in eight slabs), five
+steps of one misohip_run call against the oracle, fields and dts bit for
+bit; and 64^3 over 50 steps, where rounding differences would have time to
+grow, with the AV walks in sequence and overlapped.
 """
 import numpy as np
 import pytest
@@ -19,8 +20,10 @@ from tests.test_gpu_parity import _run_both
 pytestmark = pytest.mark.gpu
 
 # (pencil_bx, pencil_by, sweep_chunk, rk_variant, rk_zchunk, av_fuse_xy, av_xy_chunk):
-# the shapes the autotuner chose for the 512^3 FP64 step
-HEADLINE_TILES = (1, 1, 256, 5, 128, 1, 64)
+# the shapes the autotuner chose for the 512^3 FP64 step (the AV walks
+# overlapped), and its choice with the walks in sequence
+HEADLINE_TILES = (1, 1, 32, 5, 256, 4, 64)
+SEQUENTIAL_TILES = (1, 1, 256, 5, 128, 1, 64)
 
 
 def _with_tiles(ctx, tiles, fn):
@@ -35,13 +38,13 @@ def _with_tiles(ctx, tiles, fn):
         set_tiles(ctx, *saved)
 
 
-# the same with the z walk overlapped with the x-y walk (av_fuse_xy 4); the
+# the AV walks overlapped (av_fuse_xy 4) at 64^3: the
 # z walk cuts 64^3 into 16 chunks of 4 planes, one slab each
 OVERLAP_TILES = (1, 1, 16, 5, 128, 4, 64)
 
 
 @pytest.mark.parametrize("sizes,nsteps,tiles", [((256, 256, 256), 5, HEADLINE_TILES),
-                                                ((64, 64, 64), 50, HEADLINE_TILES),
+                                                ((64, 64, 64), 50, SEQUENTIAL_TILES),
                                                 ((64, 64, 64), 50, OVERLAP_TILES)])
 def test_turbulence_fp64_headline_tiles(ctx, sizes, nsteps, tiles):
     from oracle import oracle as O
