@@ -228,6 +228,8 @@ int tail_overlap_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params
   av_walk_split(sh, 2, ctx->tile.sweep_chunk, &zch, &znch);
   TailPlan P;
   if (!tail_plan(sh.Kt, sh.mk, zch, znch, &P)) return TAIL_NOT_TAKEN;
+  // a call that must not allocate uses the streams only once they exist
+  if (!ctx->tail_side[0] && ctx->no_alloc) return TAIL_NOT_TAKEN;
   if (!ctx->tail_side[0]) {
     for (hipStream_t &st : ctx->tail_side)
       MISOHIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -249,11 +251,13 @@ int tail_overlap_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params
   xs.xy_waves = ctx->tile.av_fuse_xy == 4 ? 2 : 3;
   R *cc = static_cast<R *>(w->cc);
   for (int s = 0; s < P.ns; ++s) {
-    xs.part0 = P.xy0[s];
-    xs.nparts = P.xyn[s];
-    MISOHIP_TRY(launch_av_sweep_xy<R>(sb, cfld<R>(cur), fld<R>(dst), cc,
-                                      static_cast<const R *>(g->dxi),
-                                      static_cast<const R *>(g->dyi), sh, b, xs));
+    if (P.xyn[s] > 0) {  // the last slabs of a short grid can be empty
+      xs.part0 = P.xy0[s];
+      xs.nparts = P.xyn[s];
+      MISOHIP_TRY(launch_av_sweep_xy<R>(sb, cfld<R>(cur), fld<R>(dst), cc,
+                                        static_cast<const R *>(g->dxi),
+                                        static_cast<const R *>(g->dyi), sh, b, xs));
+    }
     MISOHIP_TRY(hipEventRecord(ev_xy[s], sb));
   }
   AvScalars zs = av_scalars<R>(p, dt, ctx->tile.sweep_chunk);
@@ -890,10 +894,13 @@ int misohip_update_noalloc(misohip_ctx *ctx, const misohip_grid *g, misohip_dtyp
   if (int rc = noalloc_work(ctx, g, dt, &w)) return rc;
   // profiling records events from a pool that may have to grow; this entry
   // point must not create any, so it runs untimed
+  // (and takes the overlapped AV walks only on streams an earlier call made)
   const bool prof = ctx->profiling;
   ctx->profiling = false;
+  ctx->no_alloc = true;
   const int rc = dt == MISOHIP_F64 ? update_t<double>(ctx, g, p, bc, qq, &w, dt_step)
                                    : update_t<float>(ctx, g, p, bc, qq, &w, (float)dt_step);
+  ctx->no_alloc = false;
   ctx->profiling = prof;
   return rc;
 }

@@ -34,6 +34,9 @@ struct misohip_ctx {
   // its events: one per slab (16), the fork and the two joins
   hipStream_t tail_side[2] = {nullptr, nullptr};
   hipEvent_t tail_ev[19] = {};
+  // set for the length of a call that must not allocate or create streams
+  // or events (misohip_update_noalloc)
+  bool no_alloc = false;
   misohip_bc_fn bc_fn = nullptr;  // the user boundary callback and its argument
   void *bc_user = nullptr;
   misohip_src_fn src_fn = nullptr;  // the user source callback, its argument and arrays

@@ -19,7 +19,8 @@
 //
 // x-y slab s covers the planes z slab s reads that no earlier x-y slab
 // covered; slab 0 also the two top interior planes and the top ring plane,
-// which z slab 0 reads as its periodic ghosts.  Each z slab waits for the
+// which z slab 0 reads as its periodic ghosts (so on a short grid the last
+// slabs can be empty: nothing is left for them).  Each z slab waits for the
 // latest x-y slab it conflicts with; launches of one stream run in order.
 #pragma once
 
@@ -37,7 +38,8 @@ struct TailPlan {
   int z_xy[TAIL_SLABS];            // z slab s waits for x-y slab z_xy[s] (-1: none)
 };
 
-// false: no pipeline for this shape (fewer than two slabs, or slabs too thin)
+// false: no pipeline for this shape (fewer than two slabs, or a grid of
+// fewer than 8 planes)
 inline bool tail_plan(int Kt, int mk, int zch, int znch, TailPlan *pl) {
   const int nk = Kt - 2 * mk;
   const int ns = std::min(znch, TAIL_SLABS);
@@ -45,8 +47,7 @@ inline bool tail_plan(int Kt, int mk, int zch, int znch, TailPlan *pl) {
   TailPlan &P = *pl;
   P.ns = ns;
   for (int s = 0; s < ns; ++s) P.c1[s] = (int)((long)(s + 1) * znch / ns);
-  // the last slab keeps at least one plane below the top two
-  if (P.c1[ns - 2] * zch + 4 >= nk) return false;
+  const int top0 = Kt - mk - 2;  // the first of the two top interior planes
   auto src = [&](int q) { return q < mk ? q + nk : q >= Kt - mk ? q - nk : q; };
   using Mask = std::vector<char>;
   auto hit = [&](const Mask &x, const Mask &y) {
@@ -56,9 +57,10 @@ inline bool tail_plan(int Kt, int mk, int zch, int znch, TailPlan *pl) {
   };
   std::vector<Mask> xy_wd(ns, Mask(Kt, 0)), xy_wc(ns, Mask(Kt, 0));
   for (int s = 0; s < ns; ++s) {
-    const int hi = s + 1 < ns ? mk + P.c1[s] * zch + 2 : Kt - mk - 2;
-    P.xy0[s] = s == 0 ? Kt - mk - 2 : mk + P.c1[s - 1] * zch + 2;
-    P.xyn[s] = s == 0 ? 3 + hi - (mk - 1) : hi - P.xy0[s];
+    const int hi = s + 1 < ns ? std::min(mk + P.c1[s] * zch + 2, top0) : top0;
+    const int lo = s == 0 ? mk - 1 : std::min(mk + P.c1[s - 1] * zch + 2, top0);
+    P.xy0[s] = s == 0 ? top0 : lo;
+    P.xyn[s] = s == 0 ? 3 + hi - lo : std::max(0, hi - lo);
     for (int t = 0; t < P.xyn[s]; ++t) {
       int q = P.xy0[s] + t;
       if (q > Kt - mk) q -= nk + 2;

@@ -337,6 +337,36 @@ def test_step_fp64_fused_xy_sweeps(ctx, case, fuse):
     assert_bitwise(out, ref, f"fused x-y sweeps {case}")
 
 
+@pytest.mark.parametrize("sizes,chunk", [((24, 24, 24), 16), ((20, 16, 30), 16),
+                                         ((48, 44, 40), 64), ((64, 64, 64), 16)])
+def test_overlapped_walks_run_as_a_pipeline(ctx, sizes, chunk):
+    """The grids of the overlapped-walk parity cases above and in
+    test_parity_headline.py do take the slab pipeline (it records one launch
+    for both walks and none for the z walk alone), so those cases check its
+    bits; the same grid with a fixed z axis runs the walks in sequence."""
+    from miso_amd import mhd
+    from miso_amd.autotune import _profile, get_tiles, set_tiles
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
+    g = mhd.Grid(*sizes, 2, device="cuda:0")
+    m = mhd.MHD(mhd.Config({"mhd": {"cfl_number": 0.4}}), g, ctx, 5.0 / 3.0)
+    m.qq.load_ijk(problems.turbulence(g, device="cpu"))
+    fixed = mhd.Fields(g)
+    fixed.data.copy_(m.qq.data)
+    set_tiles(ctx, 1, 1, chunk, 5, 32, 4, 16)
+    try:
+        kinds = {}
+        for name, bc in (("periodic", mhd.BoundaryCondition((1, 1, 1))),
+                         ("fixed z", mhd.BoundaryCondition((1, 1, 2), fixed=fixed))):
+            m.integrator.apply_boundary_condition(bc, m.qq)
+            kinds[name] = _profile(ctx, lambda: m.integrator.step(bc, m.qq), totals=True)
+    finally:
+        set_tiles(ctx, *default)
+    assert "av_x" in kinds["periodic"] and "av_z" not in kinds["periodic"], kinds
+    assert "av_z" in kinds["fixed z"], kinds
+
+
 @pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",
                                   "stratified_gravity_16"])
 @pytest.mark.parametrize("fuse", [1, 3])

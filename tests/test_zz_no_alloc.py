@@ -67,6 +67,39 @@ def test_update_noalloc_equals_update(ctx):
 
 
 @pytest.mark.gpu
+def test_update_noalloc_with_the_av_walks_overlapped():
+    """With the AV walks overlapped (av_fuse_xy 4, a periodic box) on a
+    fresh context: the first no-alloc call takes the sequential walks (it
+    creates no streams), misohip_update then the overlapped ones, and a
+    second no-alloc call the overlapped ones on the streams the update made;
+    all three give misohip_update's bits."""
+    from miso_amd import mhd
+    from miso_amd.autotune import set_tiles
+    c = mhd.ExecContext(0)
+    try:
+        set_tiles(c, 1, 1, 8, 5, 32, 4, 16)  # 8-plane z chunks: four slabs at 32 planes
+        g, integ, bc, q0 = _setup(c, problems.turbulence, (40, 36, 32), np.float64)
+        outs = []
+        for which in ("noalloc", "update", "noalloc"):
+            F = _fields(c, g, q0)
+            integ.apply_boundary_condition(bc, F)
+            dt = 2.5e-3
+            if which == "update":
+                integ.update(dt, bc, F)
+            else:
+                hip.call("misohip_update_noalloc", c.handle, C.byref(g.c_struct()),
+                         integ.dtype_code, C.byref(integ.params), C.byref(bc.c_struct()),
+                         C.byref(F.c_struct()), float(dt))
+            outs.append(F.to_host(c))
+        for n, h in enumerate(outs[1:], 1):
+            same = bool(np.array_equal(h.view(np.uint8), outs[0].view(np.uint8)))
+            assert same, f"call {n} differs from the first"
+        assert not np.array_equal(outs[0], q0), "the update changed nothing"
+    finally:
+        c.close()
+
+
+@pytest.mark.gpu
 def test_update_noalloc_refuses_grids_beyond_the_scratch(ctx):
     from miso_amd import mhd
     g = mhd.Grid(72, 72, 72, 2, dtype=np.float64, device="cuda:0")
