@@ -437,10 +437,12 @@ typedef struct misohip_tiles {
                      drops out), FP64 code for three waves per SIMD; 2 = the
                      same walk compiled for two waves per SIMD (no registers
                      parked in scratch); 3, 4 = 1, 2 with the z walk
-                     overlapped with the x-y walk where z is periodic (the z
-                     walk's chunks in up to sixteen slabs, each on a second
-                     stream of the context once the x-y walk has produced
-                     the planes it reads); 0 = separate x and y sweeps.
+                     overlapped with the x-y walk where z is periodic or
+                     fixed (the z walk's chunks in up to sixteen slabs on
+                     one more stream of the context, the x-y walk's slabs
+                     on another, each z slab once the x-y walk has
+                     produced the planes it reads); 0 = separate x and y
+                     sweeps.
                      Same results. */
   int av_xy_chunk; /* rows walked per wave by that walk, 4..4096 */
 } misohip_tiles;

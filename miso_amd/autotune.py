@@ -181,10 +181,10 @@ def tune_single(ctx, model, bc, steps: int = 2, pencils=RK_PENCILS, chunks=CHUNK
 
         cands = [(fz, cz, xc) for fz in ((1, 2) if g.dtype == np.float64 else (1,))
                  for xc in chunks]
-        if bc.periodic[2] == 1:
+        if bc.periodic[2] in (1, 2):
             # 3 and 4: 1 and 2 with the z walk overlapped with the x-y walk
-            # (a periodic z axis), over z walk lengths that give it a few
-            # slabs; the x-y walk lengths around the usual choice
+            # (a periodic or fixed z axis), over z walk lengths that give it
+            # a few slabs; the x-y walk lengths around the usual choice
             cands += [(fz, c, xc) for fz in ((3, 4) if g.dtype == np.float64 else (3,))
                       for c in sorted({cz, 16, 32, 64, 128}) for xc in (32, 64, 128) if xc in chunks]
         sweep_times.update(_timed_rounds(cands, xy_time))

@@ -202,10 +202,13 @@ int ghost_t(misohip_ctx *ctx, const misohip_fields *q, const Shape &sh, const Bc
 // workspace ping-pong: every stage writes rslt, which becomes the next argm.
 // fuse_cfl: the last AV sweep also reduces the next step's CFL candidates
 // into ctx->d_slot (3D and 2D x-y runs).
-// The AV sweeps of a 3D step with a periodic z axis as a pipeline over z
-// slabs (tail_plan.hpp): the one-walk x-y sweeps in slabs on a second
-// stream of the context, the z walk in slabs on a third, each z slab once
-// the x-y slabs it reads from are done, beside the x-y walk's next slabs.
+// The AV sweeps of a 3D step with a periodic or fixed z axis as a pipeline
+// over z slabs (tail_plan.hpp): the one-walk x-y sweeps in slabs on a
+// second stream of the context, the z walk in slabs on a third, each z slab
+// once the x-y slabs it reads from are done, beside the x-y walk's next
+// slabs.  With a fixed z axis the swept fields' z ghost planes, which the
+// sequential path fills after the x-y walk from the fixed values alone, are
+// filled first.
 // The x-y slabs write planes no z slab running beside them reads and read
 // the post-RK state, which no z slab writes; the z slabs write disjoint
 // planes of qq.  So every value is what the two walks in sequence produce:
@@ -226,8 +229,10 @@ int tail_overlap_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params
   if (cs != hipStreamCaptureStatusNone) return TAIL_NOT_TAKEN;
   int zch, znch;
   av_walk_split(sh, 2, ctx->tile.sweep_chunk, &zch, &znch);
+  const bool pz = wrap & 4;
   TailPlan P;
-  if (!tail_plan(sh.Kt, sh.mk, zch, znch, &P)) return TAIL_NOT_TAKEN;
+  if ((!pz && b.periodic[2] != 2) || !tail_plan(sh.Kt, sh.mk, zch, znch, pz, &P))
+    return TAIL_NOT_TAKEN;
   // a call that must not allocate uses the streams only once they exist
   if (!ctx->tail_side[0] && ctx->no_alloc) return TAIL_NOT_TAKEN;
   if (!ctx->tail_side[0]) {
@@ -240,6 +245,7 @@ int tail_overlap_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params
   hipEvent_t ev_fork = ctx->tail_ev[TAIL_SLABS], ev_jb = ctx->tail_ev[TAIL_SLABS + 1];
   hipEvent_t ev_jz = ctx->tail_ev[TAIL_SLABS + 2];
   hipStream_t sb = ctx->tail_side[0], sz = ctx->tail_side[1];
+  if (!pz) MISOHIP_TRY(launch_ghost_fill<R>(ctx->stream, fld<R>(dst), sh, b, 4));
   // one record for both walks, from the fork to the joins
   Timed t_(ctx, MISOHIP_KIND_AV_X);
   if (fuse_cfl) MISOHIP_TRY(launch_cfl_init(ctx->stream, ctx->d_slot));
@@ -342,7 +348,7 @@ int update_t(misohip_ctx *ctx, const misohip_grid *g, const misohip_params *p,
   const bool fuse_z = g->is && g->js && !g->ks && !ctx->bc_fn;
   const misohip_fields *cur = argm;
   const misohip_fields *other = cur == &w->argm ? &w->rslt : &w->argm;
-  if (fuse_xy && ctx->tile.av_fuse_xy >= 3 && (wrap & 4)) {
+  if (fuse_xy && ctx->tile.av_fuse_xy >= 3 && ((wrap & 4) || bc->periodic[2] == 2)) {
     const int rc = tail_overlap_t<R>(ctx, g, p, b, sh, dt, cur, other, qq, w, wrap, fuse_cfl,
                                      heat_q);
     if (rc != TAIL_NOT_TAKEN) {

@@ -5,8 +5,10 @@
 // against the kernels' plane accesses.
 //
 // Planes are the array's z indices 0 .. Kt-1 with margin mk; nk = Kt - 2 mk
-// interior planes; a ghost plane of the periodic axis is read at its
-// source (q < mk: q + nk; q >= Kt - mk: q - nk).
+// interior planes; a ghost plane of a periodic axis is read at its source
+// (q < mk: q + nk; q >= Kt - mk: q - nk).  With a fixed z boundary the swept
+// fields' ghost planes are filled (from the fixed values) before the
+// pipeline starts, and every plane is read as itself.
 //
 //   x-y slab s (planes xy0[s] .. xy0[s] + xyn[s] - 1; a plane index past
 //     the top z ring plane Kt - mk continues at the bottom ring plane
@@ -18,10 +20,11 @@
 //     its planes.
 //
 // x-y slab s covers the planes z slab s reads that no earlier x-y slab
-// covered; slab 0 also the two top interior planes and the top ring plane,
-// which z slab 0 reads as its periodic ghosts (so on a short grid the last
-// slabs can be empty: nothing is left for them).  Each z slab waits for the
-// latest x-y slab it conflicts with; launches of one stream run in order.
+// covered; with a periodic z axis slab 0 also the two top interior planes
+// and the top ring plane, which z slab 0 reads as its periodic ghosts (so
+// on a short grid the last slabs can be empty: nothing is left for them).
+// Each z slab waits for the latest x-y slab it conflicts with; launches of
+// one stream run in order.
 #pragma once
 
 #include <algorithm>
@@ -40,15 +43,19 @@ struct TailPlan {
 
 // false: no pipeline for this shape (fewer than two slabs, or a grid of
 // fewer than 8 planes)
-inline bool tail_plan(int Kt, int mk, int zch, int znch, TailPlan *pl) {
+inline bool tail_plan(int Kt, int mk, int zch, int znch, bool periodic_z, TailPlan *pl) {
   const int nk = Kt - 2 * mk;
   const int ns = std::min(znch, TAIL_SLABS);
   if (nk < 8 || zch < 1 || ns < 2) return false;
   TailPlan &P = *pl;
   P.ns = ns;
   for (int s = 0; s < ns; ++s) P.c1[s] = (int)((long)(s + 1) * znch / ns);
-  const int top0 = Kt - mk - 2;  // the first of the two top interior planes
-  auto src = [&](int q) { return q < mk ? q + nk : q >= Kt - mk ? q - nk : q; };
+  // periodic: the first of the two top interior planes, which slab 0 takes;
+  // fixed: past the top ring plane
+  const int top0 = periodic_z ? Kt - mk - 2 : Kt - mk + 1;
+  auto src = [&](int q) {
+    return !periodic_z ? q : q < mk ? q + nk : q >= Kt - mk ? q - nk : q;
+  };
   using Mask = std::vector<char>;
   auto hit = [&](const Mask &x, const Mask &y) {
     for (int q = 0; q < Kt; ++q)
@@ -59,8 +66,8 @@ inline bool tail_plan(int Kt, int mk, int zch, int znch, TailPlan *pl) {
   for (int s = 0; s < ns; ++s) {
     const int hi = s + 1 < ns ? std::min(mk + P.c1[s] * zch + 2, top0) : top0;
     const int lo = s == 0 ? mk - 1 : std::min(mk + P.c1[s - 1] * zch + 2, top0);
-    P.xy0[s] = s == 0 ? top0 : lo;
-    P.xyn[s] = s == 0 ? 3 + hi - lo : std::max(0, hi - lo);
+    P.xy0[s] = s == 0 && periodic_z ? top0 : lo;
+    P.xyn[s] = s == 0 && periodic_z ? 3 + hi - lo : std::max(0, hi - lo);
     for (int t = 0; t < P.xyn[s]; ++t) {
       int q = P.xy0[s] + t;
       if (q > Kt - mk) q -= nk + 2;

@@ -1,5 +1,6 @@
 // CPU check of the AV slab pipeline's schedule (miso_amd/csrc/tail_plan.hpp):
-// for many grid heights and chunkings, every pair of launches on the two
+// for many grid heights and chunkings, with a periodic and with a fixed z
+// axis, every pair of launches on the two
 // streams that touch the same plane of the same buffer, one of them
 // writing, is ordered by the plan's waits and the streams' own order, x-y
 // slab before z slab (the order of the sequential walks); and the slabs
@@ -22,12 +23,14 @@ struct Launch {
   std::vector<std::vector<char>> rd, wr;  // [buffer][plane]
 };
 
-int check(int Kt, int mk, int zch, int znch, char *msg, int n) {
+int check(int Kt, int mk, int zch, int znch, bool pz, char *msg, int n) {
   using misohip::TailPlan;
   TailPlan P;
-  if (!misohip::tail_plan(Kt, mk, zch, znch, &P)) return 0;  // no pipeline: nothing to check
+  if (!misohip::tail_plan(Kt, mk, zch, znch, pz, &P)) return 0;  // no pipeline: nothing to check
   const int nk = Kt - 2 * mk;
-  auto src = [&](int q) { return q < mk ? q + nk : q >= Kt - mk ? q - nk : q; };
+  // a ghost plane: its periodic source; with a fixed z boundary itself (its
+  // swept fields filled before the pipeline, by no launch of it)
+  auto src = [&](int q) { return !pz ? q : q < mk ? q + nk : q >= Kt - mk ? q - nk : q; };
   auto blank = [&]() {
     Launch L;
     L.rd.assign(NBUF, std::vector<char>(Kt, 0));
@@ -89,9 +92,10 @@ int check(int Kt, int mk, int zch, int znch, char *msg, int n) {
                                 (launches[i].rd[b][q] && launches[j].wr[b][q]);
           if (conflict && !hb[i][j])
             return std::snprintf(msg, n,
-                                 "Kt %d zch %d znch %d: launch %d (pass %d) and %d (pass %d) "
-                                 "share buffer %d plane %d unordered",
-                                 Kt, zch, znch, i, launches[i].pass, j, launches[j].pass, b, q),
+                                 "Kt %d zch %d znch %d periodic %d: launch %d (pass %d) and %d "
+                                 "(pass %d) share buffer %d plane %d unordered",
+                                 Kt, zch, znch, (int)pz, i, launches[i].pass, j,
+                                 launches[j].pass, b, q),
                    1;
         }
     }
@@ -107,8 +111,10 @@ int check(int Kt, int mk, int zch, int znch, char *msg, int n) {
         const bool interior = q >= mk && q < Kt - mk, ring = q == mk - 1 || q == Kt - mk;
         const int expect = interior || (b == CC && ring) ? 1 : 0;
         if (w != expect)
-          return std::snprintf(msg, n, "Kt %d zch %d znch %d: buffer %d plane %d written %d times",
-                               Kt, zch, znch, b, q, w),
+          return std::snprintf(msg, n,
+                               "Kt %d zch %d znch %d periodic %d: buffer %d plane %d written %d "
+                               "times",
+                               Kt, zch, znch, (int)pz, b, q, w),
                  1;
       }
     }
@@ -123,12 +129,13 @@ extern "C" int tail_plan_check(char *msg, int n, int *plans) {
   const int mk = 2;
   *plans = 0;
   for (int nk = 8; nk <= 1100; nk += (nk < 100 ? 1 : 29)) {
-    for (int zreq : {4, 5, 7, 8, 16, 32, 64, 128, 256}) {
-      const int znch = (nk + zreq - 1) / zreq, zch = (nk + znch - 1) / znch;
-      misohip::TailPlan P;
-      if (misohip::tail_plan(nk + 2 * mk, mk, zch, znch, &P)) ++*plans;
-      if (check(nk + 2 * mk, mk, zch, znch, msg, n)) return 1;
-    }
+    for (int zreq : {4, 5, 7, 8, 16, 32, 64, 128, 256})
+      for (bool pz : {true, false}) {
+        const int znch = (nk + zreq - 1) / zreq, zch = (nk + znch - 1) / znch;
+        misohip::TailPlan P;
+        if (misohip::tail_plan(nk + 2 * mk, mk, zch, znch, pz, &P)) ++*plans;
+        if (check(nk + 2 * mk, mk, zch, znch, pz, msg, n)) return 1;
+      }
   }
   msg[0] = 0;
   return 0;

@@ -343,7 +343,8 @@ def test_overlapped_walks_run_as_a_pipeline(ctx, sizes, chunk):
     """The grids of the overlapped-walk parity cases above and in
     test_parity_headline.py do take the slab pipeline (it records one launch
     for both walks and none for the z walk alone), so those cases check its
-    bits; the same grid with a fixed z axis runs the walks in sequence."""
+    bits, with a periodic and with a fixed z axis; the same grid with a
+    mirrored z axis runs the walks in sequence."""
     from miso_amd import mhd
     from miso_amd.autotune import _profile, get_tiles, set_tiles
     t0 = get_tiles(ctx)
@@ -358,13 +359,15 @@ def test_overlapped_walks_run_as_a_pipeline(ctx, sizes, chunk):
     try:
         kinds = {}
         for name, bc in (("periodic", mhd.BoundaryCondition((1, 1, 1))),
-                         ("fixed z", mhd.BoundaryCondition((1, 1, 2), fixed=fixed))):
+                         ("fixed z", mhd.BoundaryCondition((1, 1, 2), fixed=fixed)),
+                         ("mirror z", mhd.BoundaryCondition((1, 1, 0)))):
             m.integrator.apply_boundary_condition(bc, m.qq)
             kinds[name] = _profile(ctx, lambda: m.integrator.step(bc, m.qq), totals=True)
     finally:
         set_tiles(ctx, *default)
-    assert "av_x" in kinds["periodic"] and "av_z" not in kinds["periodic"], kinds
-    assert "av_z" in kinds["fixed z"], kinds
+    for name in ("periodic", "fixed z"):
+        assert "av_x" in kinds[name] and "av_z" not in kinds[name], (name, kinds)
+    assert "av_z" in kinds["mirror z"], kinds
 
 
 @pytest.mark.parametrize("case", ["turbulence_24", "stratified_box_fixed_z_16x16x64",

@@ -100,18 +100,29 @@ def test_rt_source_and_heat_match_numpy(ctx):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("sizes", [(8, 64), (12, 40)])
-def test_fused_heating_step_matches_the_separate_passes(ctx, sizes):
+@pytest.mark.parametrize("overlap", [False, True])
+def test_fused_heating_step_matches_the_separate_passes(ctx, sizes, overlap):
     """misohip_update_heated (the heating and the next CFL in the last AV
     sweep) against update + RT.heat + ghost fill + cfl on the plain-RT
     stepping of the stratified box: the same dts and fields, bit for bit,
-    over six steps."""
+    over six steps; overlap: the fused steps with the AV walks overlapped in
+    slabs (the box's z axis is fixed)."""
     import torch
+    from miso_amd.autotune import get_tiles, set_tiles
+    t0 = get_tiles(ctx)
+    default = (t0.rk_pencil_bx, t0.rk_pencil_by, t0.sweep_chunk, t0.rk_variant, t0.rk_zchunk,
+               t0.av_fuse_xy, t0.av_xy_chunk)
     n_xy, nz = sizes
     states = []
     for fused in (False, True):
         box = _box(ctx, n_xy, nz, seed=9)
         box.fused_heating = fused
-        dts = [box.step_plain() for _ in range(6)]
+        if fused and overlap:
+            set_tiles(ctx, 1, 1, 8, 5, 32, 4, 16)
+        try:
+            dts = [box.step_plain() for _ in range(6)]
+        finally:
+            set_tiles(ctx, *default)
         torch.cuda.synchronize()
         states.append((dts, box.model.qq.data.clone()))
     assert states[0][0] == states[1][0]
