@@ -189,15 +189,20 @@ struct Shape {
   }
 };
 
-// Generalised minmod slope, limiter.hpp:10-22.
+// Generalised minmod slope, limiter.hpp:10-22.  The two terms of the final
+// sum, min(0, max) and max(0, min) with min <= max, are float-narrowed and
+// at least one of them is a zero, so their sum is exact in float: it is
+// formed there and widened once (the same value, zeros' signs included, as
+// the sum of the two widened terms).
 template <typename R>
 __device__ __forceinline__ R dqq_eval(R qq_dw, R qq_cn, R qq_up, R ep) {
   const R dqq_dw = qq_cn - qq_dw;
   const R dqq_up = qq_up - qq_cn;
   const R dqq_cn = (qq_up - qq_dw) * R(0.5);
-  const R dqq_max = nmax3(ep * dqq_dw, ep * dqq_up, dqq_cn);
-  const R dqq_min = nmin3(ep * dqq_dw, ep * dqq_up, dqq_cn);
-  return nmin2(R(0.0), dqq_max) + nmax2(R(0.0), dqq_min);
+  const float a = static_cast<float>(ep * dqq_dw), b = static_cast<float>(ep * dqq_up),
+              c = static_cast<float>(dqq_cn);
+  const float dqq_max = fmaxf(fmaxf(a, b), c), dqq_min = fminf(fminf(a, b), c);
+  return static_cast<R>(fminf(0.0f, dqq_max) + fmaxf(0.0f, dqq_min));
 }
 
 // Limited interface flux, limiter.hpp:24-37.
